@@ -1,0 +1,147 @@
+/*
+ * C ABI of the MI355X particle FMM (libpvfmm.so built from pvfmm_amd/csrc).
+ *
+ * The PVFMM* entry points are the particle subset of the reference's C interface
+ * (include/pvfmm.h:234-303 of dmalhotra/pvfmm, implemented there in
+ * src/pvfmm-wrapper.cpp:365-933) with the same names, argument meaning and ownership:
+ * the caller owns every array, the context owns the tree, operators and device buffers.
+ * The reference functions return void and abort on errors; these keep the void signatures
+ * and record the failure instead: PVFMMGetLastError() / PVFMMErrorString() report it and
+ * no entry point calls exit().
+ *
+ * The run is one GPU taking the MPI_COMM_SELF share of the work, so the communicator
+ * argument is accepted for signature compatibility and ignored.  It is declared as a
+ * pointer-sized integer so that both MPICH (int) and Open MPI (pointer) handles pass
+ * through the same register.
+ *
+ * The pvfmm_amd_* entry points are the library's own tree-level interface (used by the C++
+ * header include/pvfmm.hpp and by the Python package); each cites the reference call it
+ * stands for.
+ */
+#ifndef PVFMM_AMD_PVFMM_H
+#define PVFMM_AMD_PVFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef intptr_t pvfmm_comm_t;
+
+/* include/pvfmm.h:21-28.  LaplacePotential, LaplaceGradient and StokesVelocity are
+ * evaluated; the others are rejected with PVFMM_ERR_UNSUPPORTED. */
+enum PVFMMKernel {
+  PVFMMLaplacePotential = 0,
+  PVFMMLaplaceGradient = 1,
+  PVFMMStokesPressure = 2,
+  PVFMMStokesVelocity = 3,
+  PVFMMStokesVelocityGrad = 4,
+  PVFMMBiotSavartPotential = 5
+};
+
+/* Error codes of PVFMMGetLastError() (no counterpart in the reference, which aborts). */
+enum PVFMMError {
+  PVFMM_OK = 0,
+  PVFMM_ERR_INVALID_ARGUMENT = 1,
+  PVFMM_ERR_UNSUPPORTED = 2,
+  PVFMM_ERR_DEVICE = 3,
+  PVFMM_ERR_OUT_OF_MEMORY = 4,
+  PVFMM_ERR_STATE = 5,
+  PVFMM_ERR_INTERNAL = 6
+};
+
+/* ---------------------------------------------------------------- reference particle C ABI */
+
+/* include/pvfmm.h:254 -- double-precision context; box_size <= 0 selects free space with an
+ * automatic bounding box (src/pvfmm-wrapper.cpp:457-514).  Periodic boxes are unsupported. */
+void* PVFMMCreateContextD(double box_size, int n, int m, enum PVFMMKernel kernel, pvfmm_comm_t comm);
+/* include/pvfmm.h:255 -- single-precision context (FP32 inputs/outputs, FP32 evaluation). */
+void* PVFMMCreateContextF(float box_size, int n, int m, enum PVFMMKernel kernel, pvfmm_comm_t comm);
+/* include/pvfmm.h:256-257 -- the same without a communicator (MPI_COMM_WORLD there). */
+void* PVFMMCreateContextDWorld(double box_size, int n, int m, enum PVFMMKernel kernel);
+void* PVFMMCreateContextFWorld(float box_size, int n, int m, enum PVFMMKernel kernel);
+
+/* include/pvfmm.h:289-290 -- evaluate.  src_pos/trg_pos: AoS [3*n]; sl_den: [k0*n_src];
+ * dl_den must be NULL (double-layer sources are not supported); trg_val: caller-allocated
+ * [k1*n_trg].  setup != 0 rebuilds the tree for new positions, setup == 0 reuses it. */
+void PVFMMEvalD(const double* src_pos, const double* sl_den, const double* dl_den, long n_src, const double* trg_pos,
+                double* trg_val, long n_trg, const void* ctx, int setup);
+void PVFMMEvalF(const float* src_pos, const float* sl_den, const float* dl_den, long n_src, const float* trg_pos,
+                float* trg_val, long n_trg, const void* ctx, int setup);
+
+/* include/pvfmm.h:298-299 -- destroy and set *ctx = NULL (src/pvfmm-wrapper.cpp:813-825). */
+void PVFMMDestroyContextD(void** ctx);
+void PVFMMDestroyContextF(void** ctx);
+
+/* Error reporting; no counterpart in the reference (it aborts).  The state is per thread. */
+int PVFMMGetLastError(void);
+const char* PVFMMErrorString(int code);
+const char* PVFMMGetLastErrorMessage(void);
+
+/* Mixed precision for a double-precision context: FP32 P2P, M2L and up/down GEMMs with
+ * FP64 operator precompute and an FP64 per-target merge.  No counterpart in the reference;
+ * takes effect at the next setup.  Returns an error code. */
+int PVFMMSetMixedPrecision(void* ctx, int enable);
+
+/* ---------------------------------------------------------------- tree-level interface */
+
+/* FMM_Pts<...>::Initialize(m, comm, kernel) (include/fmm_pts.txx:223-331): operator
+ * precompute for a kernel and multipole order.  precision: 0 FP64, 1 mixed FP32/FP64.
+ * pinv_cutoff <= 0 keeps the reference's 4*eps*max_S truncation. */
+void* pvfmm_amd_fmm_create(int kernel, int m, int precision, double pinv_cutoff);
+void pvfmm_amd_fmm_destroy(void* fmm);
+
+/* PtFMM_CreateTree(src_coord, src_value, trg_coord, comm, max_pts, FreeSpace)
+ * (include/pvfmm.txx:215-245): coordinates in [0,1)^3, AoS. */
+void* pvfmm_amd_tree_create(const double* src, int64_t n_src, const double* trg, int64_t n_trg, int max_pts);
+/* Variant with an explicit tree-defining point set (MPI_Node::pt_coord). */
+void* pvfmm_amd_tree_create_pts(const double* src, int64_t n_src, const double* trg, int64_t n_trg, const double* pt,
+                                int64_t n_pt, int max_pts);
+void pvfmm_amd_tree_destroy(void* tree);
+
+/* FMM_Tree::SetupFMM(fmm) (include/fmm_tree.txx:72-183): builds the device plan. */
+int pvfmm_amd_tree_setup(void* tree, void* fmm, int n_vec_max);
+
+/* PtFMM_Evaluate(tree, trg_val, n_trg, &sl_den) (include/pvfmm.txx:256-328) on host arrays:
+ * den [n_vec][k0*n_src] in input order, out [n_vec][k1*n_trg] in input order. */
+int pvfmm_amd_tree_evaluate(void* tree, const double* den, double* out, int n_vec);
+/* Same with device-resident arrays; the work is queued on the tree's stream and this call
+ * returns after it completes unless `async` is nonzero. */
+int pvfmm_amd_tree_evaluate_device(void* tree, const double* d_den, double* d_out, int n_vec, int async);
+/* The HIP stream (hipStream_t) the tree's kernels run on. */
+void* pvfmm_amd_tree_stream(void* tree);
+
+/* Introspection for parity checks. */
+int64_t pvfmm_amd_tree_num_nodes(void* tree);
+/* per node (level-major order): x, y, z (units 2^-30), depth, is_leaf, parent, n_src, n_trg,
+ * pt_cnt0, pt_cnt1 */
+int pvfmm_amd_tree_nodes(void* tree, int64_t* out10);
+/* sorted position -> input index for which = 0 (sources) or 1 (targets) */
+int pvfmm_amd_tree_point_order(void* tree, int which, int64_t* out);
+int pvfmm_amd_list_count(int type);
+/* interaction list of Mat_Type `type` (include/precomp_mat.hpp:21-39): [n_nodes][count], -1 = empty */
+int pvfmm_amd_tree_lists(void* tree, int type, int32_t* out);
+/* per-phase times (ms) of the last evaluation after enabling timing; returns phase count */
+int pvfmm_amd_tree_set_timing(void* tree, int on);
+int pvfmm_amd_tree_phase_times(void* tree, double* ms, int cap);
+const char* pvfmm_amd_phase_name(int phase);
+/* plan statistics: n_nodes, n_leaves, max_depth, s2m_rows, l2t_rows, m2m_rows, l2l_rows, x_rows,
+ * x_items, v_src_slots, v_trg_slots, v_pairs, u_rows, u_items, u_pairs, far_items, far_pairs,
+ * s2m_pairs, u_src_points_streamed, device_bytes */
+int pvfmm_amd_tree_stats(void* tree, int64_t* out20);
+
+/* Operator matrices of an fmm handle (FP64), for parity checks against the oracle:
+ * which 0 uc2ue0 (V S^+), 1 uc2ue1 (U^T), 2 dc2de0, 3 dc2de1, 4 u2u (8 stacked),
+ * 5 d2d (8 side by side), 6 v1 (complex interleaved).
+ * Returns the element count; copies when out != NULL. */
+int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out);
+
+/* Number of GPUs visible; 0 when the HIP runtime finds none. */
+int pvfmm_amd_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PVFMM_AMD_PVFMM_H */

@@ -1,0 +1,411 @@
+// C ABI of libpvfmm: the reference's particle context interface (src/pvfmm-wrapper.cpp:365-933)
+// and the library's tree-level interface.  All state lives in opaque handles; errors are
+// recorded per thread and never abort the caller.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/pvfmm.h"
+#include "common.hpp"
+#include "device_fmm.hpp"
+#include "octree.hpp"
+#include "operators.hpp"
+
+using namespace pvfmm_amd;
+
+namespace {
+
+thread_local int g_err = 0;
+thread_local std::string g_msg;
+
+void set_error(int code, const std::string& msg) {
+  g_err = code;
+  g_msg = msg;
+}
+void clear_error() {
+  g_err = 0;
+  g_msg.clear();
+}
+
+template <typename F>
+int guarded(F&& f) {
+  clear_error();
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    set_error(kErrOutOfMemory, "host allocation failed");
+  } catch (const std::exception& e) {
+    set_error(kErrInternal, e.what());
+  }
+  return g_err;
+}
+
+struct FmmHandle {
+  Operators O;
+  int precision = 0;
+};
+
+struct DeviceArray {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DeviceArray() { if (p) (void)hipFree(p); }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    HIP_CHECK(hipMalloc(&p, b));
+    bytes = b;
+  }
+};
+
+struct TreeHandle {
+  Octree T;
+  std::vector<double> src, trg;
+  FmmHandle* fmm = nullptr;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<DevicePlanBase> plan;
+  DeviceArray d_den, d_out;
+  ~TreeHandle() {
+    plan.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void ensure_stream() {
+    if (!stream) HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  }
+};
+
+TreeHandle* make_tree(const double* src, int64_t n_src, const double* trg, int64_t n_trg, const double* pt, int64_t n_pt,
+                      int max_pts) {
+  if (n_src < 0 || n_trg < 0 || n_pt < 0) throw Error(kErrInvalidArgument, "negative point count");
+  if ((n_src && !src) || (n_trg && !trg) || (n_pt && !pt)) throw Error(kErrInvalidArgument, "null coordinate array");
+  auto h = std::make_unique<TreeHandle>();
+  h->src.assign(src, src + 3 * n_src);
+  h->trg.assign(trg, trg + 3 * n_trg);
+  build_octree(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pt, n_pt, max_pts);
+  return h.release();
+}
+
+void setup_tree(TreeHandle* t, FmmHandle* f, int n_vec_max) {
+  if (!t || !f) throw Error(kErrInvalidArgument, "null handle");
+  t->ensure_stream();
+  t->plan.reset();
+  t->fmm = f;
+  t->plan = make_plan(t->T, f->O, t->src.data(), t->trg.data(), f->precision, n_vec_max, t->stream);
+}
+
+void eval_device(TreeHandle* t, const double* d_den, double* d_out, int n_vec, double den_scale, double out_scale) {
+  if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
+  t->plan->evaluate(d_den, d_out, n_vec, den_scale, out_scale);
+}
+
+void eval_host(TreeHandle* t, const double* den, double* out, int n_vec, double den_scale, double out_scale) {
+  if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
+  const KernelInfo& ki = t->fmm->O.ker;
+  const size_t nd = (size_t)ki.k0 * t->T.src.n * n_vec, no = (size_t)ki.k1 * t->T.trg.n * n_vec;
+  t->d_den.ensure(std::max<size_t>(8, nd * sizeof(double)));
+  t->d_out.ensure(std::max<size_t>(8, no * sizeof(double)));
+  if (nd) HIP_CHECK(hipMemcpyAsync(t->d_den.p, den, nd * sizeof(double), hipMemcpyHostToDevice, t->stream));
+  eval_device(t, (const double*)t->d_den.p, (double*)t->d_out.p, n_vec, den_scale, out_scale);
+  if (no) HIP_CHECK(hipMemcpyAsync(out, t->d_out.p, no * sizeof(double), hipMemcpyDeviceToHost, t->stream));
+  HIP_CHECK(hipStreamSynchronize(t->stream));
+}
+
+// ----------------------------------------------------------------------------- PVFMM context
+struct Context {
+  double box_size = 0;
+  int max_pts = 0, m = 0, kernel = 0;
+  bool single = false;  // created by PVFMMCreateContextF
+  int precision = 0;
+  std::unique_ptr<FmmHandle> fmm;
+  std::vector<double> grid;  // initial uniform points of the context tree (src/pvfmm-wrapper.cpp:416-432)
+  std::unique_ptr<TreeHandle> tree;
+  int64_t n_src = -1, n_trg = -1;
+  double scale_x = 1, shift_x[3] = {0, 0, 0};
+};
+
+template <typename Real>
+Context* create_context(Real box_size, int n, int m, int kernel, bool single) {
+  if (box_size > 0) throw Error(kErrUnsupported, "periodic boxes (box_size > 0) are not supported; pass box_size <= 0");
+  if (n <= 0) throw Error(kErrInvalidArgument, "max points per box must be positive");
+  kernel_info(kernel);  // validates the kernel
+  auto c = std::make_unique<Context>();
+  c->box_size = (double)box_size;
+  c->max_pts = n;
+  c->m = m;
+  c->kernel = kernel;
+  c->single = single;
+  c->precision = single ? 1 : 0;
+  c->fmm = std::make_unique<FmmHandle>();
+  build_operators(c->fmm->O, kernel, m, 0.0, c->precision);
+  c->fmm->precision = c->precision;
+  // src/pvfmm-wrapper.cpp:416-432 at np = 1, in the context's precision
+  size_t NN = (size_t)std::ceil(std::pow((Real)n, (Real)(1.0 / 3.0)));
+  size_t total = NN * NN * NN;
+  c->grid.resize(3 * total);
+  for (size_t i = 0; i < total; i++) {
+    c->grid[3 * i + 0] = (double)((Real)(((i / 1) % NN) + 0.5) / NN);
+    c->grid[3 * i + 1] = (double)((Real)(((i / NN) % NN) + 0.5) / NN);
+    c->grid[3 * i + 2] = (double)((Real)(((i / (NN * NN)) % NN) + 0.5) / NN);
+  }
+  return c.release();
+}
+
+// Bounding-box map to [0,1)^3, src/pvfmm-wrapper.cpp:457-520
+template <typename Real>
+void bounding_box(size_t n, const Real* x, Real* scale_xr, Real* shift_xr) {
+  double mn[3], mx[3];
+  for (int k = 0; k < 3; k++) mn[k] = mx[k] = x[k];
+  for (size_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) {
+      mn[k] = std::min(mn[k], (double)x[3 * i + k]);
+      mx[k] = std::max(mx[k], (double)x[3 * i + k]);
+    }
+  Real eps = std::numeric_limits<Real>::epsilon() * 64;
+  Real s = 1 / (Real)(mx[0] - mn[0] + 2 * eps);
+  for (int k = 0; k < 3; k++) s = std::min(s, (Real)(1.0 / (mx[k] - mn[k] + 2 * eps)));
+  if (s * 0.0 != 0.0) s = 1.0;
+  *scale_xr = s;
+  for (int k = 0; k < 3; k++) shift_xr[k] = (Real)(-mn[k] * s + eps);
+}
+
+template <typename Real>
+void context_eval(Context* c, const Real* src_pos, const Real* sl_den, const Real* dl_den, long n_src, const Real* trg_pos,
+                  Real* trg_val, long n_trg, int setup) {
+  if (!c) throw Error(kErrInvalidArgument, "null context");
+  if (dl_den) throw Error(kErrUnsupported, "double-layer densities are not supported (pass dl_den = NULL)");
+  if (!sl_den && n_src > 0) throw Error(kErrInvalidArgument, "sl_den is NULL");
+  if (n_src <= 0 || n_trg <= 0) throw Error(kErrInvalidArgument, "n_src and n_trg must be positive");
+  if (!src_pos || !trg_pos || !trg_val) throw Error(kErrInvalidArgument, "null array");
+  const KernelInfo ki = kernel_info(c->kernel);
+  if (setup || !c->tree) {
+    Real scale_x, shift_x[3];
+    Real s0, x0[3], s1, x1[3];
+    bounding_box<Real>(n_src, src_pos, &s0, x0);
+    bounding_box<Real>(n_trg, trg_pos, &s1, x1);
+    Real c0[3] = {(Real)(0.5 - x0[0]) / s0, (Real)(0.5 - x0[1]) / s0, (Real)(0.5 - x0[2]) / s0};
+    Real c1[3] = {(Real)(0.5 - x1[0]) / s1, (Real)(0.5 - x1[1]) / s1, (Real)(0.5 - x1[2]) / s1};
+    scale_x = 0;
+    for (int k = 0; k < 3; k++) scale_x = std::max<Real>(scale_x, std::fabs(c0[k] - c1[k]));
+    scale_x = 1 / (scale_x + 1 / s0 + 1 / s1);
+    for (int k = 0; k < 3; k++) shift_x[k] = (Real)0.5 - (c0[k] + c1[k]) * scale_x / 2;
+    auto map = [&](const Real* p, long n, std::vector<double>& out) {
+      out.resize(3 * (size_t)n);
+      for (long i = 0; i < n; i++)
+        for (int j = 0; j < 3; j++) {
+          Real v = p[3 * i + j] * scale_x + shift_x[j];
+          while (v < 0.0) v += 1;
+          while (v >= 1.0) v -= 1;
+          out[3 * i + j] = (double)v;
+        }
+    };
+    std::vector<double> sp, tp;
+    map(src_pos, n_src, sp);
+    map(trg_pos, n_trg, tp);
+    c->tree.reset();
+    c->tree.reset(make_tree(sp.data(), n_src, tp.data(), n_trg, c->grid.data(), (int64_t)(c->grid.size() / 3), c->max_pts));
+    c->fmm->precision = c->precision;
+    setup_tree(c->tree.get(), c->fmm.get(), 1);
+    c->scale_x = (double)scale_x;
+    c->n_src = n_src;
+    c->n_trg = n_trg;
+  } else if (n_src != c->n_src || n_trg != c->n_trg) {
+    throw Error(kErrInvalidArgument, "setup = 0 requires the point counts of the last setup");
+  }
+  // densities * scale_x^src_scal, outputs * scale_x^trg_scal (src/pvfmm-wrapper.cpp:522-541, 805)
+  const double den_scale = std::pow(c->scale_x, 0.0), out_scale = std::pow(c->scale_x, ki.out_scal);
+  std::vector<double> den((size_t)ki.k0 * n_src), out((size_t)ki.k1 * n_trg);
+  for (size_t i = 0; i < den.size(); i++) den[i] = (double)sl_den[i];
+  eval_host(c->tree.get(), den.data(), out.data(), 1, den_scale, out_scale);
+  for (size_t i = 0; i < out.size(); i++) trg_val[i] = (Real)out[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+void* PVFMMCreateContextD(double box_size, int n, int m, enum PVFMMKernel kernel, pvfmm_comm_t) {
+  Context* c = nullptr;
+  guarded([&] { c = create_context<double>(box_size, n, m, (int)kernel, false); });
+  return c;
+}
+void* PVFMMCreateContextF(float box_size, int n, int m, enum PVFMMKernel kernel, pvfmm_comm_t) {
+  Context* c = nullptr;
+  guarded([&] { c = create_context<float>(box_size, n, m, (int)kernel, true); });
+  return c;
+}
+void* PVFMMCreateContextDWorld(double box_size, int n, int m, enum PVFMMKernel kernel) {
+  return PVFMMCreateContextD(box_size, n, m, kernel, 0);
+}
+void* PVFMMCreateContextFWorld(float box_size, int n, int m, enum PVFMMKernel kernel) {
+  return PVFMMCreateContextF(box_size, n, m, kernel, 0);
+}
+void PVFMMEvalD(const double* src_pos, const double* sl_den, const double* dl_den, long n_src, const double* trg_pos,
+                double* trg_val, long n_trg, const void* ctx, int setup) {
+  guarded([&] { context_eval<double>((Context*)ctx, src_pos, sl_den, dl_den, n_src, trg_pos, trg_val, n_trg, setup); });
+}
+void PVFMMEvalF(const float* src_pos, const float* sl_den, const float* dl_den, long n_src, const float* trg_pos,
+                float* trg_val, long n_trg, const void* ctx, int setup) {
+  guarded([&] { context_eval<float>((Context*)ctx, src_pos, sl_den, dl_den, n_src, trg_pos, trg_val, n_trg, setup); });
+}
+void PVFMMDestroyContextD(void** ctx) {
+  if (!ctx || !*ctx) return;
+  delete (Context*)*ctx;
+  *ctx = nullptr;
+}
+void PVFMMDestroyContextF(void** ctx) { PVFMMDestroyContextD(ctx); }
+
+int PVFMMGetLastError(void) { return g_err; }
+const char* PVFMMGetLastErrorMessage(void) { return g_msg.c_str(); }
+const char* PVFMMErrorString(int code) {
+  switch (code) {
+    case PVFMM_OK: return "no error";
+    case PVFMM_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case PVFMM_ERR_UNSUPPORTED: return "unsupported feature";
+    case PVFMM_ERR_DEVICE: return "device (HIP) error";
+    case PVFMM_ERR_OUT_OF_MEMORY: return "out of memory";
+    case PVFMM_ERR_STATE: return "invalid state";
+    case PVFMM_ERR_INTERNAL: return "internal error";
+    default: return "unknown error";
+  }
+}
+int PVFMMSetMixedPrecision(void* ctx, int enable) {
+  return guarded([&] {
+    Context* c = (Context*)ctx;
+    if (!c) throw Error(kErrInvalidArgument, "null context");
+    if (c->single && !enable) throw Error(kErrUnsupported, "single-precision contexts always evaluate in FP32");
+    c->precision = enable ? 1 : 0;
+  });
+}
+
+void* pvfmm_amd_fmm_create(int kernel, int m, int precision, double pinv_cutoff) {
+  FmmHandle* f = nullptr;
+  guarded([&] {
+    if (precision != 0 && precision != 1) throw Error(kErrInvalidArgument, "precision must be 0 or 1");
+    auto h = std::make_unique<FmmHandle>();
+    build_operators(h->O, kernel, m, pinv_cutoff, precision);
+    h->precision = precision;
+    f = h.release();
+  });
+  return f;
+}
+void pvfmm_amd_fmm_destroy(void* fmm) { delete (FmmHandle*)fmm; }
+
+void* pvfmm_amd_tree_create(const double* src, int64_t n_src, const double* trg, int64_t n_trg, int max_pts) {
+  TreeHandle* t = nullptr;
+  guarded([&] { t = make_tree(src, n_src, trg, n_trg, trg, n_trg, max_pts); });
+  return t;
+}
+void* pvfmm_amd_tree_create_pts(const double* src, int64_t n_src, const double* trg, int64_t n_trg, const double* pt,
+                                int64_t n_pt, int max_pts) {
+  TreeHandle* t = nullptr;
+  guarded([&] { t = make_tree(src, n_src, trg, n_trg, pt, n_pt, max_pts); });
+  return t;
+}
+void pvfmm_amd_tree_destroy(void* tree) { delete (TreeHandle*)tree; }
+
+int pvfmm_amd_tree_setup(void* tree, void* fmm, int n_vec_max) {
+  return guarded([&] { setup_tree((TreeHandle*)tree, (FmmHandle*)fmm, n_vec_max); });
+}
+int pvfmm_amd_tree_evaluate(void* tree, const double* den, double* out, int n_vec) {
+  return guarded([&] { eval_host((TreeHandle*)tree, den, out, n_vec, 1.0, 1.0); });
+}
+int pvfmm_amd_tree_evaluate_device(void* tree, const double* d_den, double* d_out, int n_vec, int async) {
+  return guarded([&] {
+    TreeHandle* t = (TreeHandle*)tree;
+    eval_device(t, d_den, d_out, n_vec, 1.0, 1.0);
+    if (!async) HIP_CHECK(hipStreamSynchronize(t->stream));
+  });
+}
+void* pvfmm_amd_tree_stream(void* tree) { return tree ? (void*)((TreeHandle*)tree)->stream : nullptr; }
+
+int64_t pvfmm_amd_tree_num_nodes(void* tree) { return tree ? ((TreeHandle*)tree)->T.n_nodes() : -1; }
+int pvfmm_amd_tree_nodes(void* tree, int64_t* out) {
+  return guarded([&] {
+    const Octree& T = ((TreeHandle*)tree)->T;
+    for (int32_t i = 0; i < T.n_nodes(); i++) {
+      int64_t* o = &out[10 * (size_t)i];
+      o[0] = T.x[i]; o[1] = T.y[i]; o[2] = T.z[i]; o[3] = T.depth[i]; o[4] = T.is_leaf(i); o[5] = T.parent[i];
+      o[6] = T.is_leaf(i) ? T.pt_cnt0(i) : 0; o[7] = T.is_leaf(i) ? T.pt_cnt1(i) : 0; o[8] = T.pt_cnt0(i); o[9] = T.pt_cnt1(i);
+    }
+  });
+}
+int pvfmm_amd_tree_point_order(void* tree, int which, int64_t* out) {
+  return guarded([&] {
+    const Octree& T = ((TreeHandle*)tree)->T;
+    const PointSet& P = which == 0 ? T.src : T.trg;
+    std::copy(P.perm.begin(), P.perm.end(), out);
+  });
+}
+int pvfmm_amd_list_count(int type) {
+  if (type < 0 || type >= kTypeCount) return -1;
+  return (int)rel_table(type).rel.size();
+}
+int pvfmm_amd_tree_lists(void* tree, int type, int32_t* out) {
+  return guarded([&] {
+    const Octree& T = ((TreeHandle*)tree)->T;
+    if (type < 0 || type >= kTypeCount) throw Error(kErrInvalidArgument, "bad list type");
+    const size_t n = (size_t)T.n_nodes() * T.list_count(type);
+    if (T.lists[type].size() == n) std::copy(T.lists[type].begin(), T.lists[type].end(), out);
+    else std::fill(out, out + n, -1);
+  });
+}
+int pvfmm_amd_tree_set_timing(void* tree, int on) {
+  return guarded([&] {
+    TreeHandle* t = (TreeHandle*)tree;
+    if (!t->plan) throw Error(kErrState, "no plan");
+    t->plan->set_timing(on != 0);
+  });
+}
+int pvfmm_amd_tree_phase_times(void* tree, double* ms, int cap) {
+  TreeHandle* t = (TreeHandle*)tree;
+  if (!t || !t->plan) return 0;
+  int n = std::min<int>(cap, (int)t->plan->phase_ms.size());
+  for (int i = 0; i < n; i++) ms[i] = t->plan->phase_ms[i];
+  return (int)t->plan->phase_ms.size();
+}
+const char* pvfmm_amd_phase_name(int phase) { return phase_name(phase); }
+int pvfmm_amd_tree_stats(void* tree, int64_t* o) {
+  return guarded([&] {
+    TreeHandle* t = (TreeHandle*)tree;
+    if (!t->plan) throw Error(kErrState, "no plan");
+    const PlanStats& s = t->plan->stats;
+    int64_t v[20] = {s.n_nodes, s.n_leaves, s.max_depth, s.s2m_rows, s.l2t_rows, s.m2m_rows, s.l2l_rows, s.x_rows, s.x_items,
+                     s.v_src_slots, s.v_trg_slots, s.v_pairs, s.u_rows, s.u_items, s.u_pairs, s.far_items, s.far_pairs,
+                     s.s2m_pairs, s.u_src_points_streamed, s.device_bytes};
+    std::copy(v, v + 20, o);
+  });
+}
+int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out) {
+  FmmHandle* f = (FmmHandle*)fmm;
+  if (!f) return -1;
+  const Operators& O = f->O;
+  const std::vector<double>* v = nullptr;
+  switch (which) {
+    case 0: v = &O.uc2ue0; break;
+    case 1: v = &O.uc2ue1; break;
+    case 2: v = &O.dc2de0; break;
+    case 3: v = &O.dc2de1; break;
+    case 4: v = &O.u2u; break;
+    case 5: v = &O.d2d; break;
+    case 6:
+      if (out) std::memcpy(out, O.v1.data(), sizeof(double) * 2 * O.v1.size());
+      return (int64_t)(2 * O.v1.size());
+    default: return -1;
+  }
+  if (out) std::copy(v->begin(), v->end(), out);
+  return (int64_t)v->size();
+}
+int pvfmm_amd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+}  // extern "C"

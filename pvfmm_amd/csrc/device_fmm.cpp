@@ -1,0 +1,432 @@
+// Builds the per-tree device plan and runs one evaluation (the reference's RunFMM,
+// include/fmm_tree.txx:237-738, with the list setups of include/fmm_pts.txx:2275-5737
+// expressed as flat index arrays).
+#include "device_fmm.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "kernels.hpp"
+
+namespace pvfmm_amd {
+
+void throw_device_error(hipError_t e, const char* file, int line) {
+  throw Error(e == hipErrorOutOfMemory ? kErrOutOfMemory : kErrDevice,
+              std::string("HIP error '") + hipGetErrorString(e) + "' at " + file + ":" + std::to_string(line));
+}
+
+const char* phase_name(int p) {
+  static const char* names[] = {"Gather", "S2U-P2P", "S2U-GEMM", "U2U", "X-List", "V-List:FFT", "V-List:Hadamard",
+                                "V-List:IFFT", "D2D", "D2T-GEMM", "D2T+W-List", "U-List", "Scatter"};
+  return (p >= 0 && p < kPhaseCount) ? names[p] : "?";
+}
+
+namespace {
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  int64_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { if (p) (void)hipFree(p); }
+  void alloc(int64_t count, int64_t* total) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    n = count;
+    if (count > 0) {
+      HIP_CHECK(hipMalloc(&p, sizeof(T) * count));
+      if (total) *total += sizeof(T) * count;
+    }
+  }
+  void upload(const std::vector<T>& v, int64_t* total) {
+    alloc((int64_t)v.size(), total);
+    if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  }
+};
+
+template <typename Real>
+class DevicePlan final : public DevicePlanBase {
+ public:
+  DevicePlan(const Octree& T, const Operators& O, const double* src_xyz, const double* trg_xyz, int n_vec, hipStream_t s)
+      : O_(O), st_(s) {
+    n_vec_max = std::max(1, n_vec);
+    build(T, src_xyz, trg_xyz);
+  }
+
+  hipStream_t stream() const override { return st_; }
+  bool is_fp32() const override { return sizeof(Real) == 4; }
+
+  void evaluate(const double* d_den, double* d_out, int n_vec, double den_scale, double out_scale) override {
+    if (n_vec < 1) throw Error(kErrInvalidArgument, "n_vec must be >= 1");
+    for (int v = 0; v < n_vec; v++)
+      evaluate_one(d_den + (int64_t)v * k0_ * n_src_, d_out + (int64_t)v * k1_ * n_trg_, den_scale, out_scale);
+  }
+
+ private:
+  using K = DeviceKernels<Real>;
+  const Operators& O_;
+  hipStream_t st_;
+  int64_t n_src_ = 0, n_trg_ = 0;
+  int k0_ = 1, k1_ = 1, e0_ = 1, e1_ = 1, K0_ = 0, K1_ = 0, ns_ = 0, m_ = 0, M_dim_ = 0;
+  int n_nodes_ = 0, max_depth_ = 0;
+
+  // tree arrays
+  DBuf<double> center_;
+  DBuf<float> node_size_;
+  DBuf<int32_t> src_begin_, src_end_, trg_begin_, trg_end_;
+  DBuf<int64_t> src_perm_, trg_perm_;
+  DBuf<Real> sx_, tx_, sden_;
+  DBuf<double> sx64_, tx64_;  // FP64 leaf-local coordinates for the U list
+  DBuf<double> tval_;
+  // expansions: eq_ = [up (n_nodes x K0) | L2T equivalent rows (n_l2t x K0)]
+  DBuf<Real> eq_, dn_, chk_;
+  // operators
+  DBuf<Real> uc2ue0_, uc2ue1_, dc2de0_, dc2de1_, u2u_, d2d_, V_, unit_surf_, ctmp_;
+  int ru_ = 0, rd_ = 0;
+  DBuf<int32_t> lattice_;
+  // S2M
+  DBuf<int32_t> s2m_node_, s2m_ptr_, s2m_items_, s2m_orow_;
+  DBuf<Real> s2m_alpha_;
+  int n_s2m_ = 0;
+  // M2M / L2L per level
+  struct LevelRows { int n = 0; DBuf<int32_t> ra, rc; };
+  std::vector<std::unique_ptr<LevelRows>> m2m_, l2l_;
+  // X list
+  DBuf<int32_t> x_node_, x_ptr_, x_items_, x_orow_;
+  int n_x_ = 0;
+  // V list
+  DBuf<int32_t> v_src_fc_, v_trg_fc_, v_partner_, v_offsets_;
+  DBuf<Real> v_src_scal_, v_trg_scal_, X_, Y_;
+  int n_vsrc_ = 0, n_vtrg_ = 0, n_off_ = 0;
+  // L2T
+  DBuf<int32_t> l2t_ra_, l2t_rc_, l2t_seq_;
+  DBuf<Real> l2t_alpha_;
+  int n_l2t_ = 0;
+  // target leaves: far field and U list
+  DBuf<int32_t> t_leaf_, far_ptr_, far_node_, u_ptr_, u_items_;
+  DBuf<int64_t> far_den_;
+  DBuf<double> far_rad_;
+  int n_tleaf_ = 0;
+  std::vector<hipEvent_t> ev_;
+
+ public:
+  ~DevicePlan() override {
+    for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+  }
+
+ private:
+  void mark(int phase) {
+    if (timing_) HIP_CHECK(hipEventRecord(ev_[phase], st_));
+  }
+
+  void build(const Octree& T, const double* src_xyz, const double* trg_xyz) {
+    int64_t total = 0;
+    const KernelInfo& ki = O_.ker;
+    k0_ = ki.k0; k1_ = ki.k1; e0_ = ki.e0; e1_ = ki.e1;
+    ns_ = O_.ns; m_ = O_.m; K0_ = O_.K0(); K1_ = O_.K1(); M_dim_ = O_.M_dim;
+    n_src_ = T.src.n; n_trg_ = T.trg.n;
+    n_nodes_ = T.n_nodes();
+    max_depth_ = T.max_depth;
+    if (n_src_ >= (1ll << 31) || n_trg_ >= (1ll << 31)) throw Error(kErrUnsupported, "more than 2^31 points");
+    const int N = n_nodes_;
+
+    // ---- node arrays
+    std::vector<double> center(3 * (size_t)N);
+    std::vector<float> nsz(N);
+    std::vector<int32_t> sb(N), se(N), tb(N), te(N);
+    for (int i = 0; i < N; i++) {
+      double c[3];
+      T.corner(i, c);
+      double s = T.size(i);
+      for (int d = 0; d < 3; d++) center[3 * i + d] = c[d] + 0.5 * s;
+      nsz[i] = (float)s;
+      sb[i] = (int32_t)T.src_begin[i]; se[i] = (int32_t)T.src_end[i];
+      tb[i] = (int32_t)T.trg_begin[i]; te[i] = (int32_t)T.trg_end[i];
+    }
+    center_.upload(center, &total);
+    node_size_.upload(nsz, &total);
+    src_begin_.upload(sb, &total); src_end_.upload(se, &total);
+    trg_begin_.upload(tb, &total); trg_end_.upload(te, &total);
+    src_perm_.upload(T.src.perm, &total);
+    trg_perm_.upload(T.trg.perm, &total);
+
+    // ---- leaf-local coordinates (x - leaf centre), SoA
+    std::vector<double> sx(3 * (size_t)n_src_), tx(3 * (size_t)n_trg_);
+    for (int i = 0; i < N; i++) {
+      if (!T.is_leaf(i)) continue;
+      for (int64_t k = T.src_begin[i]; k < T.src_end[i]; k++)
+        for (int d = 0; d < 3; d++) sx[d * n_src_ + k] = src_xyz[3 * T.src.perm[k] + d] - center[3 * i + d];
+      for (int64_t k = T.trg_begin[i]; k < T.trg_end[i]; k++)
+        for (int d = 0; d < 3; d++) tx[d * n_trg_ + k] = trg_xyz[3 * T.trg.perm[k] + d] - center[3 * i + d];
+    }
+    sx64_.upload(sx, &total);
+    tx64_.upload(tx, &total);
+    sx_.upload(std::vector<Real>(sx.begin(), sx.end()), &total);
+    tx_.upload(std::vector<Real>(tx.begin(), tx.end()), &total);
+    sden_.alloc((int64_t)k0_ * n_src_, &total);
+    tval_.alloc((int64_t)k1_ * n_trg_, &total);
+
+    // ---- operators in the working precision
+    auto conv = [](const std::vector<double>& v) { return std::vector<Real>(v.begin(), v.end()); };
+    uc2ue0_.upload(conv(O_.uc2ue0), &total);
+    uc2ue1_.upload(conv(O_.uc2ue1), &total);
+    dc2de0_.upload(conv(O_.dc2de0), &total);
+    dc2de1_.upload(conv(O_.dc2de1), &total);
+    ru_ = O_.rank_u;
+    rd_ = O_.rank_d;
+    u2u_.upload(conv(O_.u2u), &total);
+    d2d_.upload(conv(O_.d2d), &total);
+    {
+      std::vector<Real> V(2 * O_.v1.size());
+      for (size_t i = 0; i < O_.v1.size(); i++) { V[2 * i] = (Real)O_.v1[i].real(); V[2 * i + 1] = (Real)O_.v1[i].imag(); }
+      V_.upload(V, &total);
+    }
+    unit_surf_.upload(conv(O_.unit_surf), &total);
+    lattice_.upload(O_.lattice, &total);
+
+    const double ts = ki.trg_scal + ki.src_scal;
+    // ---- S2M rows: leaves holding sources (Source2UpSetup, include/fmm_pts.txx:2287-2290)
+    std::vector<int32_t> s2m_node, s2m_ptr{0}, s2m_items, s2m_orow;
+    std::vector<Real> s2m_alpha;
+    std::vector<int32_t> l2t_ra, l2t_rc;
+    std::vector<Real> l2t_alpha;
+    int64_t s2m_pairs = 0;
+    for (int i = 0; i < N; i++)
+      if (T.is_leaf(i) && T.pt_cnt0(i) > 0) {
+        s2m_orow.push_back((int32_t)s2m_node.size());
+        s2m_node.push_back(i);
+        s2m_items.push_back(i);
+        s2m_ptr.push_back((int32_t)s2m_items.size());
+        s2m_alpha.push_back((Real)std::pow(2.0, -ts * T.depth[i]));
+        s2m_pairs += T.pt_cnt0(i) * ns_;
+      }
+    n_s2m_ = (int)s2m_node.size();
+    s2m_node_.upload(s2m_node, &total); s2m_ptr_.upload(s2m_ptr, &total); s2m_items_.upload(s2m_items, &total);
+    s2m_orow_.upload(s2m_orow, &total); s2m_alpha_.upload(s2m_alpha, &total);
+    chk_.alloc((int64_t)std::max(1, n_s2m_) * K1_, &total);
+
+    // ---- target leaves and L2T rows (Down2TargetSetup, include/fmm_pts.txx:5427-5442)
+    std::vector<int32_t> tleaf, l2t_seq;
+    for (int i = 0; i < N; i++)
+      if (T.is_leaf(i) && T.pt_cnt1(i) > 0) {
+        l2t_seq.push_back((int32_t)l2t_ra.size());
+        l2t_ra.push_back(i);
+        l2t_rc.push_back(N + (int32_t)tleaf.size());  // rows after the up block of eq_
+        l2t_alpha.push_back((Real)std::pow(2.0, -ts * T.depth[i]));
+        tleaf.push_back(i);
+      }
+    n_l2t_ = n_tleaf_ = (int)tleaf.size();
+    l2t_ra_.upload(l2t_ra, &total); l2t_rc_.upload(l2t_rc, &total); l2t_alpha_.upload(l2t_alpha, &total);
+    l2t_seq_.upload(l2t_seq, &total);
+    ctmp_.alloc((int64_t)std::max(1, std::max(n_s2m_, n_l2t_)) * std::max(1, std::max(ru_, rd_)), &total);
+    t_leaf_.upload(tleaf, &total);
+    eq_.alloc((int64_t)(N + n_l2t_) * K0_, &total);
+    dn_.alloc((int64_t)N * K1_, &total);
+
+    // ---- M2M / L2L rows per level
+    m2m_.clear(); l2l_.clear();
+    int64_t m2m_rows = 0, l2l_rows = 0;
+    for (int l = 0; l <= max_depth_; l++) {
+      auto a = std::make_unique<LevelRows>(), b = std::make_unique<LevelRows>();
+      std::vector<int32_t> ra, rc, ra2, rc2;
+      for (int i = T.level_begin[l]; i < T.level_begin[l + 1]; i++) {
+        if (T.is_leaf(i)) continue;
+        if (T.pt_cnt0(i) > 0) { ra.push_back(T.first_child[i]); rc.push_back(i); }  // Up2UpSetup 2618-2619
+        if (T.pt_cnt1(i) > 0) { ra2.push_back(i); rc2.push_back(T.first_child[i]); }  // Down2DownSetup 3777-3778
+      }
+      a->n = (int)ra.size(); a->ra.upload(ra, &total); a->rc.upload(rc, &total);
+      b->n = (int)ra2.size(); b->ra.upload(ra2, &total); b->rc.upload(rc2, &total);
+      m2m_rows += a->n; l2l_rows += b->n;
+      m2m_.push_back(std::move(a)); l2l_.push_back(std::move(b));
+    }
+
+    const int nl_x = T.list_count(kX), nl_w = T.list_count(kW);
+    // ---- X list S2L (X_ListSetup, include/fmm_pts.txx:4444-4743)
+    std::vector<int32_t> x_node, x_ptr{0}, x_items, x_orow;
+    for (int i = 0; i < N; i++) {
+      if (T.pt_cnt1(i) == 0) continue;
+      if (T.is_leaf(i) && T.pt_cnt1(i) <= ns_) continue;  // handled in the U list
+      int cnt = 0;
+      for (int k = 0; k < nl_x; k++) {
+        int32_t s = T.lists[kX][(size_t)i * nl_x + k];
+        if (s >= 0 && T.is_leaf(s) && T.pt_cnt0(s) > 0) { x_items.push_back(s); cnt++; }
+      }
+      if (!cnt) continue;
+      x_node.push_back(i); x_orow.push_back(i);
+      x_ptr.push_back((int32_t)x_items.size());
+    }
+    n_x_ = (int)x_node.size();
+    x_node_.upload(x_node, &total); x_ptr_.upload(x_ptr, &total); x_items_.upload(x_items, &total); x_orow_.upload(x_orow, &total);
+
+    // ---- V list (V_ListSetup, include/fmm_pts.txx:3346-3595): parents holding sources / targets
+    std::vector<int32_t> slot_of(N, -1), v_src_fc, v_trg_fc, v_partner;
+    std::vector<Real> v_src_scal, v_trg_scal;
+    for (int i = 0; i < N; i++)
+      if (!T.is_leaf(i) && T.pt_cnt0(i) > 0) {
+        slot_of[i] = (int32_t)v_src_fc.size();
+        v_src_fc.push_back(T.first_child[i]);
+        v_src_scal.push_back((Real)std::pow(2.0, ki.src_scal * (T.depth[i] + 1)));
+      }
+    std::vector<int32_t> offs(O_.v1_offsets.begin(), O_.v1_offsets.end());
+    n_off_ = (int)offs.size();
+    int64_t v_pairs = 0;
+    const int nl_v1 = T.list_count(kV1);
+    const RelTable& v1t = rel_table(kV1);
+    for (int i = 0; i < N; i++) {
+      if (T.is_leaf(i) || T.pt_cnt1(i) == 0) continue;
+      std::vector<int32_t> row(n_off_, -1);
+      int cnt = 0;
+      for (int k = 0; k < nl_v1; k++) {
+        int32_t s = T.lists[kV1][(size_t)i * nl_v1 + k];
+        if (s < 0 || T.pt_cnt0(s) == 0) continue;
+        const auto& r = v1t.rel[k];
+        int R = (r[0] + 1) + 3 * (r[1] + 1) + 9 * (r[2] + 1);
+        int oi = (int)(std::find(offs.begin(), offs.end(), R) - offs.begin());
+        row[oi] = slot_of[s];
+        cnt++;
+      }
+      if (!cnt) continue;
+      v_pairs += cnt;
+      v_trg_fc.push_back(T.first_child[i]);
+      v_trg_scal.push_back((Real)std::pow(2.0, ki.trg_scal * (T.depth[i] + 1)));
+      v_partner.insert(v_partner.end(), row.begin(), row.end());
+    }
+    n_vsrc_ = (int)v_src_fc.size();
+    n_vtrg_ = (int)v_trg_fc.size();
+    v_src_fc_.upload(v_src_fc, &total); v_trg_fc_.upload(v_trg_fc, &total);
+    v_src_scal_.upload(v_src_scal, &total); v_trg_scal_.upload(v_trg_scal, &total);
+    v_partner_.upload(v_partner, &total); v_offsets_.upload(offs, &total);
+    X_.alloc((int64_t)n_vsrc_ * M_dim_ * 16 * e0_, &total);
+    Y_.alloc((int64_t)n_vtrg_ * M_dim_ * 16 * e1_, &total);
+
+    // ---- far field (L2T own surface + W list M2T) and U list per target leaf
+    std::vector<int32_t> far_ptr{0}, far_node, u_ptr{0}, u_items;
+    std::vector<int64_t> far_den;
+    std::vector<double> far_rad;
+    const int nl_u0 = T.list_count(kU0), nl_u1 = T.list_count(kU1), nl_u2 = T.list_count(kU2);
+    int64_t u_pairs = 0, far_pairs = 0, u_stream = 0;
+    for (int r = 0; r < n_tleaf_; r++) {
+      const int t = tleaf[r];
+      far_node.push_back(t); far_den.push_back((int64_t)(N + r) * K0_); far_rad.push_back(kRad1);
+      for (int k = 0; k < nl_w; k++) {  // W_ListSetup 4910-4913
+        int32_t s = T.lists[kW][(size_t)t * nl_w + k];
+        if (s < 0 || T.pt_cnt0(s) == 0) continue;
+        if (T.is_leaf(s) && T.pt_cnt0(s) <= ns_) continue;
+        far_node.push_back(s); far_den.push_back((int64_t)s * K0_); far_rad.push_back(kRad0);
+      }
+      far_pairs += (int64_t)(far_node.size() - far_ptr.back()) * ns_ * T.pt_cnt1(t);
+      far_ptr.push_back((int32_t)far_node.size());
+      auto add = [&](int32_t s) {
+        if (s >= 0 && T.is_leaf(s) && T.pt_cnt0(s) > 0) {
+          u_items.push_back(s);
+          u_pairs += T.pt_cnt0(s) * T.pt_cnt1(t);
+          u_stream += T.pt_cnt0(s);
+        }
+      };
+      for (int k = 0; k < nl_u0; k++) add(T.lists[kU0][(size_t)t * nl_u0 + k]);  // U_ListSetup 5197-5321
+      for (int k = 0; k < nl_u1; k++) add(T.lists[kU1][(size_t)t * nl_u1 + k]);
+      for (int k = 0; k < nl_u2; k++) add(T.lists[kU2][(size_t)t * nl_u2 + k]);
+      if (T.pt_cnt1(t) <= ns_)
+        for (int k = 0; k < nl_x; k++) add(T.lists[kX][(size_t)t * nl_x + k]);
+      for (int k = 0; k < nl_w; k++) {
+        int32_t s = T.lists[kW][(size_t)t * nl_w + k];
+        if (s >= 0 && T.is_leaf(s) && T.pt_cnt0(s) <= ns_) add(s);
+      }
+      u_ptr.push_back((int32_t)u_items.size());
+    }
+    far_ptr_.upload(far_ptr, &total); far_node_.upload(far_node, &total); far_den_.upload(far_den, &total);
+    far_rad_.upload(far_rad, &total);
+    u_ptr_.upload(u_ptr, &total); u_items_.upload(u_items, &total);
+
+    ev_.resize(kPhaseCount + 1);
+    for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
+    phase_ms.assign(kPhaseCount, 0.0);
+
+    stats.n_nodes = N;
+    for (int i = 0; i < N; i++) stats.n_leaves += T.is_leaf(i);
+    stats.max_depth = max_depth_;
+    stats.s2m_rows = n_s2m_; stats.l2t_rows = n_l2t_; stats.m2m_rows = m2m_rows; stats.l2l_rows = l2l_rows;
+    stats.x_rows = n_x_; stats.x_items = (int64_t)x_items.size();
+    stats.v_src_slots = n_vsrc_; stats.v_trg_slots = n_vtrg_; stats.v_pairs = v_pairs;
+    stats.u_rows = n_tleaf_; stats.u_items = (int64_t)u_items.size(); stats.u_pairs = u_pairs;
+    stats.u_src_points_streamed = u_stream;
+    stats.far_items = (int64_t)far_node.size(); stats.far_pairs = far_pairs; stats.s2m_pairs = s2m_pairs;
+    stats.device_bytes = total;
+  }
+
+  void evaluate_one(const double* d_den, double* d_out, double den_scale, double out_scale) {
+    const int pk = O_.ker.pt_kind, ek = O_.ker.exp_kind;
+    mark(0);
+    K::gather_density(st_, n_src_, k0_, d_den, src_perm_.p, den_scale, sden_.p, 1, 0);
+    K::fill(st_, eq_.p, eq_.n);
+    K::fill(st_, dn_.p, dn_.n);
+    mark(1);
+    // S2U: sources -> upward check surface (RAD1), then check -> equivalent
+    K::p2p_surface(st_, ek, n_s2m_, s2m_node_.p, s2m_ptr_.p, s2m_items_.p, src_begin_.p, src_end_.p, center_.p, node_size_.p,
+                   kRad1, unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, chk_.p, s2m_orow_.p, K1_, false);
+    mark(2);
+    K::gemm_rows(st_, n_s2m_, ru_, K1_, chk_.p, s2m_orow_.p, K1_, uc2ue0_.p, ctmp_.p, s2m_orow_.p, ru_, s2m_alpha_.p, false);
+    K::gemm_rows(st_, n_s2m_, K0_, ru_, ctmp_.p, s2m_orow_.p, ru_, uc2ue1_.p, eq_.p, s2m_node_.p, K0_, nullptr, false);
+    mark(3);
+    // U2U, bottom-up: parent row = [8 children] x stacked U2U
+    for (int l = max_depth_ - 1; l >= 0; l--) {
+      const LevelRows& L = *m2m_[l];
+      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false);
+    }
+    mark(4);
+    // X list: source points -> downward check surface (RAD0)
+    K::p2p_surface(st_, ek, n_x_, x_node_.p, x_ptr_.p, x_items_.p, src_begin_.p, src_end_.p, center_.p, node_size_.p, kRad0,
+                   unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, dn_.p, x_orow_.p, K1_, true);
+    mark(5);
+    // V list
+    K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p, K0_, e0_, m_, lattice_.p, v_src_scal_.p, X_.p);
+    mark(6);
+    K::hadamard(st_, n_vtrg_, M_dim_, e0_, e1_, v_partner_.p, v_offsets_.p, n_off_, V_.p, X_.p, Y_.p);
+    mark(7);
+    K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p, K1_, e1_, m_, lattice_.p, ns_, v_trg_scal_.p);
+    mark(8);
+    // D2D, top-down: 8 children rows += parent x [D2D_0 .. D2D_7]
+    for (int l = 1; l <= max_depth_; l++) {
+      const LevelRows& L = *l2l_[l - 1];
+      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true);
+    }
+    mark(9);
+    // D2T: check -> equivalent
+    K::gemm_rows(st_, n_l2t_, rd_, K1_, dn_.p, l2t_ra_.p, K1_, dc2de0_.p, ctmp_.p, l2t_seq_.p, rd_, l2t_alpha_.p, false);
+    K::gemm_rows(st_, n_l2t_, K0_, rd_, ctmp_.p, l2t_seq_.p, rd_, dc2de1_.p, eq_.p, l2t_rc_.p, K0_, nullptr, false);
+    mark(10);
+    // equivalent surfaces -> targets (own downward equivalent + W list)
+    K::far_points(st_, pk, n_tleaf_, t_leaf_.p, far_ptr_.p, far_node_.p, far_den_.p, far_rad_.p, trg_begin_.p, trg_end_.p,
+                  center_.p, node_size_.p, unit_surf_.p, ns_, tx_.p, n_trg_, eq_.p, tval_.p, false);
+    mark(11);
+    // U list: near-field P2P
+    K::p2p_points(st_, pk, n_tleaf_, t_leaf_.p, u_ptr_.p, u_items_.p, trg_begin_.p, trg_end_.p, src_begin_.p, src_end_.p,
+                  center_.p, tx64_.p, n_trg_, sx64_.p, sden_.p, n_src_, tval_.p, true);
+    mark(12);
+    // targets not in any leaf with targets are impossible; every target lies in exactly one leaf
+    K::scatter_output(st_, n_trg_, k1_, tval_.p, trg_perm_.p, out_scale, d_out, 1, 0);
+    mark(13);
+    if (timing_) {
+      HIP_CHECK(hipEventSynchronize(ev_[kPhaseCount]));
+      for (int p = 0; p < kPhaseCount; p++) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev_[p], ev_[p + 1]));
+        phase_ms[p] = ms;
+      }
+    }
+  }
+};
+
+}  // namespace
+
+std::unique_ptr<DevicePlanBase> make_plan(const Octree& T, const Operators& O, const double* src_xyz, const double* trg_xyz,
+                                          int precision, int n_vec_max, hipStream_t stream) {
+  if (precision == 0) return std::make_unique<DevicePlan<double>>(T, O, src_xyz, trg_xyz, n_vec_max, stream);
+  if (precision == 1) return std::make_unique<DevicePlan<float>>(T, O, src_xyz, trg_xyz, n_vec_max, stream);
+  throw Error(kErrInvalidArgument, "precision must be 0 (fp64) or 1 (mixed fp32)");
+}
+
+}  // namespace pvfmm_amd

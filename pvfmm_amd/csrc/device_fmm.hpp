@@ -1,0 +1,62 @@
+// Device-resident evaluation plan of one tree: index arrays per phase, operator copies in the
+// working precision, expansion buffers, and the launch sequence of one evaluation.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "octree.hpp"
+#include "operators.hpp"
+
+namespace pvfmm_amd {
+
+[[noreturn]] void throw_device_error(hipError_t e, const char* file, int line);
+#define HIP_CHECK(x)                                              \
+  do {                                                            \
+    hipError_t e_ = (x);                                          \
+    if (e_ != hipSuccess) throw_device_error(e_, __FILE__, __LINE__); \
+  } while (0)
+
+// Phases timed by the plan (HIP events on the plan's stream), names follow the reference's
+// Profile tags (include/fmm_tree.txx:283-713, include/fmm_pts.txx:3729-3755).
+enum Phase : int {
+  kPhGather = 0, kPhS2UP2P, kPhS2UGemm, kPhU2U, kPhXList, kPhFFT, kPhHadamard, kPhIFFT, kPhD2D, kPhD2TGemm,
+  kPhFarP2P, kPhUList, kPhScatter, kPhaseCount
+};
+const char* phase_name(int p);
+
+struct PlanStats {
+  int64_t n_nodes = 0, n_leaves = 0, max_depth = 0;
+  int64_t s2m_rows = 0, l2t_rows = 0, m2m_rows = 0, l2l_rows = 0, x_rows = 0, x_items = 0;
+  int64_t v_src_slots = 0, v_trg_slots = 0, v_pairs = 0;
+  int64_t u_rows = 0, u_items = 0, u_pairs = 0, far_items = 0, far_pairs = 0, s2m_pairs = 0;
+  int64_t u_src_points_streamed = 0;
+  int64_t device_bytes = 0;
+};
+
+class DevicePlanBase {
+ public:
+  virtual ~DevicePlanBase() = default;
+  // d_den: user layout (k0 per source, input order) for n_vec vectors with stride k0*n_src;
+  // d_out: user layout (k1 per target, input order), overwritten.
+  virtual void evaluate(const double* d_den, double* d_out, int n_vec, double den_scale, double out_scale) = 0;
+  virtual hipStream_t stream() const = 0;
+  virtual bool is_fp32() const = 0;
+  // per-phase times of the last evaluation (ms); enabled by set_timing(true)
+  void set_timing(bool on) { timing_ = on; }
+  std::vector<double> phase_ms;
+  PlanStats stats;
+  int n_vec_max = 1;
+
+ protected:
+  bool timing_ = false;
+};
+
+// precision: 0 = FP64 throughout, 1 = FP32 bulk with FP64 operators precompute and FP64 merge
+std::unique_ptr<DevicePlanBase> make_plan(const Octree& T, const Operators& O, const double* src_xyz, const double* trg_xyz,
+                                          int precision, int n_vec_max, hipStream_t stream);
+
+}  // namespace pvfmm_amd

@@ -1,0 +1,64 @@
+// Launch wrappers of the gfx950 kernels (implemented in kernels.hip).
+#pragma once
+#include <cstdint>
+
+#include <hip/hip_runtime_api.h>
+
+namespace pvfmm_amd {
+
+// Point-kernel kinds of the device code: 0 laplace potential (1x1), 1 laplace gradient
+// (1x3, r = x_trg - x_src, i.e. minus the gradient of the potential), 2 stokes velocity (3x3).
+template <typename Real>
+struct DeviceKernels {
+  // sorted SoA densities from user AoS densities: den[c*n + i] = in[k0*perm[i] + c] * scal
+  static void gather_density(hipStream_t s, int64_t n, int k0, const double* in, const int64_t* perm,
+                             double scal, Real* den, int n_vec, int64_t in_stride);
+  // out[k1*perm[i] + j] = val[j*n + i] * scal (user AoS output, in input order)
+  static void scatter_output(hipStream_t s, int64_t n, int k1, const double* val, const int64_t* perm,
+                             double scal, double* out, int n_vec, int64_t out_stride);
+  static void fill(hipStream_t s, Real* p, int64_t n);
+  static void fill64(hipStream_t s, double* p, int64_t n);
+
+  // P2P into surface points (check surfaces): per row, targets are ns surface points of
+  // node tnode[row] (offsets rad*size*unit_surf from its centre); sources are the point ranges
+  // of the leaves items[item_ptr[row] .. item_ptr[row+1]).  out + orow[row]*ld (+)= result.
+  static void p2p_surface(hipStream_t s, int kind, int n_rows, const int32_t* tnode, const int32_t* item_ptr,
+                          const int32_t* items, const int32_t* src_begin, const int32_t* src_end,
+                          const double* center, const float* node_size, double rad, const Real* unit_surf, int ns,
+                          const Real* sx, const Real* sden, int64_t n_src, Real* out, const int32_t* orow,
+                          int64_t ld, bool accumulate);
+
+  // P2P into target points of leaves: point-range sources (U list).  tx/sx are FP64
+  // leaf-local coordinates; the displacement is formed in FP64, the kernel runs in Real.
+  static void p2p_points(hipStream_t s, int kind, int n_rows, const int32_t* tleaf, const int32_t* item_ptr,
+                         const int32_t* items, const int32_t* trg_begin, const int32_t* trg_end,
+                         const int32_t* src_begin, const int32_t* src_end, const double* center,
+                         const double* tx, int64_t n_trg, const double* sx, const Real* sden, int64_t n_src,
+                         double* out, bool accumulate);
+
+  // P2P into target points from surface sources (L2T own surface and W-list equivalent
+  // surfaces).  items are pairs (node, density row pointer index): den row = dens + drow*ld.
+  static void far_points(hipStream_t s, int kind, int n_rows, const int32_t* tleaf, const int32_t* item_ptr,
+                         const int32_t* item_node, const int64_t* item_den, const double* item_rad,
+                         const int32_t* trg_begin, const int32_t* trg_end, const double* center,
+                         const float* node_size, const Real* unit_surf, int ns, const Real* tx, int64_t n_trg,
+                         const Real* dens, double* out, bool accumulate);
+
+  // C[rc[i]*ldc : +N] (+)= alpha[i] * A[ra[i]*lda : +K] * B[K x N]
+  static void gemm_rows(hipStream_t s, int M, int N, int K, const Real* A, const int32_t* ra, int64_t lda,
+                        const Real* B, Real* C, const int32_t* rc, int64_t ldc, const Real* alpha,
+                        bool accumulate);
+
+  // V-list: forward DFT of the 8 children's equivalent densities of each source parent.
+  // X layout [slot][f][8*e0] complex.
+  static void fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0,
+                     int m, const int32_t* lattice, const Real* scal, Real* X);
+  // Y[slot][f][8*e1] = sum_R V[R][f] X[partner[slot][R]][f]
+  static void hadamard(hipStream_t s, int n_slots, int M_dim, int e0, int e1, const int32_t* partner,
+                       const int32_t* offsets, int n_off, const Real* V, const Real* X, Real* Y);
+  // inverse DFT, pick the check-surface points, dn[child] += scal * value
+  static void ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn,
+                        int K1, int e1, int m, const int32_t* lattice, int ns, const Real* scal);
+};
+
+}  // namespace pvfmm_amd
