@@ -223,9 +223,12 @@ class OracleTree:
         self.ops = None
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().oracle_tree_destroy(self.h)
-            self.h = None
+        try:
+            if getattr(self, "h", None):
+                lib().oracle_tree_destroy(self.h)
+                self.h = None
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
 
     def nodes(self) -> np.ndarray:
         """(n_nodes, 10): x, y, z, depth, is_leaf, parent, n_src, n_trg, pt_cnt0, pt_cnt1 (preorder)."""
