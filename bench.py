@@ -1,0 +1,201 @@
+"""Benchmark of the MI355X particle FMM on the headline workload.
+
+Workload (one "step" = one FMM evaluation of the hot path, tree/lists/operators prebuilt,
+densities already resident in HBM): Laplace potential + gradient (laplace_grad), N = 2^21
+points on a strongly adaptive octree (50% on the sphere |x-0.5|=0.4, 30% in a Gaussian blob of
+sigma 0.02 at (0.3,0.3,0.7), 20% uniform), densities N(0,1), m = 10, max_pts = 256.
+
+The reported value is the mixed-precision evaluator's Mtargets/s (FP32 P2P, M2L and up/down
+GEMMs; FP64 operator precompute and per-target merge).  The FP64 GPU evaluator is timed the
+same way and the ratio reported; both errors are relative L2 against an FP64 direct sum on 4096
+sampled targets.  The CPU baseline is the FP64 oracle (a restatement of the reference
+algorithm) on a 2^18-point sample of the same distribution, on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--n N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_FP32_MFMA_TFLOPS = 157.3
+MI355X_FP64_TFLOPS = 78.6
+MI355X_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def time_steps(tree, dq, steps, warmup, timing):
+    import torch
+
+    for _ in range(warmup):
+        tree.evaluate_device(dq, sync=True)
+    tree.set_timing(timing)
+    phase_sum = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = tree.evaluate_device(dq, sync=True)
+        if timing:
+            for k, v in tree.phase_times().items():
+                phase_sum[k] = phase_sum.get(k, 0.0) + v
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tree.set_timing(False)
+    phases = {k: v / steps for k, v in phase_sum.items()}
+    return (t1 - t0) / steps, out, phases
+
+
+def cpu_baseline(n_sample: int):
+    """FP64 oracle on a bounded sample of the workload (test infrastructure, baseline only)."""
+    import oracle
+    from pvfmm_amd import workloads as W
+
+    c = W.config2(n_sample)
+    ops = oracle.precompute(c["kernel"], c["m"])
+    tr = oracle.OracleTree(c["src"], c["trg"], c["max_pts"])
+    tr.set_operators(ops)
+    threads = oracle.nthreads()
+    tr.evaluate(c["den"], threads=threads)  # warm-up
+    ts = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        tr.evaluate(c["den"], threads=threads)
+        ts.append(time.perf_counter() - t0)
+    t = min(ts)
+    return {"value": n_sample / t / 1e6, "unit": "Mtargets/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 FMM evaluation (tree/operators prebuilt), same distribution and m/max_pts "
+                      f"at N={n_sample}, best of 2 after 1 warm-up, {t:.2f} s per evaluation"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=2097152)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=262144)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (the single-GPU MPI_COMM_SELF share); --gpus {args.gpus} is not "
+              "supported in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import pvfmm_amd as P
+    from pvfmm_amd import workloads as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    c = W.config2(args.n)
+    n = len(c["src"])
+    log(f"workload: {c['kernel']} N={n} m={c['m']} max_pts={c['max_pts']}")
+    t0 = time.perf_counter()
+    tree = P.PtFMMTree(c["src"], c["trg"], c["max_pts"])
+    t_tree = time.perf_counter() - t0
+    dq = torch.from_numpy(c["den"]).to("cuda:0")
+    rng = np.random.default_rng(4096)
+    idx = np.sort(rng.choice(n, 4096, replace=False))
+    # FP64 direct sum on the sampled targets (error reference)
+    import oracle
+
+    ud = oracle.direct(c["kernel"], c["src"], c["den"], c["src"][idx]).reshape(-1, 3)
+
+    res = {}
+    for prec in ["fp64", "mixed"]:
+        t0 = time.perf_counter()
+        fmm = P.PtFMM(c["kernel"], c["m"], prec)
+        t_ops = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        tree.setup(fmm)
+        t_setup = time.perf_counter() - t0
+        step, out, phases = time_steps(tree, dq, args.steps, args.warmup, timing=True)
+        # un-instrumented timing of the same steps
+        step_plain, out, _ = time_steps(tree, dq, args.steps, 0, timing=False)
+        u = out.cpu().numpy().reshape(n, 3)[idx]
+        res[prec] = dict(step=step_plain, step_timed=step, phases=phases, err=rel_l2(u, ud), t_ops=t_ops,
+                         t_setup=t_setup, stats=tree.stats())
+        log(f"{prec}: {1e3 * step_plain:.2f} ms/eval ({n / step_plain / 1e6:.1f} Mtargets/s), rel-L2 {res[prec]['err']:.3e}, "
+            f"operators {t_ops:.1f}s, setup {t_setup:.1f}s")
+        log("   phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
+
+    mixed, fp64 = res["mixed"], res["fp64"]
+    st = mixed["stats"]
+    # dominant kernel of the mixed evaluator and its algorithmic work
+    ph = mixed["phases"]
+    dom = max(ph, key=ph.get)
+    m = c["m"]
+    n1 = 2 * m
+    M_dim = n1 * n1 * (n1 // 2 + 1)
+    had_flops = 512.0 * st["v_pairs"] * M_dim  # include/fmm_pts.txx:3336 accounting, k0 = k1 = 1
+    if dom == "V-List:Hadamard":
+        achieved = had_flops / (ph[dom] * 1e-3) / 1e12
+        roof = {"kernel": "k_hadamard<float,1,1> (V-list frequency-domain M2L)", "bound": "mfma", "achieved": achieved,
+                "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_MFMA_TFLOPS,
+                "traffic": None, "ms_per_launch": ph[dom],
+                "work": f"512 flop x {st['v_pairs']} V1 pairs x {M_dim} frequencies per launch"}
+    elif dom == "U-List":
+        flops = 16.0 * st["u_pairs"]
+        achieved = flops / (ph[dom] * 1e-3) / 1e12
+        roof = {"kernel": "k_p2p_points<float,1> (U-list P2P)", "bound": "mfma", "achieved": achieved,
+                "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_MFMA_TFLOPS,
+                "traffic": None, "ms_per_launch": ph[dom], "work": f"16 flop x {st['u_pairs']} pairs"}
+    else:
+        roof = {"kernel": dom, "bound": "mfma", "achieved": None, "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                "frac": None, "traffic": None, "ms_per_launch": ph[dom]}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(min(args.cpu_sample, n))
+            log(f"cpu baseline: {cpu['value']:.4f} Mtargets/s on {cpu['cores']} threads")
+        except Exception as e:  # noqa: BLE001 - the baseline is reported, not required
+            log(f"cpu baseline failed: {e}")
+
+    value = n / mixed["step"] / 1e6
+    line = {
+        "metric": "Laplace FMM Mtargets/s, N=2M, mixed vs FP64 GPU at equal L2 err; bwd/fwd ratio",
+        "value": value,
+        "unit": "Mtargets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"laplace_grad FMM evaluation, N={n} adaptive (50% sphere r=0.4, 30% gaussian "
+                               f"sigma=0.02, 20% uniform), m={m}, max_pts={c['max_pts']}, mixed FP32 bulk / FP64 "
+                               f"precompute and merge", "n_targets": n, "m": m, "max_pts": c["max_pts"]},
+        "fp64_mtargets_s": n / fp64["step"] / 1e6,
+        "fp64_ms_per_step": 1e3 * fp64["step"],
+        "mixed_over_fp64": fp64["step"] / mixed["step"],
+        "rel_l2_mixed": mixed["err"],
+        "rel_l2_fp64": fp64["err"],
+        "bwd_fwd_ratio": None,
+        "setup_s": {"tree": t_tree, "operators_mixed": mixed["t_ops"], "plan_mixed": mixed["t_setup"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -210,7 +210,11 @@ void context_eval(Context* c, const Real* src_pos, const Real* sl_den, const Rea
     map(trg_pos, n_trg, tp);
     c->tree.reset();
     c->tree.reset(make_tree(sp.data(), n_src, tp.data(), n_trg, c->grid.data(), (int64_t)(c->grid.size() / 3), c->max_pts));
-    c->fmm->precision = c->precision;
+    if (c->fmm->precision != c->precision) {
+      // the pseudo-inverse truncation depends on the evaluation precision
+      build_operators(c->fmm->O, c->kernel, c->m, 0.0, c->precision);
+      c->fmm->precision = c->precision;
+    }
     setup_tree(c->tree.get(), c->fmm.get(), 1);
     c->scale_x = (double)scale_x;
     c->n_src = n_src;
