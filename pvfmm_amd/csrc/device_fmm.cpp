@@ -85,6 +85,8 @@ class DevicePlan final : public DevicePlanBase {
   DBuf<Real> eq_, dn_, chk_;
   // operators
   DBuf<Real> uc2ue0_, uc2ue1_, dc2de0_, dc2de1_, u2u_, d2d_, V_, unit_surf_, ctmp_;
+  DBuf<Real> gemm_work_;  // split-K partial tiles of the small coarse-level GEMMs
+  static constexpr int64_t kGemmWork = int64_t(1) << 22;
   int ru_ = 0, rd_ = 0;
   DBuf<int32_t> lattice_;
   // S2M
@@ -180,9 +182,26 @@ class DevicePlan final : public DevicePlanBase {
     u2u_.upload(conv(O_.u2u), &total);
     d2d_.upload(conv(O_.d2d), &total);
     {
-      std::vector<Real> V(2 * O_.v1.size());
-      for (size_t i = 0; i < O_.v1.size(); i++) { V[2 * i] = (Real)O_.v1[i].real(); V[2 * i + 1] = (Real)O_.v1[i].imag(); }
-      V_.upload(V, &total);
+      // Transfer blocks in real form for y = x B with x, y interleaved (re, im):
+      //   B[2s][2t] = Re M[t][s], B[2s+1][2t] = -Im M[t][s], B[2s][2t+1] = Im M[t][s],
+      //   B[2s+1][2t+1] = Re M[t][s],
+      // stored per MFMA lane: [R][f][kb][nb][lane][q] = B[16kb + 4(lane/16) + q][16nb + lane%16].
+      const int E0 = e0_, E1 = e1_, NS = 8 * E0, NR = 8 * E1;
+      std::vector<Real> Vp((size_t)27 * M_dim_ * E0 * E1 * 256, (Real)0);
+      #pragma omp parallel for schedule(static)
+      for (int64_t rf = 0; rf < (int64_t)27 * M_dim_; rf++) {
+        const std::complex<double>* Mb = &O_.v1[(size_t)rf * NR * NS];  // [t][s]
+        for (int kb = 0; kb < E0; kb++)
+          for (int nb = 0; nb < E1; nb++)
+            for (int l = 0; l < 64; l++)
+              for (int q = 0; q < 4; q++) {
+                const int k = 16 * kb + 4 * (l >> 4) + q, n = 16 * nb + (l & 15);
+                const std::complex<double> v = Mb[(size_t)(n >> 1) * NS + (k >> 1)];
+                double val = ((k & 1) == 0) ? ((n & 1) == 0 ? v.real() : v.imag()) : ((n & 1) == 0 ? -v.imag() : v.real());
+                Vp[((((size_t)rf * E0 + kb) * E1 + nb) * 64 + l) * 4 + q] = (Real)val;
+              }
+      }
+      V_.upload(Vp, &total);
     }
     unit_surf_.upload(conv(O_.unit_surf), &total);
     lattice_.upload(O_.lattice, &total);
@@ -223,6 +242,7 @@ class DevicePlan final : public DevicePlanBase {
     l2t_seq_.upload(l2t_seq, &total);
     ctmp_.alloc((int64_t)std::max(1, std::max(n_s2m_, n_l2t_)) * std::max(1, std::max(ru_, rd_)), &total);
     t_leaf_.upload(tleaf, &total);
+    gemm_work_.alloc(kGemmWork, &total);
     eq_.alloc((int64_t)(N + n_l2t_) * K0_, &total);
     dn_.alloc((int64_t)N * K1_, &total);
 
@@ -368,13 +388,13 @@ class DevicePlan final : public DevicePlanBase {
     K::p2p_surface(st_, ek, n_s2m_, s2m_node_.p, s2m_ptr_.p, s2m_items_.p, src_begin_.p, src_end_.p, center_.p, node_size_.p,
                    kRad1, unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, chk_.p, s2m_orow_.p, K1_, false);
     mark(2);
-    K::gemm_rows(st_, n_s2m_, ru_, K1_, chk_.p, s2m_orow_.p, K1_, uc2ue0_.p, ctmp_.p, s2m_orow_.p, ru_, s2m_alpha_.p, false);
-    K::gemm_rows(st_, n_s2m_, K0_, ru_, ctmp_.p, s2m_orow_.p, ru_, uc2ue1_.p, eq_.p, s2m_node_.p, K0_, nullptr, false);
+    K::gemm_rows(st_, n_s2m_, ru_, K1_, chk_.p, s2m_orow_.p, K1_, uc2ue0_.p, ctmp_.p, s2m_orow_.p, ru_, s2m_alpha_.p, false, gemm_work_.p, kGemmWork);
+    K::gemm_rows(st_, n_s2m_, K0_, ru_, ctmp_.p, s2m_orow_.p, ru_, uc2ue1_.p, eq_.p, s2m_node_.p, K0_, nullptr, false, gemm_work_.p, kGemmWork);
     mark(3);
     // U2U, bottom-up: parent row = [8 children] x stacked U2U
     for (int l = max_depth_ - 1; l >= 0; l--) {
       const LevelRows& L = *m2m_[l];
-      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false);
+      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false, gemm_work_.p, kGemmWork);
     }
     mark(4);
     // X list: source points -> downward check surface (RAD0)
@@ -391,12 +411,12 @@ class DevicePlan final : public DevicePlanBase {
     // D2D, top-down: 8 children rows += parent x [D2D_0 .. D2D_7]
     for (int l = 1; l <= max_depth_; l++) {
       const LevelRows& L = *l2l_[l - 1];
-      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true);
+      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true, gemm_work_.p, kGemmWork);
     }
     mark(9);
     // D2T: check -> equivalent
-    K::gemm_rows(st_, n_l2t_, rd_, K1_, dn_.p, l2t_ra_.p, K1_, dc2de0_.p, ctmp_.p, l2t_seq_.p, rd_, l2t_alpha_.p, false);
-    K::gemm_rows(st_, n_l2t_, K0_, rd_, ctmp_.p, l2t_seq_.p, rd_, dc2de1_.p, eq_.p, l2t_rc_.p, K0_, nullptr, false);
+    K::gemm_rows(st_, n_l2t_, rd_, K1_, dn_.p, l2t_ra_.p, K1_, dc2de0_.p, ctmp_.p, l2t_seq_.p, rd_, l2t_alpha_.p, false, gemm_work_.p, kGemmWork);
+    K::gemm_rows(st_, n_l2t_, K0_, rd_, ctmp_.p, l2t_seq_.p, rd_, dc2de1_.p, eq_.p, l2t_rc_.p, K0_, nullptr, false, gemm_work_.p, kGemmWork);
     mark(10);
     // equivalent surfaces -> targets (own downward equivalent + W list)
     K::far_points(st_, pk, n_tleaf_, t_leaf_.p, far_ptr_.p, far_node_.p, far_den_.p, far_rad_.p, trg_begin_.p, trg_end_.p,

@@ -310,66 +310,113 @@ __global__ __launch_bounds__(64) void k_p2p_surface(const int32_t* __restrict__ 
   }
 }
 
+// MFMA operand helpers.  gfx950 lane layouts (checked on the device): A 16x4 lane l holds
+// (row l%16, k l/16); B 4x16 lane l holds (k l/16, col l%16); the 16x16 result holds
+// rows 4*(l/16)+r (FP32) or l/16+4r (FP64), r = 0..3, at column l%16.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+template <typename Real> struct Mfma;
+template <> struct Mfma<float> {
+  using V4 = floatx4;
+  static __device__ __forceinline__ V4 mma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+template <> struct Mfma<double> {
+  using V4 = doublex4;
+  static __device__ __forceinline__ V4 mma(double a, double b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+
 // ----------------------------------------------------------------------------- row-gathered GEMM
-template <typename Real, bool ACC>
-__global__ __launch_bounds__(256) void k_gemm_rows(int M, int N, int K, const Real* __restrict__ A, const int32_t* __restrict__ ra,
-                                                   int64_t lda, const Real* __restrict__ B, Real* __restrict__ C,
-                                                   const int32_t* __restrict__ rc, int64_t ldc, const Real* __restrict__ alpha) {
-  constexpr int BM = 64, BN = 64, BK = 16;
-  __shared__ Real As[BK][BM + 1];
-  __shared__ Real Bs[BK][BN + 1];
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+// C[rc[i]*ldc + n] (+)= alpha[i] * sum_k A[ra[i]*lda + k] * B[k*N + n] on MFMA.
+// Block tile 64x64 (4 waves, each a 32x32 quadrant = 2x2 MFMA 16x16 tiles), k staged 16 at a
+// time in LDS with k contiguous for both operands so each lane feeds its four k values of an
+// MFMA group from one 4-vector read.  With ksplit > 1 the block handles one k range and writes
+// its partial tile to work[z][i][n]; k_gemm_reduce then sums the partials in a fixed order.
+template <typename Real, bool ACC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_gemm_rows(int M, int N, int K, int kchunk, const Real* __restrict__ A,
+                                                   const int32_t* __restrict__ ra, int64_t lda, const Real* __restrict__ B,
+                                                   Real* __restrict__ C, const int32_t* __restrict__ rc, int64_t ldc,
+                                                   const Real* __restrict__ alpha, Real* __restrict__ work) {
+  using MF = Mfma<Real>;
+  using V4 = typename MF::V4;
+  constexpr int BM = 64, BN = 64, BK = 16, LD = BK + 4;
+  __shared__ Real As[BM * LD];
+  __shared__ Real Bt[BN * LD];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  int64_t abase[4];
-  bool aval[4];
+  const int kb0 = blockIdx.z * kchunk, kb1 = min(K, kb0 + kchunk);
+  // global->LDS assignments: A: 4 consecutive k of one row; B: 4 consecutive n of one k
+  const int ar = tid >> 2, ak = (tid & 3) * 4;
+  const int bk = tid >> 4, bn = (tid & 15) * 4;
+  const bool arow_ok = m0 + ar < M;
+  const int64_t abase = arow_ok ? (int64_t)ra[m0 + ar] * lda : 0;
+  V4 acc[2][2];
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int e = tid + 256 * q, r = e >> 4, gm = m0 + r;
-    aval[q] = gm < M;
-    abase[q] = aval[q] ? (int64_t)ra[gm] * lda : 0;
-  }
-  Real acc[4][4];
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0;
-  for (int k0 = 0; k0 < K; k0 += BK) {
+    for (int j = 0; j < 2; j++) acc[i][j] = V4{0, 0, 0, 0};
+  for (int k0 = kb0; k0 < kb1; k0 += BK) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int e = tid + 256 * q, r = e >> 4, c = e & 15, gk = k0 + c;
-      As[c][r] = (aval[q] && gk < K) ? A[abase[q] + gk] : Real(0);
+      const int k = k0 + ak + q;
+      As[ar * LD + ak + q] = (arow_ok && k < kb1) ? A[abase + k] : Real(0);
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int e = tid + 256 * q, r = e >> 6, c = e & 63, gk = k0 + r, gn = n0 + c;
-      Bs[r][c] = (gk < K && gn < N) ? B[(int64_t)gk * N + gn] : Real(0);
+      const int k = k0 + bk, n = n0 + bn + q;
+      Bt[(bn + q) * LD + bk] = (k < kb1 && n < N) ? B[(int64_t)k * N + n] : Real(0);
     }
     __syncthreads();
+    V4 av[2], bv[2];
 #pragma unroll
-    for (int kk = 0; kk < BK; kk++) {
-      Real a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) a[i] = As[kk][ty * 4 + i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx * 4 + j];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    for (int t = 0; t < 2; t++) {
+      av[t] = *reinterpret_cast<const V4*>(&As[(wm * 32 + t * 16 + (l & 15)) * LD + 4 * (l >> 4)]);
+      bv[t] = *reinterpret_cast<const V4*>(&Bt[(wn * 32 + t * 16 + (l & 15)) * LD + 4 * (l >> 4)]);
     }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        acc[i][j] = MF::mma(av[i][0], bv[j][0], acc[i][j]);
+        acc[i][j] = MF::mma(av[i][1], bv[j][1], acc[i][j]);
+        acc[i][j] = MF::mma(av[i][2], bv[j][2], acc[i][j]);
+        acc[i][j] = MF::mma(av[i][3], bv[j][3], acc[i][j]);
+      }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int gm = m0 + ty * 4 + i;
-    if (gm >= M) continue;
-    Real* crow = C + (int64_t)rc[gm] * ldc;
-    const Real al = alpha ? alpha[gm] : Real(1);
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int gn = n0 + tx * 4 + j;
-      if (gn < N) crow[gn] = ACC ? fma(al, acc[i][j], crow[gn]) : al * acc[i][j];
-    }
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gm = m0 + wm * 32 + i * 16 + MF::row(l, r);
+        const int gn = n0 + wn * 32 + j * 16 + (l & 15);
+        if (gm >= M || gn >= N) continue;
+        const Real v = acc[i][j][r];
+        if (SPLIT) {
+          work[((int64_t)blockIdx.z * M + gm) * N + gn] = v;
+        } else {
+          Real* crow = C + (int64_t)rc[gm] * ldc;
+          const Real al = alpha ? alpha[gm] : Real(1);
+          crow[gn] = ACC ? fma(al, v, crow[gn]) : al * v;
+        }
+      }
+}
+
+template <typename Real, bool ACC>
+__global__ void k_gemm_reduce(int M, int N, int ksplit, const Real* __restrict__ work, Real* __restrict__ C,
+                              const int32_t* __restrict__ rc, int64_t ldc, const Real* __restrict__ alpha) {
+  const int64_t total = (int64_t)M * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / N), n = (int)(e % N);
+    Real v = 0;
+    for (int z = 0; z < ksplit; z++) v += work[((int64_t)z * M + i) * N + n];
+    Real* crow = C + (int64_t)rc[i] * ldc;
+    const Real al = alpha ? alpha[i] : Real(1);
+    crow[n] = ACC ? fma(al, v, crow[n]) : al * v;
   }
 }
 
@@ -385,42 +432,46 @@ __device__ void init_twiddles(Real* twr, Real* twi, int n1, double sign) {
   }
 }
 
-// grid: n_slots * 8 * e0 blocks of 256 threads.  Per block one child grid: m^3 lattice
-// (surface values only) -> n1 x n1 x nh spectrum written to X[slot][f][c*e0+i].
+// Forward DFT of CB children of one source parent per block (grid: n_slots * e0 * (8/CB)
+// blocks of 512 threads).  The m^3 lattice cube of each child (surface values only) sits in
+// LDS; the transform runs one x-frequency slice at a time: x (real->complex, m taps), y and z
+// (m taps each, the pruned inputs).  Spectra are written as X[slot][f][c*e0+i] complex so
+// that the children of one frequency form one contiguous row.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_fft_up(const int32_t* __restrict__ first_child, const Real* __restrict__ up, int K0, int e0,
-                                                int m, const int32_t* __restrict__ lattice, const Real* __restrict__ scal,
+__global__ __launch_bounds__(512) void k_fft_up(const int32_t* __restrict__ first_child, const Real* __restrict__ up, int K0, int e0,
+                                                int m, int CB, const int32_t* __restrict__ lattice, const Real* __restrict__ scal,
                                                 Real* __restrict__ X) {
   extern __shared__ unsigned char smem_raw[];
   Real* sm = reinterpret_cast<Real*>(smem_raw);
   const int n1 = 2 * m, nh = m + 1, M_dim = n1 * n1 * nh, ns = 6 * (m - 1) * (m - 1) + 2;
-  Real* cube = sm;                       // m^3
-  Real* twr = cube + m * m * m;          // n1
-  Real* twi = twr + n1;                  // n1
-  Real* s1r = twi + n1;                  // m*m   [z][y]
-  Real* s1i = s1r + m * m;
-  Real* s2r = s1i + m * m;               // m*n1  [z][ky]
-  Real* s2i = s2r + m * n1;
-  const int b = blockIdx.x;
-  const int slot = b / (8 * e0), rem = b % (8 * e0), c = rem / e0, i = rem % e0;
-  const int child = first_child[slot] + c;
+  const int m2 = m * m, m3 = m2 * m, mn = m * n1;
+  Real* cube = sm;                 // CB * m^3
+  Real* twr = cube + CB * m3;      // n1
+  Real* twi = twr + n1;
+  Real* s1r = twi + n1;            // CB * m*m   [c][z][y]
+  Real* s1i = s1r + CB * m2;
+  Real* s2r = s1i + CB * m2;       // CB * m*n1  [c][z][ky]
+  Real* s2i = s2r + CB * mn;
+  const int nblk = 8 / CB;
+  const int slot = blockIdx.x / (e0 * nblk), rem = blockIdx.x % (e0 * nblk), i = rem / nblk, c0 = (rem % nblk) * CB;
+  const int fc = first_child[slot] + c0;
   const Real sc = scal[slot];
+  const int tid = threadIdx.x, nt = blockDim.x;
   init_twiddles(twr, twi, n1, -1.0);
-  for (int q = threadIdx.x; q < m * m * m; q += blockDim.x) cube[q] = 0;
+  for (int q = tid; q < CB * m3; q += nt) cube[q] = 0;
   __syncthreads();
-  const Real* u = up + (int64_t)child * K0;
-  for (int p = threadIdx.x; p < ns; p += blockDim.x) {
+  for (int q = tid; q < CB * ns; q += nt) {
+    const int c = q / ns, p = q % ns;
     const int x = lattice[3 * p], y = lattice[3 * p + 1], z = lattice[3 * p + 2];
-    cube[(z * m + y) * m + x] = u[p * e0 + i] * sc;
+    cube[c * m3 + (z * m + y) * m + x] = up[(int64_t)(fc + c) * K0 + p * e0 + i] * sc;
   }
   __syncthreads();
-  Real* Xs = X + (int64_t)slot * M_dim * (16 * e0) + 2 * (c * e0 + i);
+  Real* Xs = X + ((int64_t)slot * M_dim * 8 * e0 + c0 * e0 + i) * 2;
   for (int kx = 0; kx < nh; kx++) {
-    for (int o = threadIdx.x; o < m * m; o += blockDim.x) {  // x: real -> complex
-      const Real* line = &cube[o * m];
+    for (int o = tid; o < CB * m2; o += nt) {  // x: real -> complex
+      const Real* line = &cube[(o / m2) * m3 + (o % m2) * m];
       Real ar = 0, ai = 0;
-      for (int x = 0; x < m; x++) {
-        const int e = (kx * x) % n1;
+      for (int x = 0, e = 0; x < m; x++, e = (e + kx >= n1) ? e + kx - n1 : e + kx) {
         ar = fma(line[x], twr[e], ar);
         ai = fma(line[x], twi[e], ai);
       }
@@ -428,91 +479,98 @@ __global__ __launch_bounds__(256) void k_fft_up(const int32_t* __restrict__ firs
       s1i[o] = ai;
     }
     __syncthreads();
-    for (int o = threadIdx.x; o < m * n1; o += blockDim.x) {  // y
-      const int z = o / n1, ky = o % n1;
+    for (int o = tid; o < CB * mn; o += nt) {  // y
+      const int c = o / mn, r = o % mn, z = r / n1, ky = r % n1;
+      const Real* vr = &s1r[c * m2 + z * m];
+      const Real* vi = &s1i[c * m2 + z * m];
       Real ar = 0, ai = 0;
-      for (int y = 0; y < m; y++) {
-        const int e = (ky * y) % n1;
-        const Real vr = s1r[z * m + y], vi = s1i[z * m + y];
-        ar += vr * twr[e] - vi * twi[e];
-        ai += vr * twi[e] + vi * twr[e];
+      for (int y = 0, e = 0; y < m; y++, e = (e + ky >= n1) ? e + ky - n1 : e + ky) {
+        ar += vr[y] * twr[e] - vi[y] * twi[e];
+        ai += vr[y] * twi[e] + vi[y] * twr[e];
       }
       s2r[o] = ar;
       s2i[o] = ai;
     }
     __syncthreads();
-    for (int o = threadIdx.x; o < n1 * n1; o += blockDim.x) {  // z
-      const int ky = o % n1, kz = o / n1;
+    for (int o = tid; o < CB * n1 * n1; o += nt) {  // z; children fastest so rows are written contiguously
+      const int c = o % CB, q = o / CB, ky = q % n1, kz = q / n1;
+      const Real* vr = &s2r[c * mn + ky];
+      const Real* vi = &s2i[c * mn + ky];
       Real ar = 0, ai = 0;
-      for (int z = 0; z < m; z++) {
-        const int e = (kz * z) % n1;
-        const Real vr = s2r[z * n1 + ky], vi = s2i[z * n1 + ky];
-        ar += vr * twr[e] - vi * twi[e];
-        ai += vr * twi[e] + vi * twr[e];
+      for (int z = 0, e = 0; z < m; z++, e = (e + kz >= n1) ? e + kz - n1 : e + kz) {
+        ar += vr[z * n1] * twr[e] - vi[z * n1] * twi[e];
+        ai += vr[z * n1] * twi[e] + vi[z * n1] * twr[e];
       }
-      const int f = kx + nh * (ky + n1 * kz);
-      Xs[(int64_t)f * (16 * e0)] = ar;
-      Xs[(int64_t)f * (16 * e0) + 1] = ai;
+      const int64_t f = kx + nh * (ky + n1 * kz);
+      Real* d = Xs + (f * 8 * e0 + c * e0) * 2;
+      d[0] = ar;
+      d[1] = ai;
     }
     __syncthreads();
   }
 }
 
-// grid: n_slots * 8 * e1 blocks of 256 threads.  Inverse DFT of one child spectrum,
-// evaluated at the ns check-surface lattice points only; dn[child] += scal * value.
+// Inverse DFT of CB children of one target parent per block (grid: n_slots * e1 * (8/CB)
+// blocks of 512 threads), evaluated only at the check-surface lattice points; per x-frequency
+// slice the n1 x n1 spectrum plane of each child is read once (contiguous child rows), then z,
+// y (n1 taps, m outputs) and the c2r x-sum is accumulated in registers.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_ifft_down(const int32_t* __restrict__ first_child, const Real* __restrict__ Y,
-                                                   Real* __restrict__ dn, int K1, int e1, int m, const int32_t* __restrict__ lattice,
-                                                   int ns, const Real* __restrict__ scal) {
+__global__ __launch_bounds__(512) void k_ifft_down(const int32_t* __restrict__ first_child, const Real* __restrict__ Y,
+                                                   Real* __restrict__ dn, int K1, int e1, int m, int CB,
+                                                   const int32_t* __restrict__ lattice, int ns, const Real* __restrict__ scal) {
   extern __shared__ unsigned char smem_raw[];
   Real* sm = reinterpret_cast<Real*>(smem_raw);
   const int n1 = 2 * m, nh = m + 1, M_dim = n1 * n1 * nh;
-  Real* twr = sm;                 // n1
+  const int nn = n1 * n1, nm = n1 * m, m2 = m * m;
+  Real* twr = sm;                  // n1
   Real* twi = twr + n1;
-  Real* yr = twi + n1;            // n1*n1 [kz][ky]
-  Real* yi = yr + n1 * n1;
-  Real* s1r = yi + n1 * n1;       // n1*m  [ky][z]
-  Real* s1i = s1r + n1 * m;
-  Real* s2r = s1i + n1 * m;       // m*m   [y][z]
-  Real* s2i = s2r + m * m;
-  constexpr int MAXP = 8;
-  const int b = blockIdx.x;
-  const int slot = b / (8 * e1), rem = b % (8 * e1), c = rem / e1, j = rem % e1;
-  const int child = first_child[slot] + c;
+  Real* yr = twi + n1;             // CB * n1*n1  [c][kz][ky]
+  Real* yi = yr + CB * nn;
+  Real* s1r = yi + CB * nn;        // CB * n1*m   [c][ky][z]
+  Real* s1i = s1r + CB * nm;
+  Real* s2r = s1i + CB * nm;       // CB * m*m    [c][y][z]
+  Real* s2i = s2r + CB * m2;
+  constexpr int MAXQ = 24;
+  const int nblk = 8 / CB;
+  const int slot = blockIdx.x / (e1 * nblk), rem = blockIdx.x % (e1 * nblk), j = rem / nblk, c0 = (rem % nblk) * CB;
+  const int fc = first_child[slot] + c0;
+  const int tid = threadIdx.x, nt = blockDim.x;
   init_twiddles(twr, twi, n1, 1.0);
-  Real acc[MAXP];
+  Real acc[MAXQ];
 #pragma unroll
-  for (int q = 0; q < MAXP; q++) acc[q] = 0;
-  const Real* Ys = Y + (int64_t)slot * M_dim * (16 * e1) + 2 * (c * e1 + j);
+  for (int q = 0; q < MAXQ; q++) acc[q] = 0;
+  const Real* Ys = Y + ((int64_t)slot * M_dim * 8 * e1 + c0 * e1 + j) * 2;
   __syncthreads();
   for (int kx = 0; kx < nh; kx++) {
-    for (int o = threadIdx.x; o < n1 * n1; o += blockDim.x) {  // o = ky + n1*kz
-      const int64_t f = kx + (int64_t)nh * o;
-      yr[o] = Ys[f * (16 * e1)];
-      yi[o] = Ys[f * (16 * e1) + 1];
+    for (int o = tid; o < CB * nn; o += nt) {  // children fastest: contiguous rows of Y
+      const int c = o % CB, q = o / CB;         // q = ky + n1*kz
+      const int64_t f = kx + (int64_t)nh * q;
+      const Real* src = Ys + (f * 8 * e1 + c * e1) * 2;
+      yr[c * nn + q] = src[0];
+      yi[c * nn + q] = src[1];
     }
     __syncthreads();
-    for (int o = threadIdx.x; o < n1 * m; o += blockDim.x) {  // z
-      const int ky = o / m, z = o % m;
+    for (int o = tid; o < CB * nm; o += nt) {  // z
+      const int c = o / nm, r = o % nm, ky = r / m, z = r % m;
+      const Real* vr = &yr[c * nn + ky];
+      const Real* vi = &yi[c * nn + ky];
       Real ar = 0, ai = 0;
-      for (int kz = 0; kz < n1; kz++) {
-        const int e = (kz * z) % n1;
-        const Real vr = yr[ky + n1 * kz], vi = yi[ky + n1 * kz];
-        ar += vr * twr[e] - vi * twi[e];
-        ai += vr * twi[e] + vi * twr[e];
+      for (int kz = 0, e = 0; kz < n1; kz++, e = (e + z >= n1) ? e + z - n1 : e + z) {
+        ar += vr[kz * n1] * twr[e] - vi[kz * n1] * twi[e];
+        ai += vr[kz * n1] * twi[e] + vi[kz * n1] * twr[e];
       }
       s1r[o] = ar;
       s1i[o] = ai;
     }
     __syncthreads();
-    for (int o = threadIdx.x; o < m * m; o += blockDim.x) {  // y
-      const int y = o / m, z = o % m;
+    for (int o = tid; o < CB * m2; o += nt) {  // y
+      const int c = o / m2, r = o % m2, y = r / m, z = r % m;
+      const Real* vr = &s1r[c * nm + z];
+      const Real* vi = &s1i[c * nm + z];
       Real ar = 0, ai = 0;
-      for (int ky = 0; ky < n1; ky++) {
-        const int e = (ky * y) % n1;
-        const Real vr = s1r[ky * m + z], vi = s1i[ky * m + z];
-        ar += vr * twr[e] - vi * twi[e];
-        ai += vr * twi[e] + vi * twr[e];
+      for (int ky = 0, e = 0; ky < n1; ky++, e = (e + y >= n1) ? e + y - n1 : e + y) {
+        ar += vr[ky * m] * twr[e] - vi[ky * m] * twi[e];
+        ai += vr[ky * m] * twi[e] + vi[ky * m] * twr[e];
       }
       s2r[o] = ar;
       s2i[o] = ai;
@@ -520,71 +578,101 @@ __global__ __launch_bounds__(256) void k_ifft_down(const int32_t* __restrict__ f
     __syncthreads();
     const Real wk = (kx == 0 || 2 * kx == n1) ? Real(1) : Real(2);
 #pragma unroll
-    for (int q = 0; q < MAXP; q++) {
-      const int p = threadIdx.x + q * blockDim.x;
-      if (p < ns) {
+    for (int q = 0; q < MAXQ; q++) {
+      const int o = tid + q * nt;
+      if (o < CB * ns) {
+        const int c = o / ns, p = o % ns;
         const int x = lattice[3 * p], y = lattice[3 * p + 1], z = lattice[3 * p + 2];
-        const int e = (kx * x) % n1;
-        acc[q] += wk * (s2r[y * m + z] * twr[e] - s2i[y * m + z] * twi[e]);
+        const int kxx = kx * x, e = kxx - n1 * (kxx / n1);
+        const int si = c * m2 + y * m + z;
+        acc[q] += wk * (s2r[si] * twr[e] - s2i[si] * twi[e]);
       }
     }
     __syncthreads();
   }
   const Real sc = scal[slot];
-  Real* d = dn + (int64_t)child * K1;
 #pragma unroll
-  for (int q = 0; q < MAXP; q++) {
-    const int p = threadIdx.x + q * blockDim.x;
-    if (p < ns) d[p * e1 + j] += acc[q] * sc;
+  for (int q = 0; q < MAXQ; q++) {
+    const int o = tid + q * nt;
+    if (o < CB * ns) {
+      const int c = o / ns, p = o % ns;
+      dn[(int64_t)(fc + c) * K1 + p * e1 + j] += acc[q] * sc;
+    }
   }
 }
 
 // ----------------------------------------------------------------------------- V list: Hadamard
-// One wave = 64 target slots at one frequency; blocks are assigned so that all blocks of a
-// frequency run on the same XCD (block b lands on XCD b % 8), keeping X[.][f] in that L2.
-template <typename Real, int E0, int E1>
-__global__ __launch_bounds__(64) void k_hadamard(int n_slots, int M_dim, int n_tblk, const int32_t* __restrict__ partner,
-                                                 const int32_t* __restrict__ offsets, int n_off, const Real* __restrict__ V,
-                                                 const Real* __restrict__ X, Real* __restrict__ Y) {
-  const int b = blockIdx.x;
-  const int xcd = b & 7, local = b >> 3;
-  const int f = (local / n_tblk) * 8 + xcd;
-  const int tblk = local % n_tblk;
-  if (f >= M_dim) return;
-  const int slot = tblk * 64 + threadIdx.x;
-  if (slot >= n_slots) return;
-  constexpr int NR = 8 * E1, NS = 8 * E0;
-  Real yr[NR], yi[NR];
+// Frequency-domain M2L as a real GEMM per frequency f:
+//   Y[t-slots][f][16*E1 reals] = sum over the 26 parent offsets R of
+//   X[partner_R(t)][f][16*E0 reals] x B_R(f)[16*E0 x 16*E1],
+// the complex 8x8 (x E0,E1) transfer block in real form.  A wave owns 16*TG target slots at
+// one frequency; a block's 4 waves take 4 consecutive frequencies.  Blocks are mapped so
+// that all blocks of one frequency group run on one XCD (block b runs on XCD b % 8) and
+// share X[.][f] in that XCD's L2.  Vp holds B pre-permuted per lane: lane l's four values
+// for k-step q are B[16kb + 4(l/16) + q][16nb + l%16], so both operands load as 4-vectors.
+template <typename Real, int E0, int E1, int TG>
+__global__ __launch_bounds__(256) void k_hadamard(int n_slots, int M_dim, int n_tsg, const int32_t* __restrict__ partner,
+                                                  const int32_t* __restrict__ offsets, int n_off, const Real* __restrict__ Vp,
+                                                  const Real* __restrict__ X, Real* __restrict__ Y) {
+  using M = Mfma<Real>;
+  using V4 = typename M::V4;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
+  const int fg = (local / n_tsg) * 8 + xcd, tsg = local % n_tsg;
+  const int f = fg * 4 + w;
+  if (f >= M_dim) return;  // whole wave
+  const int row = l & 15, kg = l >> 4;
+  V4 acc[TG][E1];
 #pragma unroll
-  for (int r = 0; r < NR; r++) yr[r] = yi[r] = 0;
+  for (int g = 0; g < TG; g++)
+#pragma unroll
+    for (int nb = 0; nb < E1; nb++) acc[g][nb] = V4{0, 0, 0, 0};
+  int slot[TG];
+  bool valid[TG];
+#pragma unroll
+  for (int g = 0; g < TG; g++) {
+    slot[g] = (tsg * TG + g) * 16 + row;
+    valid[g] = slot[g] < n_slots;
+  }
   for (int oi = 0; oi < n_off; oi++) {
-    const int p = partner[(int64_t)slot * n_off + oi];
-    if (p < 0) continue;
-    const int R = offsets[oi];
-    const Real* x = X + ((int64_t)p * M_dim + f) * (2 * NS);
-    Real xr[NS], xi[NS];
+    int p[TG];
+    bool mine = false;
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      xr[s] = x[2 * s];
-      xi[s] = x[2 * s + 1];
+    for (int g = 0; g < TG; g++) {
+      p[g] = valid[g] ? partner[(int64_t)slot[g] * n_off + oi] : -1;
+      mine |= p[g] >= 0;
     }
-    const Real* v = V + ((int64_t)R * M_dim + f) * (2 * NR * NS);
+    if (!__any(mine)) continue;  // no target of this wave has a partner at offset R
+    const int R = offsets[oi];
+    const Real* vb = Vp + (int64_t)((int64_t)R * M_dim + f) * (E0 * E1 * 256);
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
+    for (int kb = 0; kb < E0; kb++) {
+      V4 bq[E1];
 #pragma unroll
-      for (int s = 0; s < NS; s++) {
-        const Real vr = v[2 * (r * NS + s)], vi = v[2 * (r * NS + s) + 1];
-        yr[r] = fma(vr, xr[s], fma(-vi, xi[s], yr[r]));
-        yi[r] = fma(vr, xi[s], fma(vi, xr[s], yi[r]));
+      for (int nb = 0; nb < E1; nb++) bq[nb] = *reinterpret_cast<const V4*>(vb + ((kb * E1 + nb) * 64 + l) * 4);
+#pragma unroll
+      for (int g = 0; g < TG; g++) {
+        V4 a = V4{0, 0, 0, 0};
+        if (p[g] >= 0) a = *reinterpret_cast<const V4*>(X + ((int64_t)p[g] * M_dim + f) * (16 * E0) + 16 * kb + 4 * kg);
+#pragma unroll
+        for (int nb = 0; nb < E1; nb++) {
+          acc[g][nb] = M::mma(a[0], bq[nb][0], acc[g][nb]);
+          acc[g][nb] = M::mma(a[1], bq[nb][1], acc[g][nb]);
+          acc[g][nb] = M::mma(a[2], bq[nb][2], acc[g][nb]);
+          acc[g][nb] = M::mma(a[3], bq[nb][3], acc[g][nb]);
+        }
       }
     }
   }
-  Real* y = Y + ((int64_t)slot * M_dim + f) * (2 * NR);
 #pragma unroll
-  for (int r = 0; r < NR; r++) {
-    y[2 * r] = yr[r];
-    y[2 * r + 1] = yi[r];
-  }
+  for (int g = 0; g < TG; g++)
+#pragma unroll
+    for (int nb = 0; nb < E1; nb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int s = (tsg * TG + g) * 16 + M::row(l, r);
+        if (s < n_slots) Y[((int64_t)s * M_dim + f) * (16 * E1) + 16 * nb + row] = acc[g][nb][r];
+      }
 }
 
 // ----------------------------------------------------------------------------- launchers
@@ -673,12 +761,42 @@ void DeviceKernels<Real>::far_points(hipStream_t s, int kind, int n_rows, const 
 
 template <typename Real>
 void DeviceKernels<Real>::gemm_rows(hipStream_t s, int M, int N, int K, const Real* A, const int32_t* ra, int64_t lda,
-                                    const Real* B, Real* C, const int32_t* rc, int64_t ldc, const Real* alpha, bool accumulate) {
-  if (M == 0) return;
-  dim3 g((N + 63) / 64, (M + 63) / 64);
-  if (accumulate) hipLaunchKernelGGL((k_gemm_rows<Real, true>), g, dim3(256), 0, s, M, N, K, A, ra, lda, B, C, rc, ldc, alpha);
-  else hipLaunchKernelGGL((k_gemm_rows<Real, false>), g, dim3(256), 0, s, M, N, K, A, ra, lda, B, C, rc, ldc, alpha);
+                                    const Real* B, Real* C, const int32_t* rc, int64_t ldc, const Real* alpha, bool accumulate,
+                                    Real* work, int64_t work_elems) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  const int gx = (N + 63) / 64, gy = (M + 63) / 64;
+  // split K when the tile grid cannot fill the 256 CUs, in chunks of whole 16-wide k steps
+  int ksplit = 1;
+  const int blocks = gx * gy;
+  if (work && blocks < 512) {
+    ksplit = std::min((1024 + blocks - 1) / blocks, (K + 63) / 64);
+    while (ksplit > 1 && (int64_t)ksplit * M * N > work_elems) ksplit--;
+  }
+  const int kchunk = ((K + ksplit - 1) / ksplit + 15) / 16 * 16;
+  ksplit = (K + kchunk - 1) / kchunk;
+  dim3 g(gx, gy, ksplit);
+  if (ksplit > 1) {
+    hipLaunchKernelGGL((k_gemm_rows<Real, false, true>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, ldc, alpha, work);
+    HIP_LAUNCH_CHECK();
+    const int64_t tot = (int64_t)M * N;
+    if (accumulate)
+      hipLaunchKernelGGL((k_gemm_reduce<Real, true>), dim3(grid_for(tot, 256)), dim3(256), 0, s, M, N, ksplit, work, C, rc, ldc, alpha);
+    else
+      hipLaunchKernelGGL((k_gemm_reduce<Real, false>), dim3(grid_for(tot, 256)), dim3(256), 0, s, M, N, ksplit, work, C, rc, ldc, alpha);
+  } else if (accumulate) {
+    hipLaunchKernelGGL((k_gemm_rows<Real, true, false>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, ldc, alpha, work);
+  } else {
+    hipLaunchKernelGGL((k_gemm_rows<Real, false, false>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, ldc, alpha, work);
+  }
   HIP_LAUNCH_CHECK();
+}
+
+// children per block: all 8 when the LDS working set fits in 64 KiB, else fewer
+template <typename Real>
+static int fft_children_per_block(size_t per_child, size_t fixed) {
+  for (int cb = 8; cb > 1; cb >>= 1)
+    if (fixed + cb * per_child <= 64 * 1024) return cb;
+  return 1;
 }
 
 template <typename Real>
@@ -686,21 +804,29 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
                                  const int32_t* lattice, const Real* scal, Real* X) {
   if (n_slots == 0) return;
   const int n1 = 2 * m;
-  size_t lds = sizeof(Real) * (m * m * m + 2 * n1 + 2 * m * m + 2 * m * n1);
-  hipLaunchKernelGGL(k_fft_up<Real>, dim3(n_slots * 8 * e0), dim3(256), lds, s, first_child, up, K0, e0, m, lattice, scal, X);
+  const size_t per_child = sizeof(Real) * (m * m * m + 2 * m * m + 2 * m * n1), fixed = sizeof(Real) * 2 * n1;
+  const int cb = fft_children_per_block<Real>(per_child, fixed);
+  const size_t lds = fixed + cb * per_child;
+  hipLaunchKernelGGL(k_fft_up<Real>, dim3(n_slots * e0 * (8 / cb)), dim3(512), lds, s, first_child, up, K0, e0, m, cb, lattice,
+                     scal, X);
   HIP_LAUNCH_CHECK();
 }
 
 template <typename Real>
 void DeviceKernels<Real>::hadamard(hipStream_t s, int n_slots, int M_dim, int e0, int e1, const int32_t* partner,
-                                   const int32_t* offsets, int n_off, const Real* V, const Real* X, Real* Y) {
+                                   const int32_t* offsets, int n_off, const Real* Vp, const Real* X, Real* Y) {
   if (n_slots == 0) return;
-  const int n_tblk = (n_slots + 63) / 64;
-  const int64_t nb = (int64_t)n_tblk * ((M_dim + 7) / 8) * 8;
+  constexpr int TG = 2;
+  const int n_tsg = (n_slots + 16 * TG - 1) / (16 * TG);
+  const int n_fg = (M_dim + 3) / 4;
+  const int64_t nb = (int64_t)n_tsg * ((n_fg + 7) / 8) * 8;
+  if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   if (e0 == 1 && e1 == 1)
-    hipLaunchKernelGGL((k_hadamard<Real, 1, 1>), dim3((unsigned)nb), dim3(64), 0, s, n_slots, M_dim, n_tblk, partner, offsets, n_off, V, X, Y);
+    hipLaunchKernelGGL((k_hadamard<Real, 1, 1, TG>), dim3((unsigned)nb), dim3(256), 0, s, n_slots, M_dim, n_tsg, partner, offsets,
+                       n_off, Vp, X, Y);
   else if (e0 == 3 && e1 == 3)
-    hipLaunchKernelGGL((k_hadamard<Real, 3, 3>), dim3((unsigned)nb), dim3(64), 0, s, n_slots, M_dim, n_tblk, partner, offsets, n_off, V, X, Y);
+    hipLaunchKernelGGL((k_hadamard<Real, 3, 3, 1>), dim3((unsigned)((int64_t)((n_slots + 15) / 16) * ((n_fg + 7) / 8) * 8)),
+                       dim3(256), 0, s, n_slots, M_dim, (n_slots + 15) / 16, partner, offsets, n_off, Vp, X, Y);
   else
     throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   HIP_LAUNCH_CHECK();
@@ -710,10 +836,13 @@ template <typename Real>
 void DeviceKernels<Real>::ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn, int K1, int e1,
                                     int m, const int32_t* lattice, int ns, const Real* scal) {
   if (n_slots == 0) return;
-  if (ns > 8 * 256) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   const int n1 = 2 * m;
-  size_t lds = sizeof(Real) * (2 * n1 + 2 * n1 * n1 + 2 * n1 * m + 2 * m * m);
-  hipLaunchKernelGGL(k_ifft_down<Real>, dim3(n_slots * 8 * e1), dim3(256), lds, s, first_child, Y, dn, K1, e1, m, lattice, ns, scal);
+  const size_t per_child = sizeof(Real) * (2 * n1 * n1 + 2 * n1 * m + 2 * m * m), fixed = sizeof(Real) * 2 * n1;
+  const int cb = fft_children_per_block<Real>(per_child, fixed);
+  if (cb * ns > 24 * 512) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // register accumulators
+  const size_t lds = fixed + cb * per_child;
+  hipLaunchKernelGGL(k_ifft_down<Real>, dim3(n_slots * e1 * (8 / cb)), dim3(512), lds, s, first_child, Y, dn, K1, e1, m, cb, lattice,
+                     ns, scal);
   HIP_LAUNCH_CHECK();
 }
 

@@ -44,18 +44,20 @@ struct DeviceKernels {
                          const float* node_size, const Real* unit_surf, int ns, const Real* tx, int64_t n_trg,
                          const Real* dens, double* out, bool accumulate);
 
-  // C[rc[i]*ldc : +N] (+)= alpha[i] * A[ra[i]*lda : +K] * B[K x N]
+  // C[rc[i]*ldc : +N] (+)= alpha[i] * A[ra[i]*lda : +K] * B[K x N] on MFMA; `work` (work_elems
+  // Reals, may be null) enables a deterministic split-K for grids too small to fill the GPU.
   static void gemm_rows(hipStream_t s, int M, int N, int K, const Real* A, const int32_t* ra, int64_t lda,
                         const Real* B, Real* C, const int32_t* rc, int64_t ldc, const Real* alpha,
-                        bool accumulate);
+                        bool accumulate, Real* work = nullptr, int64_t work_elems = 0);
 
   // V-list: forward DFT of the 8 children's equivalent densities of each source parent.
   // X layout [slot][f][8*e0] complex.
   static void fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0,
                      int m, const int32_t* lattice, const Real* scal, Real* X);
-  // Y[slot][f][8*e1] = sum_R V[R][f] X[partner[slot][R]][f]
+  // Y[slot][f][8*e1] = sum_R V[R][f] X[partner[slot][R]][f] on MFMA; Vp is the transfer block
+  // in real form, pre-permuted per lane (layout [R][f][kb][nb][lane 64][4], see make_vp).
   static void hadamard(hipStream_t s, int n_slots, int M_dim, int e0, int e1, const int32_t* partner,
-                       const int32_t* offsets, int n_off, const Real* V, const Real* X, Real* Y);
+                       const int32_t* offsets, int n_off, const Real* Vp, const Real* X, Real* Y);
   // inverse DFT, pick the check-surface points, dn[child] += scal * value
   static void ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn,
                         int K1, int e1, int m, const int32_t* lattice, int ns, const Real* scal);
