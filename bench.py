@@ -82,6 +82,68 @@ def cpu_baseline(n_sample: int):
                       f"at N={n_sample}, best of 2 after 1 warm-up, {t:.2f} s per evaluation"}
 
 
+def read_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE runs, calibrated against a copy kernel of known size), or None if absent."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_config2.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch") and f"[{kernel}]" in line:
+            return float(line.split("=")[1])
+    return None
+
+
+def roofline(ph: dict, st: dict, m: int, real_bytes: int):
+    """Roofline of the dominant kernel of the mixed evaluator from algorithmic counts.
+
+    V-list Hadamard (target-stationary model): per (V1 pair, frequency) the source spectrum
+    16*k0 reals are read, per (target parent, frequency) 16*k1 reals are written, and the 26 V1
+    matrices (128*k0*k1 reals per frequency) are read once; it is HBM-bound in that model
+    (512*k0*k1 flop per pair and frequency is ~2.5 flop/byte).  U-list P2P: 16 flop per pair
+    for laplace_grad, VALU-bound; its bytes are each target leaf's points and outputs plus the
+    U sources it streams."""
+    n1 = 2 * m
+    M_dim = n1 * n1 * (n1 // 2 + 1)
+    dom = max(ph, key=ph.get)
+    t = ph[dom] * 1e-3
+    if dom == "V-List:Hadamard":
+        name = "k_hadamard<float,1,1,2>"
+        bytes_alg = (16.0 * M_dim * real_bytes * st["v_pairs"] + 16.0 * M_dim * real_bytes * st["v_trg_slots"]
+                     + 26.0 * M_dim * 128 * real_bytes)
+        flops = 512.0 * st["v_pairs"] * M_dim
+        achieved = bytes_alg / t / 1e9
+        traffic = read_traffic(name)
+        return {"kernel": name + " (V-list frequency-domain M2L)", "bound": "hbm", "achieved": achieved,
+                "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS, "traffic": traffic,
+                "ms_per_launch": ph[dom], "algorithmic_bytes": bytes_alg,
+                "work": f"{st['v_pairs']} V1 pairs x {M_dim} frequencies x 16 reals read + {st['v_trg_slots']} "
+                        f"target parents x {M_dim} x 16 reals written + 26 V1 matrices; {flops / t / 1e12:.1f} "
+                        f"TFLOP/s of {MI355X_FP32_MFMA_TFLOPS} FP32 MFMA"}
+    if dom == "U-List":
+        name = "k_p2p_points<float,1>"
+        flops = 16.0 * st["u_pairs"]
+        achieved = flops / t / 1e12
+        return {"kernel": name + " (U-list P2P)", "bound": "valu", "achieved": achieved, "peak": MI355X_FP32_MFMA_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_MFMA_TFLOPS, "traffic": read_traffic(name),
+                "ms_per_launch": ph[dom], "work": f"16 flop x {st['u_pairs']} pairs"}
+    return {"kernel": dom, "bound": "hbm", "achieved": None, "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": None,
+            "traffic": None, "ms_per_launch": ph[dom]}
+
+
+def copy_probe(nbytes: int, reps: int = 3):
+    """A plain device copy of known size, run under rocprofv3 --pmc to calibrate FETCH_SIZE and
+    WRITE_SIZE (their reading for nbytes read and nbytes written)."""
+    import torch
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").uniform_()
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: {reps} x {nbytes} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +152,9 @@ def main():
     ap.add_argument("--n", type=int, default=2097152)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=262144)
+    ap.add_argument("--prec", choices=["both", "mixed", "fp64"], default="both",
+                    help="evaluators to time (profiling runs); the JSON line needs both")
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the single-GPU MPI_COMM_SELF share); --gpus {args.gpus} is not "
@@ -102,6 +167,9 @@ def main():
     from pvfmm_amd import workloads as W
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     c = W.config2(args.n)
     n = len(c["src"])
     log(f"workload: {c['kernel']} N={n} m={c['m']} max_pts={c['max_pts']}")
@@ -117,7 +185,7 @@ def main():
     ud = oracle.direct(c["kernel"], c["src"], c["den"], c["src"][idx]).reshape(-1, 3)
 
     res = {}
-    for prec in ["fp64", "mixed"]:
+    for prec in (["fp64", "mixed"] if args.prec == "both" else [args.prec]):
         t0 = time.perf_counter()
         fmm = P.PtFMM(c["kernel"], c["m"], prec)
         t_ops = time.perf_counter() - t0
@@ -134,30 +202,12 @@ def main():
             f"operators {t_ops:.1f}s, setup {t_setup:.1f}s")
         log("   phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
 
+    if args.prec != "both":
+        return
     mixed, fp64 = res["mixed"], res["fp64"]
     st = mixed["stats"]
-    # dominant kernel of the mixed evaluator and its algorithmic work
-    ph = mixed["phases"]
-    dom = max(ph, key=ph.get)
     m = c["m"]
-    n1 = 2 * m
-    M_dim = n1 * n1 * (n1 // 2 + 1)
-    had_flops = 512.0 * st["v_pairs"] * M_dim  # include/fmm_pts.txx:3336 accounting, k0 = k1 = 1
-    if dom == "V-List:Hadamard":
-        achieved = had_flops / (ph[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_hadamard<float,1,1> (V-list frequency-domain M2L)", "bound": "mfma", "achieved": achieved,
-                "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_MFMA_TFLOPS,
-                "traffic": None, "ms_per_launch": ph[dom],
-                "work": f"512 flop x {st['v_pairs']} V1 pairs x {M_dim} frequencies per launch"}
-    elif dom == "U-List":
-        flops = 16.0 * st["u_pairs"]
-        achieved = flops / (ph[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_p2p_points<float,1> (U-list P2P)", "bound": "mfma", "achieved": achieved,
-                "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_MFMA_TFLOPS,
-                "traffic": None, "ms_per_launch": ph[dom], "work": f"16 flop x {st['u_pairs']} pairs"}
-    else:
-        roof = {"kernel": dom, "bound": "mfma", "achieved": None, "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                "frac": None, "traffic": None, "ms_per_launch": ph[dom]}
+    roof = roofline(mixed["phases"], st, m, real_bytes=4)
 
     cpu = None
     if not args.no_cpu:

@@ -88,7 +88,7 @@ class DevicePlan final : public DevicePlanBase {
   DBuf<Real> gemm_work_;  // split-K partial tiles of the small coarse-level GEMMs
   static constexpr int64_t kGemmWork = int64_t(1) << 22;
   int ru_ = 0, rd_ = 0;
-  DBuf<int32_t> lattice_;
+  DBuf<int32_t> lattice_, lattice_inv_;
   // S2M
   DBuf<int32_t> s2m_node_, s2m_ptr_, s2m_items_, s2m_orow_;
   DBuf<Real> s2m_alpha_;
@@ -205,6 +205,12 @@ class DevicePlan final : public DevicePlanBase {
     }
     unit_surf_.upload(conv(O_.unit_surf), &total);
     lattice_.upload(O_.lattice, &total);
+    {
+      std::vector<int32_t> inv((size_t)m_ * m_ * m_, -1);
+      for (int p = 0; p < ns_; p++)
+        inv[((size_t)O_.lattice[3 * p + 2] * m_ + O_.lattice[3 * p + 1]) * m_ + O_.lattice[3 * p]] = p;
+      lattice_inv_.upload(inv, &total);
+    }
 
     const double ts = ki.trg_scal + ki.src_scal;
     // ---- S2M rows: leaves holding sources (Source2UpSetup, include/fmm_pts.txx:2287-2290)
@@ -402,11 +408,11 @@ class DevicePlan final : public DevicePlanBase {
                    unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, dn_.p, x_orow_.p, K1_, true);
     mark(5);
     // V list
-    K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p, K0_, e0_, m_, lattice_.p, v_src_scal_.p, X_.p);
+    K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p, X_.p);
     mark(6);
     K::hadamard(st_, n_vtrg_, M_dim_, e0_, e1_, v_partner_.p, v_offsets_.p, n_off_, V_.p, X_.p, Y_.p);
     mark(7);
-    K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p, K1_, e1_, m_, lattice_.p, ns_, v_trg_scal_.p);
+    K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p, K1_, e1_, m_, lattice_.p, lattice_inv_.p, ns_, v_trg_scal_.p);
     mark(8);
     // D2D, top-down: 8 children rows += parent x [D2D_0 .. D2D_7]
     for (int l = 1; l <= max_depth_; l++) {

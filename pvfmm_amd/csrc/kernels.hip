@@ -13,6 +13,8 @@
 //    the inverse DFT evaluates only the check-surface points.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace pvfmm_amd {
@@ -601,6 +603,332 @@ __global__ __launch_bounds__(512) void k_ifft_down(const int32_t* __restrict__ f
   }
 }
 
+// ----------------------------------------------------------------------------- V list: DFTs, fixed m
+// Twiddles w^k = exp(2 pi i k / N) as compile-time constants (Taylor series on the angle
+// reduced to [0, pi/4], 14 terms: below double rounding), so fully unrolled tap loops carry
+// them as literals.
+constexpr double ct_sin_small(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 14; k++) { term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0)); sum += term; }
+  return sum;
+}
+constexpr double ct_cos_small(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; k++) { term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k)); sum += term; }
+  return sum;
+}
+// cos or sin of 2 pi k / n: quadrant q, remainder angle (pi/2) r/n folded into [0, pi/4]
+constexpr double ct_cs(int k, int n, bool want_sin) {
+  k %= n;
+  if (k < 0) k += n;
+  const int q = (4 * k) / n, r = 4 * k - q * n;
+  const double half_pi = 1.57079632679489661923;
+  double c = 0, s = 0;
+  if (2 * r <= n) {
+    c = ct_cos_small(half_pi * r / n);
+    s = ct_sin_small(half_pi * r / n);
+  } else {
+    c = ct_sin_small(half_pi * (n - r) / n);
+    s = ct_cos_small(half_pi * (n - r) / n);
+  }
+  if (q == 1) return want_sin ? c : -s;
+  if (q == 2) return want_sin ? -s : -c;
+  if (q == 3) return want_sin ? -c : s;
+  return want_sin ? s : c;
+}
+template <int N>
+struct TwTable {
+  double c[N], s[N];
+  constexpr TwTable() : c(), s() {
+    for (int k = 0; k < N; k++) { c[k] = ct_cs(k, N, false); s[k] = ct_cs(k, N, true); }
+  }
+};
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i < N.
+template <int I, int N, typename F>
+__device__ __forceinline__ void sfor_(F& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    sfor_<I + 1, N>(f);
+  }
+}
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F f) { sfor_<0, N>(f); }
+
+// s += g * w^{SIGN K}, w = exp(2 pi i / N), K compile-time: twiddles +-1 and +-i become adds,
+// real or imaginary twiddles one FMA per part.
+template <int N, int K, int SIGN, typename Real>
+__device__ __forceinline__ void cmac_tw(Real gr, Real gi, Real& sr, Real& si) {
+  constexpr double c = ct_cs(K, N, false), s = SIGN * ct_cs(K, N, true);
+  if constexpr (s == 0.0) {
+    if constexpr (c == 1.0) { sr += gr; si += gi; }
+    else if constexpr (c == -1.0) { sr -= gr; si -= gi; }
+    else { sr = fma(gr, (Real)c, sr); si = fma(gi, (Real)c, si); }
+  } else if constexpr (c == 0.0) {
+    if constexpr (s == 1.0) { sr -= gi; si += gr; }
+    else if constexpr (s == -1.0) { sr += gi; si -= gr; }
+    else { sr = fma(-gi, (Real)s, sr); si = fma(gr, (Real)s, si); }
+  } else {
+    sr = fma(gr, (Real)c, sr);
+    sr = fma(-gi, (Real)s, sr);
+    si = fma(gr, (Real)s, si);
+    si = fma(gi, (Real)c, si);
+  }
+}
+
+// Shape of the fixed-m V-list transforms.  Lattice lines (c, y, z) are owned by threads:
+// NT = m^2 * CG threads, thread t -> (y, z) = t % m^2 and children cg + CG*q (q < LPT).
+// KC x-frequency planes of the intermediate are staged in LDS at a time, sized so that two
+// blocks share one CU's 160 KiB.
+template <typename Real, int M>
+struct DftShape {
+  static constexpr int N1 = 2 * M, NH = M + 1, MD = N1 * N1 * NH, M2 = M * M;
+  static constexpr int CG = (8 * M2 <= 512) ? 8 : (4 * M2 <= 512) ? 4 : (2 * M2 <= 512) ? 2 : 1;
+  static constexpr int NT = M2 * CG;
+  static constexpr int LPT = 8 / CG;
+  static constexpr int kc_for(int per_plane) {
+    const int budget = 78 * 1024;
+    const int kmax = budget / per_plane < 1 ? 1 : (budget / per_plane > NH ? NH : budget / per_plane);
+    const int nch = (NH + kmax - 1) / kmax;
+    return (NH + nch - 1) / nch;
+  }
+  static constexpr int KC_UP = kc_for(8 * M2 * 2 * (int)sizeof(Real));       // s1[c][kx][z][y]
+  static constexpr int KC_DN = kc_for(8 * N1 * M * 2 * (int)sizeof(Real));   // s1[c][kx][ky][z]
+  // FP64 recomputes the y stage per half (ky, ky+m) instead of holding both in registers
+  static constexpr bool SPLIT_H = sizeof(Real) == 8;
+};
+
+// Forward DFT of the 8 children of one source parent, component i (grid n_slots * e0).
+// x stage (real -> complex, kx < m+1) on thread-owned lattice lines into LDS; then one task
+// per (c, kx, ky < m) runs the y stage for ky and ky+m (w^{-(ky+m)y} = (-1)^y w^{-ky y}) and the
+// z stage of both (X[kz] = E + O, X[kz+m] = E - O over even/odd z), writing the spectra as
+// X[slot][f][c*e0+i] complex, f = kx + (m+1)(ky + 2m kz).
+template <typename Real, int M>
+__global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const int32_t* __restrict__ first_child,
+                                                                         const Real* __restrict__ up, int K0, int e0,
+                                                                         const int32_t* __restrict__ lat_inv,
+                                                                         const Real* __restrict__ scal, Real* __restrict__ X) {
+  using S = DftShape<Real, M>;
+  constexpr int N1 = S::N1, NH = S::NH, MD = S::MD, M2 = S::M2, NT = S::NT, LPT = S::LPT, CG = S::CG, KC = S::KC_UP;
+  constexpr int CST = KC * M2 + 1;  // child stride (odd, spreads the children over LDS banks)
+  constexpr TwTable<N1> TW{};
+  __shared__ Real twc[N1], tws[N1];
+  __shared__ Real s1r[8 * CST], s1i[8 * CST];
+  const int slot = blockIdx.x / e0, i = blockIdx.x % e0;
+  const int fc = first_child[slot];
+  const Real sc = scal[slot];
+  const int t = threadIdx.x, yz = t % M2, cg = t / M2, y = yz % M, z = yz / M;
+  if (t < N1) { twc[t] = (Real)TW.c[t]; tws[t] = (Real)TW.s[t]; }
+  Real v[LPT][M];  // surface values on the owned lines (zero inside the cube)
+#pragma unroll
+  for (int x = 0; x < M; x++) {
+    const int p = lat_inv[(z * M + y) * M + x];
+#pragma unroll
+    for (int q = 0; q < LPT; q++) v[q][x] = p >= 0 ? up[(int64_t)(fc + cg + CG * q) * K0 + p * e0 + i] * sc : Real(0);
+  }
+  __syncthreads();
+  Real* Xs = X + ((int64_t)slot * MD * 8 * e0 + i) * 2;
+  for (int k0 = 0; k0 < NH; k0 += KC) {
+    const int kn = (NH - k0 < KC) ? NH - k0 : KC;
+#pragma unroll
+    for (int q = 0; q < LPT; q++) {  // x stage
+      const int c = cg + CG * q;
+      for (int kl = 0; kl < kn; kl++) {
+        const int kx = k0 + kl;
+        Real ar = 0, ai = 0;
+#pragma unroll
+        for (int x = 0, e = 0; x < M; x++) {
+          ar = fma(v[q][x], twc[e], ar);
+          ai = fma(v[q][x], -tws[e], ai);
+          e += kx;
+          if (e >= N1) e -= N1;
+        }
+        s1r[c * CST + kl * M2 + yz] = ar;
+        s1i[c * CST + kl * M2 + yz] = ai;
+      }
+    }
+    __syncthreads();
+    const int ntask = 8 * M * kn;
+    for (int T = t; T < ntask; T += NT) {  // y and z stages
+      const int c = T % 8, ky = (T / 8) % M, kl = T / (8 * M), kx = k0 + kl;
+      const Real* pr = &s1r[c * CST + kl * M2];
+      const Real* pi = &s1i[c * CST + kl * M2];
+      // z stage of one y-stage output column g (z < m) for frequency row kyy
+      auto zstage = [&](const Real* gr, const Real* gi, int kyy) __attribute__((always_inline)) {
+        sfor<M>([&](auto ikz) __attribute__((always_inline)) {
+          constexpr int kz = decltype(ikz)::value;
+          Real er = 0, ei = 0, orr = 0, oi = 0;
+          sfor<M>([&](auto izz) __attribute__((always_inline)) {
+            constexpr int zz = decltype(izz)::value;
+            if constexpr (zz % 2 == 0)
+              cmac_tw<N1, (kz * zz) % N1, -1>(gr[zz], gi[zz], er, ei);
+            else
+              cmac_tw<N1, (kz * zz) % N1, -1>(gr[zz], gi[zz], orr, oi);
+          });
+          const int64_t f0 = kx + (int64_t)NH * (kyy + N1 * kz), f1 = kx + (int64_t)NH * (kyy + N1 * (kz + M));
+          // an opaque copy of the base keeps the 64-bit store addresses from being hoisted
+          // out of the unrolled loop (they would not fit in registers for FP64)
+          Real* xb = Xs;
+          asm volatile("" : "+s"(xb));
+          Real* d0 = xb + (f0 * 8 * e0 + c * e0) * 2;
+          Real* d1 = xb + (f1 * 8 * e0 + c * e0) * 2;
+          d0[0] = er + orr; d0[1] = ei + oi;
+          d1[0] = er - orr; d1[1] = ei - oi;
+        });
+      };
+      // y stage: column zz of ky (E + O) and of ky + m (E - O)
+      auto ystage = [&](int zz, Real& er, Real& ei, Real& orr, Real& oi) __attribute__((always_inline)) {
+        er = ei = orr = oi = 0;
+#pragma unroll
+        for (int yy = 0, e = 0; yy < M; yy++) {
+          const Real dr = pr[zz * M + yy], di = pi[zz * M + yy];
+          const Real wc = twc[e], ws = -tws[e];
+          const Real tr = dr * wc - di * ws, ti = dr * ws + di * wc;
+          if (yy % 2 == 0) { er += tr; ei += ti; } else { orr += tr; oi += ti; }
+          e += ky;
+          if (e >= N1) e -= N1;
+        }
+      };
+      if constexpr (S::SPLIT_H) {
+        Real gr[M], gi[M];
+#pragma unroll
+        for (int zz = 0; zz < M; zz++) {
+          Real er, ei, orr, oi;
+          ystage(zz, er, ei, orr, oi);
+          gr[zz] = er + orr; gi[zz] = ei + oi;
+        }
+        zstage(gr, gi, ky);
+#pragma unroll
+        for (int zz = 0; zz < M; zz++) {
+          Real er, ei, orr, oi;
+          ystage(zz, er, ei, orr, oi);
+          gr[zz] = er - orr; gi[zz] = ei - oi;
+        }
+        zstage(gr, gi, ky + M);
+      } else {
+        Real ar[M], ai[M], br[M], bi[M];
+#pragma unroll
+        for (int zz = 0; zz < M; zz++) {
+          Real er, ei, orr, oi;
+          ystage(zz, er, ei, orr, oi);
+          ar[zz] = er + orr; ai[zz] = ei + oi;
+          br[zz] = er - orr; bi[zz] = ei - oi;
+        }
+        zstage(ar, ai, ky);
+        zstage(br, bi, ky + M);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Inverse DFT of the 8 children of one target parent, component j (grid n_slots * e1),
+// evaluated at the check-surface lattice points only.  z stage per (c, kx, ky) task straight
+// from Y (output z < m needs P = Y[a] + Y[a+m] for even z, Q = Y[a] - Y[a+m] for odd z) into
+// LDS; then each thread runs the y stage
+// of its own lines (c, y, z) and accumulates the c2r x sum in registers; the lattice values
+// are added into dn at the end.
+template <typename Real, int M>
+__global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_ifft_down_m(const int32_t* __restrict__ first_child,
+                                                                            const Real* __restrict__ Y, Real* __restrict__ dn,
+                                                                            int K1, int e1, const int32_t* __restrict__ lat_inv,
+                                                                            const Real* __restrict__ scal) {
+  using S = DftShape<Real, M>;
+  constexpr int N1 = S::N1, NH = S::NH, MD = S::MD, M2 = S::M2, NT = S::NT, LPT = S::LPT, CG = S::CG, KC = S::KC_DN;
+  constexpr int PL = N1 * M;         // one plane of one child: [ky][z]
+  constexpr int CST = KC * PL + 1;   // child stride (odd)
+  constexpr TwTable<N1> TW{};
+  __shared__ Real twc[N1], tws[N1];
+  __shared__ Real s1r[8 * CST], s1i[8 * CST];
+  const int slot = blockIdx.x / e1, j = blockIdx.x % e1;
+  const int fc = first_child[slot];
+  const int t = threadIdx.x, yz = t % M2, cg = t / M2, y = yz % M, z = yz / M;
+  if (t < N1) { twc[t] = (Real)TW.c[t]; tws[t] = (Real)TW.s[t]; }
+  __syncthreads();
+  const Real sgn = (y & 1) ? Real(-1) : Real(1);
+  Real acc[LPT][M];
+#pragma unroll
+  for (int q = 0; q < LPT; q++)
+#pragma unroll
+    for (int x = 0; x < M; x++) acc[q][x] = 0;
+  const Real* Ys = Y + ((int64_t)slot * MD * 8 * e1 + j) * 2;
+  for (int k0 = 0; k0 < NH; k0 += KC) {
+    const int kn = (NH - k0 < KC) ? NH - k0 : KC;
+    const int ntask = 8 * N1 * kn;
+    for (int T = t; T < ntask; T += NT) {  // z stage
+      const int c = T % 8, ky = (T / 8) % N1, kl = T / (8 * N1), kx = k0 + kl;
+      Real pr_[M], pi_[M], qr_[M], qi_[M];
+#pragma unroll
+      for (int a = 0; a < M; a++) {
+        const Real* s0 = Ys + ((kx + (int64_t)NH * (ky + N1 * a)) * 8 * e1 + c * e1) * 2;
+        const Real* s1 = Ys + ((kx + (int64_t)NH * (ky + N1 * (a + M))) * 8 * e1 + c * e1) * 2;
+        const Real ur = s0[0], ui = s0[1], vr = s1[0], vi = s1[1];
+        pr_[a] = ur + vr; pi_[a] = ui + vi;
+        qr_[a] = ur - vr; qi_[a] = ui - vi;
+      }
+      Real* dr = &s1r[c * CST + kl * PL + ky * M];
+      Real* di = &s1i[c * CST + kl * PL + ky * M];
+      sfor<M>([&](auto izz) __attribute__((always_inline)) {
+        constexpr int zz = decltype(izz)::value;
+        Real sr = 0, si = 0;
+        sfor<M>([&](auto ia) __attribute__((always_inline)) {
+          constexpr int a = decltype(ia)::value;
+          if constexpr (zz % 2 == 0)
+            cmac_tw<N1, (a * zz) % N1, 1>(pr_[a], pi_[a], sr, si);
+          else
+            cmac_tw<N1, (a * zz) % N1, 1>(qr_[a], qi_[a], sr, si);
+        });
+        dr[zz] = sr;
+        di[zz] = si;
+      });
+    }
+    __syncthreads();
+    for (int kl = 0; kl < kn; kl++) {  // y stage and c2r x sum of the owned lines
+      const int kx = k0 + kl;
+      const Real wk = (kx == 0 || 2 * kx == N1) ? Real(1) : Real(2);
+      Real sr[LPT], si[LPT];
+#pragma unroll
+      for (int q = 0; q < LPT; q++) {
+        const int c = cg + CG * q;
+        const Real* pr = &s1r[c * CST + kl * PL + z];
+        const Real* pi = &s1i[c * CST + kl * PL + z];
+        Real ar = 0, ai = 0;
+#pragma unroll
+        for (int a = 0, e = 0; a < M; a++) {
+          const Real gr = pr[a * M] + sgn * pr[(a + M) * M];
+          const Real gi = pi[a * M] + sgn * pi[(a + M) * M];
+          const Real wc = twc[e], ws = tws[e];
+          ar += gr * wc - gi * ws;
+          ai += gr * ws + gi * wc;
+          e += y;
+          if (e >= N1) e -= N1;
+        }
+        sr[q] = wk * ar;
+        si[q] = wk * ai;
+      }
+#pragma unroll
+      for (int x = 0, e = 0; x < M; x++) {
+        const Real wc = twc[e], ws = tws[e];
+#pragma unroll
+        for (int q = 0; q < LPT; q++) acc[q][x] += sr[q] * wc - si[q] * ws;
+        e += kx;
+        if (e >= N1) e -= N1;
+      }
+    }
+    __syncthreads();
+  }
+  const Real sc = scal[slot];
+#pragma unroll
+  for (int q = 0; q < LPT; q++) {
+    const int c = cg + CG * q;
+#pragma unroll
+    for (int x = 0; x < M; x++) {
+      const int p = lat_inv[(z * M + y) * M + x];
+      if (p >= 0) dn[(int64_t)(fc + c) * K1 + p * e1 + j] += acc[q][x] * sc;
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------- V list: Hadamard
 // Frequency-domain M2L as a real GEMM per frequency f:
 //   Y[t-slots][f][16*E1 reals] = sum over the 26 parent offsets R of
@@ -801,8 +1129,24 @@ static int fft_children_per_block(size_t per_child, size_t fixed) {
 
 template <typename Real>
 void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0, int m,
-                                 const int32_t* lattice, const Real* scal, Real* X) {
+                                 const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X) {
   if (n_slots == 0) return;
+  const dim3 g((unsigned)(n_slots * e0));
+#define PVFMM_FFT_UP_M(MM)                                                                                        \
+  case MM:                                                                                                        \
+    hipLaunchKernelGGL((k_fft_up_m<Real, MM>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, up, K0, e0, lat_inv, \
+                       scal, X);                                                                                  \
+    HIP_LAUNCH_CHECK();                                                                                           \
+    return;
+  switch (m) {
+    PVFMM_FFT_UP_M(4)
+    PVFMM_FFT_UP_M(6)
+    PVFMM_FFT_UP_M(8)
+    PVFMM_FFT_UP_M(10)
+    PVFMM_FFT_UP_M(12)
+    default: break;
+  }
+#undef PVFMM_FFT_UP_M
   const int n1 = 2 * m;
   const size_t per_child = sizeof(Real) * (m * m * m + 2 * m * m + 2 * m * n1), fixed = sizeof(Real) * 2 * n1;
   const int cb = fft_children_per_block<Real>(per_child, fixed);
@@ -834,8 +1178,24 @@ void DeviceKernels<Real>::hadamard(hipStream_t s, int n_slots, int M_dim, int e0
 
 template <typename Real>
 void DeviceKernels<Real>::ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn, int K1, int e1,
-                                    int m, const int32_t* lattice, int ns, const Real* scal) {
+                                    int m, const int32_t* lattice, const int32_t* lat_inv, int ns, const Real* scal) {
   if (n_slots == 0) return;
+  const dim3 g((unsigned)(n_slots * e1));
+#define PVFMM_IFFT_DN_M(MM)                                                                                       \
+  case MM:                                                                                                        \
+    hipLaunchKernelGGL((k_ifft_down_m<Real, MM>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, Y, dn, K1, e1,     \
+                       lat_inv, scal);                                                                            \
+    HIP_LAUNCH_CHECK();                                                                                           \
+    return;
+  switch (m) {
+    PVFMM_IFFT_DN_M(4)
+    PVFMM_IFFT_DN_M(6)
+    PVFMM_IFFT_DN_M(8)
+    PVFMM_IFFT_DN_M(10)
+    PVFMM_IFFT_DN_M(12)
+    default: break;
+  }
+#undef PVFMM_IFFT_DN_M
   const int n1 = 2 * m;
   const size_t per_child = sizeof(Real) * (2 * n1 * n1 + 2 * n1 * m + 2 * m * m), fixed = sizeof(Real) * 2 * n1;
   const int cb = fft_children_per_block<Real>(per_child, fixed);

@@ -51,16 +51,18 @@ struct DeviceKernels {
                         bool accumulate, Real* work = nullptr, int64_t work_elems = 0);
 
   // V-list: forward DFT of the 8 children's equivalent densities of each source parent.
-  // X layout [slot][f][8*e0] complex.
+  // X layout [slot][f][8*e0] complex.  lattice[3p..3p+2] is the m^3 lattice cell of surface
+  // point p; lat_inv[(z*m + y)*m + x] is its inverse (-1 inside the cube).
   static void fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0,
-                     int m, const int32_t* lattice, const Real* scal, Real* X);
+                     int m, const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X);
   // Y[slot][f][8*e1] = sum_R V[R][f] X[partner[slot][R]][f] on MFMA; Vp is the transfer block
   // in real form, pre-permuted per lane (layout [R][f][kb][nb][lane 64][4], see make_vp).
   static void hadamard(hipStream_t s, int n_slots, int M_dim, int e0, int e1, const int32_t* partner,
                        const int32_t* offsets, int n_off, const Real* Vp, const Real* X, Real* Y);
   // inverse DFT, pick the check-surface points, dn[child] += scal * value
   static void ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn,
-                        int K1, int e1, int m, const int32_t* lattice, int ns, const Real* scal);
+                        int K1, int e1, int m, const int32_t* lattice, const int32_t* lat_inv, int ns,
+                        const Real* scal);
 };
 
 }  // namespace pvfmm_amd
