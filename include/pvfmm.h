@@ -128,8 +128,8 @@ int pvfmm_amd_tree_phase_times(void* tree, double* ms, int cap);
 const char* pvfmm_amd_phase_name(int phase);
 /* plan statistics: n_nodes, n_leaves, max_depth, s2m_rows, l2t_rows, m2m_rows, l2l_rows, x_rows,
  * x_items, v_src_slots, v_trg_slots, v_pairs, u_rows, u_items, u_pairs, far_items, far_pairs,
- * s2m_pairs, u_src_points_streamed, device_bytes */
-int pvfmm_amd_tree_stats(void* tree, int64_t* out20);
+ * s2m_pairs, u_src_points_streamed, device_bytes, v_tiles, v_tile_partners */
+int pvfmm_amd_tree_stats(void* tree, int64_t* out22);
 
 /* Operator matrices of an fmm handle (FP64), for parity checks against the oracle:
  * which 0 uc2ue0 (V S^+), 1 uc2ue1 (U^T), 2 dc2de0, 3 dc2de1, 4 u2u (8 stacked),

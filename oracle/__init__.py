@@ -12,7 +12,16 @@ Split:
     (include/fmm_pts.txx:453-764) written with numpy/LAPACK, so that the product's own
     Jacobi-SVD pseudo-inverses and DFTs are checked against an independent implementation.
 
-Parity status of each part is documented in DESIGN.md ("Oracle and what pins it").
+What pins each part (tests/test_oracle.py unless noted):
+  * point kernels: closed forms, a finite-difference gradient, translation invariance;
+  * DFT: numpy FFT; M2L via FFT == dense equivalent->check interaction (exact identity);
+  * pseudo-inverses: the C2E*E2C projector property;
+  * tree and lists: brute-force geometric list definitions, 2:1 balance and tiling
+    invariants, reference table counts (include/interac_list.txx:39-56);
+  * whole FMM: relative L2 against an FP64 direct sum, convergence in m.
+Bitwise agreement with the reference's floating point (sctl::approx_rsqrt, LAPACK SVD signs,
+OpenMP summation order) is not pinned: SCTL is not vendored in the reference checkout, so no
+reference build exists to compare with.
 """
 from __future__ import annotations
 

@@ -380,10 +380,10 @@ int pvfmm_amd_tree_stats(void* tree, int64_t* o) {
     TreeHandle* t = (TreeHandle*)tree;
     if (!t->plan) throw Error(kErrState, "no plan");
     const PlanStats& s = t->plan->stats;
-    int64_t v[20] = {s.n_nodes, s.n_leaves, s.max_depth, s.s2m_rows, s.l2t_rows, s.m2m_rows, s.l2l_rows, s.x_rows, s.x_items,
+    int64_t v[22] = {s.n_nodes, s.n_leaves, s.max_depth, s.s2m_rows, s.l2t_rows, s.m2m_rows, s.l2l_rows, s.x_rows, s.x_items,
                      s.v_src_slots, s.v_trg_slots, s.v_pairs, s.u_rows, s.u_items, s.u_pairs, s.far_items, s.far_pairs,
-                     s.s2m_pairs, s.u_src_points_streamed, s.device_bytes};
-    std::copy(v, v + 20, o);
+                     s.s2m_pairs, s.u_src_points_streamed, s.device_bytes, s.v_tiles, s.v_tile_partners};
+    std::copy(v, v + 22, o);
   });
 }
 int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out) {

@@ -103,6 +103,10 @@ class DevicePlan final : public DevicePlanBase {
   DBuf<int32_t> v_src_fc_, v_trg_fc_, v_partner_, v_offsets_;
   DBuf<Real> v_src_scal_, v_trg_scal_, X_, Y_;
   int n_vsrc_ = 0, n_vtrg_ = 0, n_off_ = 0;
+  // V-list target tiles for the LDS-staged Hadamard
+  DBuf<int32_t> vt_t0_, vt_u0_, vt_usrc_;
+  DBuf<uint16_t> vt_uidx_;
+  int n_vtiles_ = 0;
   // L2T
   DBuf<int32_t> l2t_ra_, l2t_rc_, l2t_seq_;
   DBuf<Real> l2t_alpha_;
@@ -325,6 +329,56 @@ class DevicePlan final : public DevicePlanBase {
     v_src_fc_.upload(v_src_fc, &total); v_trg_fc_.upload(v_trg_fc, &total);
     v_src_scal_.upload(v_src_scal, &total); v_trg_scal_.upload(v_trg_scal, &total);
     v_partner_.upload(v_partner, &total); v_offsets_.upload(offs, &total);
+    {
+      // Tiles of consecutive target parents: at most 64 targets, at most umax distinct partners
+      // (greedy in slot order, which is Morton order within a level).
+      const int umax = K::hadamard_union_max(e0_);
+      std::vector<int32_t> t0{0}, u0{0}, usrc;
+      std::vector<uint16_t> uidx;
+      std::vector<int32_t> pos(n_vsrc_, -1);
+      std::vector<int32_t> cur;  // distinct partners of the open tile
+      auto close_tile = [&](int t_end) {
+        for (int32_t sl : cur) pos[sl] = -1;
+        usrc.insert(usrc.end(), cur.begin(), cur.end());
+        cur.clear();
+        t0.push_back(t_end);
+        u0.push_back((int32_t)usrc.size());
+      };
+      // first pass: tile boundaries and union lists
+      for (int t = 0; t < n_vtrg_; t++) {
+        int add = 0;
+        for (int k = 0; k < n_off_; k++) {
+          const int32_t sl = v_partner[(size_t)t * n_off_ + k];
+          if (sl >= 0 && pos[sl] < 0) { pos[sl] = -2; add++; }  // provisional mark
+        }
+        for (int k = 0; k < n_off_; k++) {
+          const int32_t sl = v_partner[(size_t)t * n_off_ + k];
+          if (sl >= 0 && pos[sl] == -2) pos[sl] = -1;
+        }
+        if (t > t0.back() && (t - t0.back() == 64 || (int)cur.size() + add > umax)) close_tile(t);
+        for (int k = 0; k < n_off_; k++) {
+          const int32_t sl = v_partner[(size_t)t * n_off_ + k];
+          if (sl >= 0 && pos[sl] < 0) { pos[sl] = (int32_t)cur.size(); cur.push_back(sl); }
+        }
+      }
+      if (n_vtrg_ > t0.back()) close_tile(n_vtrg_);
+      n_vtiles_ = (int)t0.size() - 1;
+      uidx.assign((size_t)n_vtiles_ * 64 * n_off_, 0xFFFF);
+      for (int k = 0; k < n_vtiles_; k++) {
+        for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = u - u0[k];
+        for (int t = t0[k]; t < t0[k + 1]; t++)
+          for (int o = 0; o < n_off_; o++) {
+            const int32_t sl = v_partner[(size_t)t * n_off_ + o];
+            if (sl >= 0) uidx[((size_t)k * 64 + (t - t0[k])) * n_off_ + o] = (uint16_t)pos[sl];
+          }
+        for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = -1;
+        if (u0[k + 1] - u0[k] > umax || t0[k + 1] - t0[k] > 64)
+          throw Error(kErrInternal, "V-list tile exceeds its staging capacity");
+      }
+      stats.v_tiles = n_vtiles_;
+      stats.v_tile_partners = (int64_t)usrc.size();
+      vt_t0_.upload(t0, &total); vt_u0_.upload(u0, &total); vt_usrc_.upload(usrc, &total); vt_uidx_.upload(uidx, &total);
+    }
     X_.alloc((int64_t)n_vsrc_ * M_dim_ * 16 * e0_, &total);
     Y_.alloc((int64_t)n_vtrg_ * M_dim_ * 16 * e1_, &total);
 
@@ -410,7 +464,8 @@ class DevicePlan final : public DevicePlanBase {
     // V list
     K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p, X_.p);
     mark(6);
-    K::hadamard(st_, n_vtrg_, M_dim_, e0_, e1_, v_partner_.p, v_offsets_.p, n_off_, V_.p, X_.p, Y_.p);
+    K::hadamard_tiled(st_, n_vtiles_, M_dim_, e0_, e1_, n_off_, vt_t0_.p, vt_u0_.p, vt_usrc_.p, vt_uidx_.p, v_offsets_.p, V_.p,
+                      X_.p, Y_.p);
     mark(7);
     K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p, K1_, e1_, m_, lattice_.p, lattice_inv_.p, ns_, v_trg_scal_.p);
     mark(8);

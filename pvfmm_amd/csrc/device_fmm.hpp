@@ -35,6 +35,7 @@ struct PlanStats {
   int64_t u_rows = 0, u_items = 0, u_pairs = 0, far_items = 0, far_pairs = 0, s2m_pairs = 0;
   int64_t u_src_points_streamed = 0;
   int64_t device_bytes = 0;
+  int64_t v_tiles = 0, v_tile_partners = 0;
 };
 
 class DevicePlanBase {

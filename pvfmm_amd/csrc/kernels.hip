@@ -1003,6 +1003,105 @@ __global__ __launch_bounds__(256) void k_hadamard(int n_slots, int M_dim, int n_
       }
 }
 
+// Tiled frequency-domain M2L.  A block owns one tile of consecutive target parents (at most
+// 64 = 4 groups of 16, with at most UMAX distinct V1 partners) and FB consecutive
+// frequencies.  The partners' spectra for those frequencies are staged in LDS once (entry
+// stride padded by one 4-vector against bank conflicts); each wave then runs the 26 parent
+// offsets for one frequency over TG target groups, with the transfer block of the offset as
+// the shared MFMA B operand and the outputs accumulated in registers.  Blocks are mapped so
+// that all tiles of one frequency group run on one XCD and share its L2.
+template <typename Real, int E0, int E1, int FB, int TG, int UMAX>
+__global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, const int32_t* __restrict__ tile_t0,
+                                                        const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
+                                                        const uint16_t* __restrict__ uidx, const int32_t* __restrict__ offsets,
+                                                        const Real* __restrict__ Vp, const Real* __restrict__ X,
+                                                        Real* __restrict__ Y) {
+  using MF = Mfma<Real>;
+  using V4 = typename MF::V4;
+  constexpr int NOFF = 26;                        // V1 parent offsets
+  constexpr int RW = 16 * E0;                     // reals of one (slot, frequency) spectrum row
+  constexpr int PAD = 4;                          // one 4-vector of pad per staged entry
+  constexpr int EST = FB * RW + PAD;              // staged entry stride (reals)
+  constexpr int WPF = 4 / FB;                     // waves per frequency
+  constexpr int NB = E0 * E1;                     // 4-vectors of one offset's transfer block per lane
+  constexpr int PD = NB == 1 ? 4 : 2;             // transfer-block prefetch depth (offsets)
+  static_assert(WPF * TG == 4, "a tile has 4 target groups of 16");
+  constexpr uint16_t NONE = 0xFFFF;
+  __shared__ __align__(16) Real Xl[(UMAX + 1) * EST];  // entry UMAX stays zero
+  __shared__ uint16_t ul[64 * NOFF];
+  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
+  const int fg = (local / n_tiles) * 8 + xcd, tile = local % n_tiles;
+  const int f0 = fg * FB;
+  if (f0 >= M_dim) return;  // whole block
+  const int t0 = tile_t0[tile], nt = tile_t0[tile + 1] - t0;
+  const int u0 = tile_u0[tile], nu = tile_u0[tile + 1] - u0;
+  const int fn = min(FB, M_dim - f0);
+  constexpr int V4PU = FB * RW / 4;  // 4-vectors per entry
+  for (int e = threadIdx.x; e < nu * V4PU; e += 256) {
+    const int u = e / V4PU, r = e - u * V4PU, fl = r / (RW / 4);
+    V4 val = V4{0, 0, 0, 0};
+    if (fl < fn) val = *reinterpret_cast<const V4*>(X + ((int64_t)usrc[u0 + u] * M_dim + f0) * RW + 4 * r);
+    *reinterpret_cast<V4*>(&Xl[u * EST + 4 * r]) = val;
+  }
+  for (int e = threadIdx.x; e < 64 * NOFF; e += 256) ul[e] = uidx[(int64_t)tile * 64 * NOFF + e];
+  for (int e = threadIdx.x; e < EST; e += 256) Xl[UMAX * EST + e] = Real(0);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int fl = w / WPF, gsub = w % WPF, f = f0 + (fl < fn ? fl : 0);
+  // transfer blocks of the first PD offsets, in flight during the staging barrier
+  const Real* vf = Vp + (int64_t)f * (NB * 256) + l * 4;
+  const int64_t vstride = (int64_t)M_dim * NB * 256;
+  V4 bq[PD][NB];
+#pragma unroll
+  for (int p = 0; p < PD; p++)
+#pragma unroll
+    for (int q = 0; q < NB; q++) bq[p][q] = *reinterpret_cast<const V4*>(vf + offsets[p] * vstride + q * 256);
+  __syncthreads();
+  if (fl >= fn) return;  // no barrier follows
+  const int row = l & 15, kg = l >> 4;
+  V4 acc[TG][E1];
+#pragma unroll
+  for (int g = 0; g < TG; g++)
+#pragma unroll
+    for (int nb = 0; nb < E1; nb++) acc[g][nb] = V4{0, 0, 0, 0};
+#pragma unroll
+  for (int oi = 0; oi < NOFF; oi++) {
+    V4 cur[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) cur[q] = bq[oi % PD][q];
+    if (oi + PD < NOFF) {
+#pragma unroll
+      for (int q = 0; q < NB; q++) bq[oi % PD][q] = *reinterpret_cast<const V4*>(vf + offsets[oi + PD] * vstride + q * 256);
+    }
+#pragma unroll
+    for (int g = 0; g < TG; g++) {
+      const int tg = (gsub * TG + g) * 16 + row;
+      const uint16_t u = ul[tg * NOFF + oi];  // rows past the tile's targets hold NONE
+      const int ue = (u == NONE) ? UMAX : (int)u;  // missing partner -> the zero entry
+#pragma unroll
+      for (int kb = 0; kb < E0; kb++) {
+        const V4 a = *reinterpret_cast<const V4*>(&Xl[ue * EST + fl * RW + 16 * kb + 4 * kg]);
+#pragma unroll
+        for (int nb = 0; nb < E1; nb++) {
+          const V4 bv = cur[kb * E1 + nb];
+          acc[g][nb] = MF::mma(a[0], bv[0], acc[g][nb]);
+          acc[g][nb] = MF::mma(a[1], bv[1], acc[g][nb]);
+          acc[g][nb] = MF::mma(a[2], bv[2], acc[g][nb]);
+          acc[g][nb] = MF::mma(a[3], bv[3], acc[g][nb]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < TG; g++)
+#pragma unroll
+    for (int nb = 0; nb < E1; nb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int tg = (gsub * TG + g) * 16 + MF::row(l, r);
+        if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * (16 * E1) + 16 * nb + row] = acc[g][nb][r];
+      }
+}
+
 // ----------------------------------------------------------------------------- launchers
 static inline int grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
@@ -1171,6 +1270,30 @@ void DeviceKernels<Real>::hadamard(hipStream_t s, int n_slots, int M_dim, int e0
   else if (e0 == 3 && e1 == 3)
     hipLaunchKernelGGL((k_hadamard<Real, 3, 3, 1>), dim3((unsigned)((int64_t)((n_slots + 15) / 16) * ((n_fg + 7) / 8) * 8)),
                        dim3(256), 0, s, n_slots, M_dim, (n_slots + 15) / 16, partner, offsets, n_off, Vp, X, Y);
+  else
+    throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+  HIP_LAUNCH_CHECK();
+}
+
+template <typename Real>
+int DeviceKernels<Real>::hadamard_union_max(int e0) { return e0 == 1 ? 256 : 80; }
+
+template <typename Real>
+void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
+                                         const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx,
+                                         const int32_t* offsets, const Real* Vp, const Real* X, Real* Y) {
+  if (n_tiles == 0) return;
+  if (n_off != 26) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // the 26 V1 parent offsets
+  constexpr int FB = sizeof(Real) == 4 ? 4 : 2, TG = 4 / (4 / FB);
+  const int n_fg = (M_dim + FB - 1) / FB;
+  const int64_t nb = (int64_t)n_tiles * ((n_fg + 7) / 8) * 8;
+  if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+  if (e0 == 1 && e1 == 1)
+    hipLaunchKernelGGL((k_hadamard_tiled<Real, 1, 1, FB, TG, 256>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
+                       tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
+  else if (e0 == 3 && e1 == 3)
+    hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, FB, TG, 80>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
+                       tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
   else
     throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   HIP_LAUNCH_CHECK();

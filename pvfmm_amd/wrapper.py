@@ -284,11 +284,11 @@ class PtFMMTree:
         return {ffi.lib().pvfmm_amd_phase_name(i).decode(): float(buf[i]) for i in range(n)}
 
     def stats(self) -> dict:
-        out = np.zeros(20, dtype=np.int64)
+        out = np.zeros(22, dtype=np.int64)
         ffi.check(ffi.lib().pvfmm_amd_tree_stats(self._h, out.ctypes.data_as(ffi.c_int64_p)))
         names = ["n_nodes", "n_leaves", "max_depth", "s2m_rows", "l2t_rows", "m2m_rows", "l2l_rows", "x_rows",
                  "x_items", "v_src_slots", "v_trg_slots", "v_pairs", "u_rows", "u_items", "u_pairs", "far_items",
-                 "far_pairs", "s2m_pairs", "u_src_points_streamed", "device_bytes"]
+                 "far_pairs", "s2m_pairs", "u_src_points_streamed", "device_bytes", "v_tiles", "v_tile_partners"]
         return dict(zip(names, out.tolist()))
 
 
