@@ -21,6 +21,7 @@
 #ifndef PVFMM_AMD_PVFMM_H
 #define PVFMM_AMD_PVFMM_H
 
+#include <stdbool.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -84,6 +85,39 @@ const char* PVFMMGetLastErrorMessage(void);
  * takes effect at the next setup.  Returns an error code. */
 int PVFMMSetMixedPrecision(void* ctx, int enable);
 
+/* Volume (Chebyshev) FMM, include/pvfmm.h:43-228 of the reference: not provided.  These
+ * entry points exist so that bindings which resolve every symbol when they load the library
+ * (python/src/pvfmm/ffi.py of the reference) can load this one; each records
+ * PVFMM_ERR_UNSUPPORTED and returns NULL, 0 or nothing (the Destroy calls set *ctx = NULL). */
+void* PVFMMCreateVolumeFMMD(int m, int q, enum PVFMMKernel kernel, pvfmm_comm_t comm);
+void* PVFMMCreateVolumeFMMF(int m, int q, enum PVFMMKernel kernel, pvfmm_comm_t comm);
+void* PVFMMCreateVolumeTreeD(int cheb_deg, int data_dim, void (*fn_ptr)(const double* coord, long n, double* out, const void* ctx),
+                             const void* fn_ctx, const double* trg_coord, long n_trg, pvfmm_comm_t comm, double tol,
+                             int max_pts, bool periodic, int init_depth);
+void* PVFMMCreateVolumeTreeF(int cheb_deg, int data_dim, void (*fn_ptr)(const float* coord, long n, float* out, const void* ctx),
+                             const void* fn_ctx, const float* trg_coord, long n_trg, pvfmm_comm_t comm, float tol, int max_pts,
+                             bool periodic, int init_depth);
+void* PVFMMCreateVolumeTreeFromCoeffD(long n_leaf, int cheb_deg, int data_dim, const double* leaf_coord, const double* fn_coeff,
+                                      const double* trg_coord, long n_trg, pvfmm_comm_t comm, bool periodic);
+void* PVFMMCreateVolumeTreeFromCoeffF(long n_leaf, int cheb_deg, int data_dim, const float* leaf_coord, const float* fn_coeff,
+                                      const float* trg_coord, long n_trg, pvfmm_comm_t comm, bool periodic);
+void PVFMMEvaluateVolumeFMMD(double* trg_value, void* tree, const void* fmm, long loc_size);
+void PVFMMEvaluateVolumeFMMF(float* trg_value, void* tree, const void* fmm, long loc_size);
+void PVFMMDestroyVolumeFMMD(void** ctx);
+void PVFMMDestroyVolumeFMMF(void** ctx);
+void PVFMMDestroyVolumeTreeD(void** ctx);
+void PVFMMDestroyVolumeTreeF(void** ctx);
+long PVFMMGetLeafCountD(const void* tree);
+long PVFMMGetLeafCountF(const void* tree);
+void PVFMMGetLeafCoordD(double* leaf_coord, const void* tree);
+void PVFMMGetLeafCoordF(float* leaf_coord, const void* tree);
+void PVFMMGetPotentialCoeffD(double* coeff, const void* tree);
+void PVFMMGetPotentialCoeffF(float* coeff, const void* tree);
+void PVFMMCoeff2NodesD(double* node_val, long n_leaf, int cheb_deg, int dof, const double* coeff);
+void PVFMMCoeff2NodesF(float* node_val, long n_leaf, int cheb_deg, int dof, const float* coeff);
+void PVFMMNodes2CoeffD(double* coeff, long n_leaf, int cheb_deg, int dof, const double* node_val);
+void PVFMMNodes2CoeffF(float* coeff, long n_leaf, int cheb_deg, int dof, const float* node_val);
+
 /* ---------------------------------------------------------------- tree-level interface */
 
 /* FMM_Pts<...>::Initialize(m, comm, kernel) (include/fmm_pts.txx:223-331): operator
@@ -136,6 +170,12 @@ int pvfmm_amd_tree_stats(void* tree, int64_t* out22);
  * 5 d2d (8 side by side), 6 v1 (complex interleaved).
  * Returns the element count; copies when out != NULL. */
 int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out);
+
+/* Kernel<T>::ker_poten (include/kernel.hpp:84, used by the reference's examples for their
+ * error checks): FP64 all-pairs sum on the GPU, out[k1*n_trg] = K(trg, src) den with the
+ * kernel's 1/(4 pi) or 1/(8 pi) scale; r = 0 pairs contribute nothing.  Host arrays, AoS. */
+int pvfmm_amd_direct(int kernel, const double* src, int64_t n_src, const double* den, const double* trg, int64_t n_trg,
+                     double* out);
 
 /* Number of GPUs visible; 0 when the HIP runtime finds none. */
 int pvfmm_amd_device_count(void);

@@ -12,6 +12,7 @@ from .wrapper import (  # noqa: F401
     PtFMM,
     PtFMMTree,
     device_count,
+    direct,
     kernel_dims,
     kernel_id,
 )
@@ -24,6 +25,7 @@ __all__ = [
     "PtFMM",
     "PtFMMTree",
     "device_count",
+    "direct",
     "kernel_dims",
     "kernel_id",
 ]

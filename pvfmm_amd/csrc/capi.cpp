@@ -13,6 +13,7 @@
 #include "../../include/pvfmm.h"
 #include "common.hpp"
 #include "device_fmm.hpp"
+#include "kernels.hpp"
 #include "octree.hpp"
 #include "operators.hpp"
 
@@ -406,10 +407,77 @@ int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out) {
   if (out) std::copy(v->begin(), v->end(), out);
   return (int64_t)v->size();
 }
+int pvfmm_amd_direct(int kernel, const double* src, int64_t n_src, const double* den, const double* trg, int64_t n_trg,
+                     double* out) {
+  return guarded([&] {
+    const KernelInfo& ki = kernel_info(kernel);
+    if (n_src < 0 || n_trg < 0 || (n_src && (!src || !den)) || (n_trg && (!trg || !out)))
+      throw Error(kErrInvalidArgument, "bad direct-sum arguments");
+    if (n_trg == 0) return;
+    if (n_src == 0) {
+      std::fill(out, out + ki.k1 * n_trg, 0.0);
+      return;
+    }
+    struct Buf {
+      double* p = nullptr;
+      ~Buf() { if (p) (void)hipFree(p); }
+    } ds, dd, dt, dout;
+    HIP_CHECK(hipMalloc(&ds.p, sizeof(double) * 3 * n_src));
+    HIP_CHECK(hipMalloc(&dd.p, sizeof(double) * ki.k0 * n_src));
+    HIP_CHECK(hipMalloc(&dt.p, sizeof(double) * 3 * n_trg));
+    HIP_CHECK(hipMalloc(&dout.p, sizeof(double) * ki.k1 * n_trg));
+    HIP_CHECK(hipMemcpy(ds.p, src, sizeof(double) * 3 * n_src, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dd.p, den, sizeof(double) * ki.k0 * n_src, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(dt.p, trg, sizeof(double) * 3 * n_trg, hipMemcpyHostToDevice));
+    direct_sum(nullptr, ki.pt_kind, ds.p, n_src, dd.p, dt.p, n_trg, dout.p);
+    HIP_CHECK(hipMemcpy(out, dout.p, sizeof(double) * ki.k1 * n_trg, hipMemcpyDeviceToHost));
+  });
+}
 int pvfmm_amd_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
 }
+
+// ---- volume (Chebyshev) FMM, include/pvfmm.h:43-228 of the reference: not provided.  The
+// entry points exist so that bindings resolving every symbol at load time load this
+// library; each records PVFMM_ERR_UNSUPPORTED and returns NULL / 0 / nothing.
+static void volume_unsupported() { set_error(kErrUnsupported, "the volume (Chebyshev) FMM is not provided by this library"); }
+void* PVFMMCreateVolumeFMMD(int, int, enum PVFMMKernel, pvfmm_comm_t) { volume_unsupported(); return nullptr; }
+void* PVFMMCreateVolumeFMMF(int, int, enum PVFMMKernel, pvfmm_comm_t) { volume_unsupported(); return nullptr; }
+void* PVFMMCreateVolumeTreeD(int, int, void (*)(const double*, long, double*, const void*), const void*, const double*, long,
+                             pvfmm_comm_t, double, int, bool, int) {
+  volume_unsupported();
+  return nullptr;
+}
+void* PVFMMCreateVolumeTreeF(int, int, void (*)(const float*, long, float*, const void*), const void*, const float*, long,
+                             pvfmm_comm_t, float, int, bool, int) {
+  volume_unsupported();
+  return nullptr;
+}
+void* PVFMMCreateVolumeTreeFromCoeffD(long, int, int, const double*, const double*, const double*, long, pvfmm_comm_t, bool) {
+  volume_unsupported();
+  return nullptr;
+}
+void* PVFMMCreateVolumeTreeFromCoeffF(long, int, int, const float*, const float*, const float*, long, pvfmm_comm_t, bool) {
+  volume_unsupported();
+  return nullptr;
+}
+void PVFMMEvaluateVolumeFMMD(double*, void*, const void*, long) { volume_unsupported(); }
+void PVFMMEvaluateVolumeFMMF(float*, void*, const void*, long) { volume_unsupported(); }
+void PVFMMDestroyVolumeFMMD(void** ctx) { if (ctx) *ctx = nullptr; }
+void PVFMMDestroyVolumeFMMF(void** ctx) { if (ctx) *ctx = nullptr; }
+void PVFMMDestroyVolumeTreeD(void** ctx) { if (ctx) *ctx = nullptr; }
+void PVFMMDestroyVolumeTreeF(void** ctx) { if (ctx) *ctx = nullptr; }
+long PVFMMGetLeafCountD(const void*) { volume_unsupported(); return 0; }
+long PVFMMGetLeafCountF(const void*) { volume_unsupported(); return 0; }
+void PVFMMGetLeafCoordD(double*, const void*) { volume_unsupported(); }
+void PVFMMGetLeafCoordF(float*, const void*) { volume_unsupported(); }
+void PVFMMGetPotentialCoeffD(double*, const void*) { volume_unsupported(); }
+void PVFMMGetPotentialCoeffF(float*, const void*) { volume_unsupported(); }
+void PVFMMCoeff2NodesD(double*, long, int, int, const double*) { volume_unsupported(); }
+void PVFMMCoeff2NodesF(float*, long, int, int, const float*) { volume_unsupported(); }
+void PVFMMNodes2CoeffD(double*, long, int, int, const double*) { volume_unsupported(); }
+void PVFMMNodes2CoeffF(float*, long, int, int, const float*) { volume_unsupported(); }
 
 }  // extern "C"

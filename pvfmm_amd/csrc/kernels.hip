@@ -93,6 +93,51 @@ __global__ void k_fill(T* p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = T(0);
 }
 
+// ----------------------------------------------------------------------------- direct sum
+// FP64 all-pairs evaluation (the reference's Kernel<T>::ker_poten, include/kernel.hpp:84):
+// one thread per target, sources staged through LDS 256 at a time.  AoS inputs and outputs.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_direct(const double* __restrict__ src, int64_t n_src, const double* __restrict__ den,
+                                                const double* __restrict__ trg, int64_t n_trg, double* __restrict__ out) {
+  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1;
+  __shared__ double sc[3 * 256];
+  __shared__ double sd[K0 * 256];
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  const bool valid = i < n_trg;
+  const double tx = valid ? trg[3 * i] : 0.0, ty = valid ? trg[3 * i + 1] : 0.0, tz = valid ? trg[3 * i + 2] : 0.0;
+  double u[K1];
+#pragma unroll
+  for (int c = 0; c < K1; c++) u[c] = 0.0;
+  for (int64_t j0 = 0; j0 < n_src; j0 += 256) {
+    const int64_t j = j0 + threadIdx.x;
+    const int nj = (int)min((int64_t)256, n_src - j0);
+    __syncthreads();
+    if (j < n_src) {
+      for (int d = 0; d < 3; d++) sc[3 * threadIdx.x + d] = src[3 * j + d];
+      for (int c = 0; c < K0; c++) sd[K0 * threadIdx.x + c] = den[K0 * j + c];
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nj; jj++)
+      interact<KIND, double>(tx - sc[3 * jj], ty - sc[3 * jj + 1], tz - sc[3 * jj + 2], &sd[K0 * jj], u);
+  }
+  if (valid)
+#pragma unroll
+    for (int c = 0; c < K1; c++) out[K1 * i + c] = u[c] * kscale<KIND>();
+}
+
+void direct_sum(hipStream_t s, int kind, const double* src, int64_t n_src, const double* den, const double* trg, int64_t n_trg,
+                double* out) {
+  if (n_trg == 0) return;
+  const dim3 g((unsigned)((n_trg + 255) / 256));
+  if (kind == 0)
+    hipLaunchKernelGGL(k_direct<0>, g, dim3(256), 0, s, src, n_src, den, trg, n_trg, out);
+  else if (kind == 1)
+    hipLaunchKernelGGL(k_direct<1>, g, dim3(256), 0, s, src, n_src, den, trg, n_trg, out);
+  else
+    hipLaunchKernelGGL(k_direct<2>, g, dim3(256), 0, s, src, n_src, den, trg, n_trg, out);
+  HIP_LAUNCH_CHECK();
+}
+
 // ----------------------------------------------------------------------------- P2P: points -> points
 // Coordinates are FP64 leaf-local offsets; the pair displacement is formed in FP64 and then
 // rounded to Real, so that near-coincident pairs keep their digits in the FP32 evaluator.
@@ -1006,11 +1051,14 @@ __global__ __launch_bounds__(256) void k_hadamard(int n_slots, int M_dim, int n_
 // Tiled frequency-domain M2L.  A block owns one tile of consecutive target parents (at most
 // 64 = 4 groups of 16, with at most UMAX distinct V1 partners) and FB consecutive
 // frequencies.  The partners' spectra for those frequencies are staged in LDS once (entry
-// stride padded by one 4-vector against bank conflicts); each wave then runs the 26 parent
-// offsets for one frequency over TG target groups, with the transfer block of the offset as
-// the shared MFMA B operand and the outputs accumulated in registers.  Blocks are mapped so
-// that all tiles of one frequency group run on one XCD and share its L2.
-template <typename Real, int E0, int E1, int FB, int TG, int UMAX>
+// stride padded by one 4-vector against bank conflicts, plus one zero entry that stands for
+// a missing partner).  The WPF = 4/FB waves of one frequency split the 26 parent offsets;
+// each runs its offsets over all 4 target groups with the transfer block of the offset as the
+// shared MFMA B operand, held in registers, and the outputs accumulated in registers.  The
+// partial sums of the waves of one frequency are added through LDS in a fixed order.  The
+// instruction stream has no data-dependent branches, so the accumulators stay in place.
+// Blocks are mapped so that all tiles of one frequency group run on one XCD and share its L2.
+template <typename Real, int E0, int E1, int FB, int UMAX>
 __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, const int32_t* __restrict__ tile_t0,
                                                         const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
                                                         const uint16_t* __restrict__ uidx, const int32_t* __restrict__ offsets,
@@ -1022,10 +1070,14 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
   constexpr int RW = 16 * E0;                     // reals of one (slot, frequency) spectrum row
   constexpr int PAD = 4;                          // one 4-vector of pad per staged entry
   constexpr int EST = FB * RW + PAD;              // staged entry stride (reals)
-  constexpr int WPF = 4 / FB;                     // waves per frequency
+  constexpr int WPF = 4 / FB;                     // waves per frequency (split the offsets)
+  constexpr int NOW = NOFF / WPF;                 // offsets per wave
+  static_assert(NOW * WPF == NOFF, "offsets split evenly");
   constexpr int NB = E0 * E1;                     // 4-vectors of one offset's transfer block per lane
-  constexpr int PD = NB == 1 ? 4 : 2;             // transfer-block prefetch depth (offsets)
-  static_assert(WPF * TG == 4, "a tile has 4 target groups of 16");
+  // transfer blocks held in registers, PD offsets ahead (all of the wave's offsets when they
+  // fit in ~104 VGPRs); the first PD are loaded before the staging barrier
+  constexpr int PD0 = 104 / (NB * (int)sizeof(V4) / 4);
+  constexpr int PD = PD0 < 1 ? 1 : (PD0 > NOW ? NOW : PD0);
   constexpr uint16_t NONE = 0xFFFF;
   __shared__ __align__(16) Real Xl[(UMAX + 1) * EST];  // entry UMAX stays zero
   __shared__ uint16_t ul[64 * NOFF];
@@ -1046,37 +1098,44 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
   for (int e = threadIdx.x; e < 64 * NOFF; e += 256) ul[e] = uidx[(int64_t)tile * 64 * NOFF + e];
   for (int e = threadIdx.x; e < EST; e += 256) Xl[UMAX * EST + e] = Real(0);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int fl = w / WPF, gsub = w % WPF, f = f0 + (fl < fn ? fl : 0);
-  // transfer blocks of the first PD offsets, in flight during the staging barrier
+  const int fl = w / WPF, gsub = w % WPF;
+  const bool live = fl < fn;                  // waves past the last frequency compute, not store
+  const int f = f0 + (live ? fl : 0);
+  const int ob = gsub * NOW;                  // this wave's first offset
   const Real* vf = Vp + (int64_t)f * (NB * 256) + l * 4;
   const int64_t vstride = (int64_t)M_dim * NB * 256;
   V4 bq[PD][NB];
 #pragma unroll
   for (int p = 0; p < PD; p++)
 #pragma unroll
-    for (int q = 0; q < NB; q++) bq[p][q] = *reinterpret_cast<const V4*>(vf + offsets[p] * vstride + q * 256);
-  __syncthreads();
-  if (fl >= fn) return;  // no barrier follows
-  const int row = l & 15, kg = l >> 4;
-  V4 acc[TG][E1];
+    for (int q = 0; q < NB; q++) bq[p][q] = *reinterpret_cast<const V4*>(vf + offsets[ob + p] * vstride + q * 256);
+  // pin the blocks in registers here: otherwise the loads are sunk to their first use past
+  // the barrier and the wave stalls on each of them in the offset loop
 #pragma unroll
-  for (int g = 0; g < TG; g++)
+  for (int p = 0; p < PD; p++)
+#pragma unroll
+    for (int q = 0; q < NB; q++) asm volatile("" : "+v"(bq[p][q]));
+  __syncthreads();
+  const int row = l & 15, kg = l >> 4;
+  V4 acc[4][E1];
+#pragma unroll
+  for (int g = 0; g < 4; g++)
 #pragma unroll
     for (int nb = 0; nb < E1; nb++) acc[g][nb] = V4{0, 0, 0, 0};
 #pragma unroll
-  for (int oi = 0; oi < NOFF; oi++) {
+  for (int j = 0; j < NOW; j++) {
+    const int oi = ob + j;
     V4 cur[NB];
 #pragma unroll
-    for (int q = 0; q < NB; q++) cur[q] = bq[oi % PD][q];
-    if (oi + PD < NOFF) {
+    for (int q = 0; q < NB; q++) cur[q] = bq[j % PD][q];
+    if (j + PD < NOW) {
 #pragma unroll
-      for (int q = 0; q < NB; q++) bq[oi % PD][q] = *reinterpret_cast<const V4*>(vf + offsets[oi + PD] * vstride + q * 256);
+      for (int q = 0; q < NB; q++) bq[j % PD][q] = *reinterpret_cast<const V4*>(vf + offsets[oi + PD] * vstride + q * 256);
     }
 #pragma unroll
-    for (int g = 0; g < TG; g++) {
-      const int tg = (gsub * TG + g) * 16 + row;
-      const uint16_t u = ul[tg * NOFF + oi];  // rows past the tile's targets hold NONE
-      const int ue = (u == NONE) ? UMAX : (int)u;  // missing partner -> the zero entry
+    for (int g = 0; g < 4; g++) {
+      const uint16_t u = ul[(g * 16 + row) * NOFF + oi];  // rows past the tile's targets hold NONE
+      const int ue = (u == NONE) ? UMAX : (int)u;          // missing partner -> the zero entry
 #pragma unroll
       for (int kb = 0; kb < E0; kb++) {
         const V4 a = *reinterpret_cast<const V4*>(&Xl[ue * EST + fl * RW + 16 * kb + 4 * kg]);
@@ -1091,15 +1150,167 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
       }
     }
   }
+  if constexpr (WPF > 1) {
+    // partial sums of the other waves of this frequency, added in wave order
+    static_assert(WPF == 2, "two waves per frequency");
+    static_assert((int)sizeof(Xl) >= 2 * 4 * E1 * 256 * (int)sizeof(Real), "exchange fits in the staging area");
+    __syncthreads();  // every wave is done reading Xl
+    Real* ex = Xl + fl * (4 * E1 * 256);
+    if (gsub == 1) {
 #pragma unroll
-  for (int g = 0; g < TG; g++)
+      for (int g = 0; g < 4; g++)
+#pragma unroll
+        for (int nb = 0; nb < E1; nb++) *reinterpret_cast<V4*>(&ex[((g * E1 + nb) * 64 + l) * 4]) = acc[g][nb];
+    }
+    __syncthreads();
+    if (gsub == 1) return;
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int nb = 0; nb < E1; nb++) acc[g][nb] += *reinterpret_cast<const V4*>(&ex[((g * E1 + nb) * 64 + l) * 4]);
+  }
+  if (!live) return;
+#pragma unroll
+  for (int g = 0; g < 4; g++)
 #pragma unroll
     for (int nb = 0; nb < E1; nb++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int tg = (gsub * TG + g) * 16 + MF::row(l, r);
+        const int tg = g * 16 + MF::row(l, r);
         if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * (16 * E1) + 16 * nb + row] = acc[g][nb][r];
       }
+}
+
+// 16-byte direct global -> LDS copy of one lane (gfx950 global_load_lds_dwordx4): the wave's
+// lanes write 16 bytes each at lds_wave_base + 16 * lane.  The builtin exists only in the
+// device compilation pass; the host pass sees an empty body.
+__device__ __forceinline__ void lds_copy16(const void* g, void* lds_wave_base) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_global_load_lds(g, lds_wave_base, 16, 0, 0);
+#endif
+}
+
+// Persistent, double-buffered variant of the tiled M2L for scalar kernels (e0 = e1 = 1).
+// One block per CU, 8 * nper blocks; block b serves XCD x = b % 8, which owns the frequency
+// groups fg = x, x+8, ...  The block walks its work items (tile, fg) in fg-major order,
+// item = b/8 + nper*j.  Four MFMA waves compute one item from one LDS buffer while a fifth
+// wave copies the next item's partner spectra and partner table global -> LDS into the
+// other buffer with the direct LDS loads of gfx950 (global_load_lds, 16 bytes per lane:
+// lane i of a wave writes 16 bytes at the wave's LDS base + 16 i).  Keeping the copies in
+// their own wave keeps the compiler from making the MFMA waves wait on them.  A staged entry
+// is 16 chunks of 16 bytes; chunk q of entry u holds global chunk q ^ (u & 15), which spreads
+// the 16 rows of an MFMA operand read over the LDS banks.  WPF MFMA waves share a frequency,
+// each with TG of the 4 target groups and the 26 transfer blocks of the frequency in
+// registers, reloaded when the block moves to its next frequency group.
+template <typename Real, int FB, int UMAX>
+__global__ __launch_bounds__(320) void k_hadamard_pipe(int M_dim, int n_tiles, int n_fg, int nper,
+                                                       const int32_t* __restrict__ tile_t0, const int32_t* __restrict__ tile_u0,
+                                                       const int32_t* __restrict__ usrc, const uint16_t* __restrict__ uidx,
+                                                       const int32_t* __restrict__ offsets, const Real* __restrict__ Vp,
+                                                       const Real* __restrict__ X, Real* __restrict__ Y) {
+  using MF = Mfma<Real>;
+  using V4 = typename MF::V4;
+  constexpr int NOFF = 26;
+  constexpr int RW = 16;                        // reals of one (slot, frequency) spectrum row
+  constexpr int ENT = FB * RW;                  // reals per staged entry
+  static_assert(ENT * (int)sizeof(Real) == 256, "an entry is 16 chunks of 16 bytes");
+  constexpr int CPE = 16, RPC = 16 / (int)sizeof(Real);  // chunks per entry, reals per chunk
+  constexpr int CPF = CPE / FB;                 // chunks per frequency
+  constexpr int NMW = 4, WPF = NMW / FB, TG = 4 / WPF;     // MFMA waves; wave 4 copies
+  static_assert(WPF * FB == NMW && TG * WPF == 4, "wave split");
+  constexpr int UTB = 64 * NOFF * 2;            // partner table bytes per tile
+  static_assert(UTB % 16 == 0, "partner table is whole chunks");
+  constexpr uint16_t NONE = 0xFFFF;
+  __shared__ __align__(16) Real xb[2][(UMAX + 1) * ENT];  // entry UMAX stays zero
+  __shared__ __align__(16) uint16_t ub[2][64 * NOFF];
+  const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
+  const int fgx = n_fg > x ? (n_fg - x + 7) / 8 : 0;
+  const int n_items = fgx * n_tiles;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  for (int e = tid; e < ENT; e += 5 * 64) {
+    xb[0][UMAX * ENT + e] = Real(0);
+    xb[1][UMAX * ENT + e] = Real(0);
+  }
+  // copy wave: item `it` into buffer `buf`
+  auto copy = [&](int it, int buf) __attribute__((always_inline)) {
+    const int tile = it % n_tiles, f0 = (x + 8 * (it / n_tiles)) * FB, fn = min(FB, M_dim - f0);
+    const int u0 = tile_u0[tile], nu = tile_u0[tile + 1] - u0;
+    for (int c0 = 0; c0 < nu * CPE; c0 += 64) {
+      const int c = c0 + l, u = c / CPE, q = c % CPE, gq = q ^ (u & 15);
+      if (u < nu && gq / CPF < fn)
+        lds_copy16(X + ((int64_t)usrc[u0 + u] * M_dim + f0) * RW + gq * RPC, &xb[buf][(c0 / CPE) * ENT]);
+    }
+    for (int c0 = 0; c0 < UTB / 16; c0 += 64) {
+      const int c = c0 + l;
+      if (c < UTB / 16)
+        lds_copy16(reinterpret_cast<const unsigned char*>(uidx) + (int64_t)tile * UTB + c * 16,
+                   reinterpret_cast<unsigned char*>(&ub[buf][0]) + c0 * 16);
+    }
+  };
+  const int fl = w / WPF, gsub = w % WPF, row = l & 15, kg = l >> 4;
+  int item = k, j = 0;
+  if (w == NMW && item < n_items) copy(item, 0);
+  for (int fgi = 0; fgi < fgx; fgi++) {
+    const int f0 = (x + 8 * fgi) * FB, fn = min(FB, M_dim - f0);
+    const bool live = w < NMW && fl < fn;  // waves past the last frequency compute, not store
+    const int f = f0 + ((w < NMW && fl < fn) ? fl : 0);
+    // the 26 transfer blocks of this wave's frequency (lane layout of the B operand)
+    V4 bq[NOFF];
+    if (w < NMW) {
+      const Real* vf = Vp + (int64_t)f * 256 + l * 4;
+#pragma unroll
+      for (int o = 0; o < NOFF; o++) bq[o] = *reinterpret_cast<const V4*>(vf + (int64_t)offsets[o] * M_dim * 256);
+    }
+    for (; item < (fgi + 1) * n_tiles; item += nper, j++) {
+      const int buf = j & 1;
+      __builtin_amdgcn_s_waitcnt(0);  // the copy wave's copies of `item` landed
+      __syncthreads();                // ... for everyone; the other buffer is free
+      if (w == NMW) {
+        if (item + nper < n_items) copy(item + nper, buf ^ 1);
+        continue;
+      }
+      const int tile = item % n_tiles;
+      const int t0 = tile_t0[tile], nt = tile_t0[tile + 1] - t0;
+      const Real* xs = &xb[buf][0];
+      const uint16_t* us = &ub[buf][0];
+      V4 acc[TG];
+#pragma unroll
+      for (int g = 0; g < TG; g++) acc[g] = V4{0, 0, 0, 0};
+#pragma unroll
+      for (int oi = 0; oi < NOFF; oi++) {
+#pragma unroll
+        for (int g = 0; g < TG; g++) {
+          const uint16_t u = us[((gsub * TG + g) * 16 + row) * NOFF + oi];  // rows past the tile hold NONE
+          const int ue = (u == NONE) ? UMAX : (int)u;
+          const Real* ent = xs + ue * ENT;
+          V4 a;
+          if constexpr (sizeof(Real) == 4) {
+            a = *reinterpret_cast<const V4*>(ent + ((fl * CPF + kg) ^ (ue & 15)) * RPC);
+          } else {
+            using D2 = double __attribute__((ext_vector_type(2)));
+            const int gq = fl * CPF + 2 * kg;
+            const D2 lo = *reinterpret_cast<const D2*>(ent + (gq ^ (ue & 15)) * RPC);
+            const D2 hi = *reinterpret_cast<const D2*>(ent + ((gq + 1) ^ (ue & 15)) * RPC);
+            a = V4{lo[0], lo[1], hi[0], hi[1]};
+          }
+          acc[g] = MF::mma(a[0], bq[oi][0], acc[g]);
+          acc[g] = MF::mma(a[1], bq[oi][1], acc[g]);
+          acc[g] = MF::mma(a[2], bq[oi][2], acc[g]);
+          acc[g] = MF::mma(a[3], bq[oi][3], acc[g]);
+        }
+      }
+      if (live) {
+#pragma unroll
+        for (int g = 0; g < TG; g++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int tg = (gsub * TG + g) * 16 + MF::row(l, r);
+            if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * 16 + row] = acc[g][r];
+          }
+      }
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);
 }
 
 // ----------------------------------------------------------------------------- launchers
@@ -1284,18 +1495,24 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
                                          const int32_t* offsets, const Real* Vp, const Real* X, Real* Y) {
   if (n_tiles == 0) return;
   if (n_off != 26) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // the 26 V1 parent offsets
-  constexpr int FB = sizeof(Real) == 4 ? 4 : 2, TG = 4 / (4 / FB);
+  constexpr int FB = sizeof(Real) == 4 ? 4 : 2;
   const int n_fg = (M_dim + FB - 1) / FB;
   const int64_t nb = (int64_t)n_tiles * ((n_fg + 7) / 8) * 8;
   if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-  if (e0 == 1 && e1 == 1)
-    hipLaunchKernelGGL((k_hadamard_tiled<Real, 1, 1, FB, TG, 256>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
+  if (e0 == 1 && e1 == 1) {
+    // persistent blocks: one per CU (the LDS double buffer fills a CU), grouped 8 per XCD round
+    int dev = 0, n_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      throw_device_error(hipErrorInvalidDevice, __FILE__, __LINE__);
+    const int nper = std::max(1, n_cu / 8);
+    hipLaunchKernelGGL((k_hadamard_pipe<Real, FB, 256>), dim3((unsigned)(8 * nper)), dim3(320), 0, s, M_dim, n_tiles, n_fg, nper,
                        tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
-  else if (e0 == 3 && e1 == 3)
-    hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, FB, TG, 80>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
+  } else if (e0 == 3 && e1 == 3) {
+    hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, FB, 80>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
                        tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
-  else
+  } else {
     throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+  }
   HIP_LAUNCH_CHECK();
 }
 

@@ -73,4 +73,9 @@ struct DeviceKernels {
                         const Real* scal);
 };
 
+// FP64 all-pairs sum on the device (AoS src[3n], den[k0 n], trg[3n], out[k1 n] device arrays);
+// kind as in DeviceKernels (0 laplace potential, 1 laplace gradient, 2 stokes velocity).
+void direct_sum(hipStream_t s, int kind, const double* src, int64_t n_src, const double* den, const double* trg, int64_t n_trg,
+                double* out);
+
 }  // namespace pvfmm_amd

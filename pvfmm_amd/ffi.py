@@ -72,6 +72,7 @@ def _declare(L):
         "pvfmm_amd_tree_stats": (i, [vp, c_int64_p]),
         "pvfmm_amd_fmm_operator": (i64, [vp, i, c_double_p]),
         "pvfmm_amd_device_count": (i, []),
+        "pvfmm_amd_direct": (i, [i, c_double_p, i64, c_double_p, c_double_p, i64, c_double_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -294,3 +294,19 @@ class PtFMMTree:
 
 def device_count() -> int:
     return ffi.lib().pvfmm_amd_device_count()
+
+
+def direct(kernel, src, den, trg) -> np.ndarray:
+    """Kernel<T>::ker_poten (include/kernel.hpp:84): FP64 all-pairs sum on the GPU.
+
+    src [n_src, 3], den [n_src * k0], trg [n_trg, 3] -> [n_trg * k1] (AoS, input order)."""
+    kid = kernel_id(kernel)
+    k0, k1 = kernel_dims(kernel)
+    s = np.ascontiguousarray(src, dtype=np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(trg, dtype=np.float64).reshape(-1, 3)
+    d = np.ascontiguousarray(den, dtype=np.float64).ravel()
+    if d.size != k0 * len(s):
+        raise ValueError("den must hold k0 values per source")
+    out = np.zeros(k1 * len(t))
+    ffi.check(ffi.lib().pvfmm_amd_direct(kid, ffi.dptr(s), len(s), ffi.dptr(d), ffi.dptr(t), len(t), ffi.dptr(out)))
+    return out

@@ -196,3 +196,16 @@ def test_device_tensor_evaluation(gpu):
     t = P.PtFMMTree(x, x, 64).setup(P.PtFMM("laplace_poten", 6))
     u_dev = t.evaluate_device(torch.from_numpy(q).to(gpu)).cpu().numpy()
     assert np.array_equal(u_dev, t.evaluate(q))
+
+
+def test_gpu_direct_sum_matches_oracle(gpu):
+    """pvfmm_amd_direct (Kernel<T>::ker_poten) against the oracle's FP64 direct sum."""
+    rng = np.random.default_rng(20)
+    src = rng.random((3000, 3))
+    trg = np.concatenate([rng.random((700, 3)), src[:5]])  # includes r = 0 pairs
+    for kernel in ("laplace_poten", "laplace_grad", "stokes_vel"):
+        k0, _ = O.KDIM[KID[kernel]]
+        q = rng.standard_normal(k0 * len(src))
+        u = P.direct(kernel, src, q, trg)
+        ud = O.direct(kernel, src, q, trg)
+        assert rel(u, ud) < 1e-13, kernel
