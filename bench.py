@@ -144,6 +144,132 @@ def copy_probe(nbytes: int, reps: int = 3):
     log(f"copy probe: {reps} x {nbytes} bytes")
 
 
+def bwd_fwd(steps: int, warmup: int, n: int):
+    """Energy E = 1/2 q . phi (laplace_poten, uniform points, m = 10, max_pts = 256, mixed) with
+    the adjoint-FMM backward: per-step times of E and of dE/dx, dE/dq."""
+    import torch
+
+    from pvfmm_amd import workloads as W
+    from pvfmm_amd.autograd import LaplaceFMM
+
+    c = W.config5(n)
+    fmm = LaplaceFMM(c["src"], m=c["m"], max_pts=c["max_pts"], precision="mixed")
+    x = torch.tensor(c["src"], device="cuda:0", requires_grad=True)
+    q = torch.tensor(c["den"], device="cuda:0", requires_grad=True)
+    tf = tb = 0.0
+    for it in range(warmup + steps):
+        x.grad = q.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E = fmm.energy(x, q)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        E.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if it >= warmup:
+            tf += t1 - t0
+            tb += t2 - t1
+    tf /= steps
+    tb /= steps
+    return {"workload": f"laplace_poten energy, N={len(c['src'])} uniform, m={c['m']}, max_pts={c['max_pts']}, mixed",
+            "fwd_ms": 1e3 * tf, "bwd_ms": 1e3 * tb, "ratio": tb / tf}
+
+
+def bench_config(name: str, steps: int, warmup: int, check: bool):
+    """Secondary workloads (one JSON line each): 1 laplace_poten FP64 uniform; 3 stokes_vel on an
+    ellipsoid (mixed and FP64); 4a 32 density vectors on one tree (mixed); 4b 64 independent
+    clouds evaluated concurrently on their own streams (mixed); 5 energy forward/backward."""
+    import torch
+
+    import pvfmm_amd as P
+    from pvfmm_amd import workloads as W
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    def err(kernel, src, den, out, k1):
+        if not check:
+            return None
+        import oracle
+
+        rng = np.random.default_rng(4096)
+        idx = np.sort(rng.choice(len(src), min(4096, len(src)), replace=False))
+        ud = oracle.direct(kernel, src, den, src[idx])
+        return rel_l2(out.reshape(len(src), k1)[idx].ravel(), ud)
+
+    line = {"n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "data": "synthetic"}
+    if name in ("1", "3"):
+        c = W.config1() if name == "1" else W.config3()
+        n = len(c["src"])
+        k0, k1 = P.kernel_dims(c["kernel"])
+        tree = P.PtFMMTree(c["src"], c["trg"], c["max_pts"])
+        dq = torch.from_numpy(c["den"]).to("cuda:0")
+        res = {}
+        for prec in (["fp64"] if name == "1" else ["mixed", "fp64"]):
+            tree.setup(P.PtFMM(c["kernel"], c["m"], prec))
+            t = timed(lambda: tree.evaluate_device(dq))
+            out = tree.evaluate_device(dq).cpu().numpy()
+            res[prec] = {"ms_per_step": 1e3 * t, "mtargets_s": n / t / 1e6, "rel_l2": err(c["kernel"], c["src"], c["den"], out, k1)}
+            log(f"config {name} {prec}: {1e3 * t:.2f} ms ({n / t / 1e6:.1f} Mtargets/s), rel-L2 {res[prec]['rel_l2']}")
+        main_prec = "fp64" if name == "1" else "mixed"
+        line.update(metric=f"{c['kernel']} FMM Mtargets/s", value=res[main_prec]["mtargets_s"], unit="Mtargets/s",
+                    ms_per_step=res[main_prec]["ms_per_step"], dtype="f64" if main_prec == "fp64" else "f32",
+                    config={"workload": f"{c['kernel']}, N={n}, m={c['m']}, max_pts={c['max_pts']}", "config": name},
+                    results=res)
+    elif name == "4a":
+        c = W.config4a()
+        n, nd = c["den"].shape[1], c["den"].shape[0]
+        tree = P.PtFMMTree(c["src"], c["trg"], c["max_pts"]).setup(P.PtFMM(c["kernel"], c["m"], "mixed"), n_vec_max=nd)
+        dq = torch.from_numpy(c["den"]).to("cuda:0")
+        t = timed(lambda: tree.evaluate_device(dq))
+        out = tree.evaluate_device(dq)[0].cpu().numpy()
+        e = err(c["kernel"], c["src"], c["den"][0], out, 1)
+        log(f"config 4a: {1e3 * t:.2f} ms for {nd} densities ({n * nd / t / 1e6:.1f} Mtarget-densities/s), rel-L2 {e}")
+        line.update(metric="laplace_poten FMM Mtarget-densities/s", value=n * nd / t / 1e6, unit="Mtargets/s",
+                    ms_per_step=1e3 * t, dtype="f32", rel_l2=e,
+                    config={"workload": f"laplace_poten, N={n} uniform, {nd} density vectors, m={c['m']}, "
+                                        f"max_pts={c['max_pts']}, mixed", "config": "4a"})
+    elif name == "4b":
+        c = W.config4b()
+        trees, dens = [], []
+        for x, q in c["clouds"]:
+            trees.append(P.PtFMMTree(x, x, c["max_pts"]).setup(P.PtFMM(c["kernel"], c["m"], "mixed")))
+            dens.append(torch.from_numpy(q).to("cuda:0"))
+        outs = [torch.empty_like(d) for d in dens]
+        ntot = sum(len(q) for _, q in c["clouds"])
+
+        def run_all():
+            for tr, d, o in zip(trees, dens, outs):
+                tr.evaluate_device(d, out=o, sync=False)  # each tree has its own stream
+            for tr in trees:
+                torch.cuda.ExternalStream(tr.stream_handle()).synchronize()
+
+        t = timed(run_all)
+        e = err(c["kernel"], c["clouds"][0][0], c["clouds"][0][1], outs[0].cpu().numpy(), 1)
+        log(f"config 4b: {1e3 * t:.2f} ms for {len(trees)} clouds, {ntot} targets ({ntot / t / 1e6:.1f} Mtargets/s)")
+        line.update(metric="laplace_poten FMM Mtargets/s over 64 clouds", value=ntot / t / 1e6, unit="Mtargets/s",
+                    ms_per_step=1e3 * t, dtype="f32", rel_l2_cloud0=e,
+                    config={"workload": f"64 uniform clouds, {ntot} points, m={c['m']}, max_pts={c['max_pts']}, mixed, "
+                                        "one stream per cloud", "config": "4b"})
+    elif name == "5":
+        bf = bwd_fwd(steps, warmup, 1048576)
+        line.update(metric="laplace energy t_bwd / t_fwd", value=bf["ratio"], unit="ratio", higher_is_better=False,
+                    ms_per_step=bf["fwd_ms"] + bf["bwd_ms"], dtype="f32", config={"workload": bf["workload"], "config": "5"},
+                    bwd_fwd=bf)
+    else:
+        raise SystemExit(f"unknown config {name}")
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +282,8 @@ def main():
     ap.add_argument("--prec", choices=["both", "mixed", "fp64"], default="both",
                     help="evaluators to time (profiling runs); the JSON line needs both")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
+    ap.add_argument("--no-bwd", action="store_true", help="skip the energy forward/backward (config 5) timing")
+    ap.add_argument("--config", default="2", help="2 (headline, default), or a secondary workload: 1, 3, 4a, 4b, 5")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the single-GPU MPI_COMM_SELF share); --gpus {args.gpus} is not "
@@ -170,6 +298,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.config != "2":
+        bench_config(args.config, args.steps, args.warmup, not args.no_check)
         return
     c = W.config2(args.n)
     n = len(c["src"])
@@ -210,6 +341,11 @@ def main():
     m = c["m"]
     roof = roofline(mixed["phases"], st, m, real_bytes=4)
 
+    bf = None
+    if not args.no_bwd:
+        bf = bwd_fwd(max(3, args.steps // 2), 1, 1048576)
+        log(f"energy fwd {bf['fwd_ms']:.2f} ms, bwd {bf['bwd_ms']:.2f} ms, ratio {bf['ratio']:.2f}")
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -240,7 +376,8 @@ def main():
         "mixed_over_fp64": fp64["step"] / mixed["step"],
         "rel_l2_mixed": mixed["err"],
         "rel_l2_fp64": fp64["err"],
-        "bwd_fwd_ratio": None,
+        "bwd_fwd_ratio": None if bf is None else bf["ratio"],
+        "bwd_fwd": bf,
         "setup_s": {"tree": t_tree, "operators_mixed": mixed["t_ops"], "plan_mixed": mixed["t_setup"]},
         "roofline": roof,
         "cpu_baseline": cpu,
