@@ -177,12 +177,26 @@ class DevicePlan final : public DevicePlanBase {
 
     // ---- operators in the working precision
     auto conv = [](const std::vector<double>& v) { return std::vector<Real>(v.begin(), v.end()); };
-    uc2ue0_.upload(conv(O_.uc2ue0), &total);
-    uc2ue1_.upload(conv(O_.uc2ue1), &total);
-    dc2de0_.upload(conv(O_.dc2de0), &total);
-    dc2de1_.upload(conv(O_.dc2de1), &total);
-    ru_ = O_.rank_u;
-    rd_ = O_.rank_d;
+    // check -> equivalent factors with the rank padded to a multiple of 16 (zero columns of
+    // the first factor, zero rows of the second), so the GEMM operands stay 4-vector aligned
+    ru_ = (O_.rank_u + 15) / 16 * 16;
+    rd_ = (O_.rank_d + 15) / 16 * 16;
+    auto pad_cols = [&](const std::vector<double>& m, int rows, int r, int rp) {
+      std::vector<Real> out((size_t)rows * rp, Real(0));
+      for (int i = 0; i < rows; i++)
+        for (int j = 0; j < r; j++) out[(size_t)i * rp + j] = (Real)m[(size_t)i * r + j];
+      return out;
+    };
+    auto pad_rows = [&](const std::vector<double>& m, int r, int cols, int rp) {
+      std::vector<Real> out((size_t)rp * cols, Real(0));
+      for (int i = 0; i < r; i++)
+        for (int j = 0; j < cols; j++) out[(size_t)i * cols + j] = (Real)m[(size_t)i * cols + j];
+      return out;
+    };
+    uc2ue0_.upload(pad_cols(O_.uc2ue0, K1_, O_.rank_u, ru_), &total);
+    uc2ue1_.upload(pad_rows(O_.uc2ue1, O_.rank_u, K0_, ru_), &total);
+    dc2de0_.upload(pad_cols(O_.dc2de0, K1_, O_.rank_d, rd_), &total);
+    dc2de1_.upload(pad_rows(O_.dc2de1, O_.rank_d, K0_, rd_), &total);
     u2u_.upload(conv(O_.u2u), &total);
     d2d_.upload(conv(O_.d2d), &total);
     {

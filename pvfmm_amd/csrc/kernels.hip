@@ -292,6 +292,189 @@ __global__ __launch_bounds__(64) void k_far_points(const int32_t* __restrict__ t
   }
 }
 
+// ----------------------------------------------------------------------------- P2P, 256-thread leaf blocks
+// One target leaf per block of 256 threads.  Threads are (target t, split j) with
+// tid = t + ntc * j, j < nsplit = 256 / ntc (ntc targets in the current chunk of at most 256);
+// thread (t, j) sums the staged sources s = j, j + nsplit, ...  Sources are packed into LDS
+// chunks of up to SC points across consecutive items (so a chunk usually holds several
+// small leaves and one barrier pair serves them all); the partial sums of a chunk are added
+// into an FP64 accumulator, and the nsplit partials of a target are added in split order at
+// the end (deterministic).
+template <typename Real, int KIND>
+__global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tleaf, const int32_t* __restrict__ item_ptr,
+                                                  const int32_t* __restrict__ items, const int32_t* __restrict__ trg_begin,
+                                                  const int32_t* __restrict__ trg_end, const int32_t* __restrict__ src_begin,
+                                                  const int32_t* __restrict__ src_end, const double* __restrict__ center,
+                                                  const double* __restrict__ tx, int64_t n_trg, const double* __restrict__ sx,
+                                                  const Real* __restrict__ sden, int64_t n_src, double* __restrict__ out,
+                                                  bool accumulate) {
+  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1, NT = 256, SC = 512;
+  __shared__ double lc[3 * SC];  // FP64 source coordinates in the target leaf's frame
+  __shared__ Real ld[K0 * SC];
+  __shared__ double red[K1 * NT];
+  const int row = blockIdx.x, tid = threadIdx.x;
+  const int T = tleaf[row];
+  const int tb = trg_begin[T], nt = trg_end[T] - tb;
+  const double ct0 = center[3 * T], ct1 = center[3 * T + 1], ct2 = center[3 * T + 2];
+  const int ib = item_ptr[row], ie = item_ptr[row + 1];
+  for (int c0 = 0; c0 < nt; c0 += NT) {
+    const int ntc = min(NT, nt - c0), nsplit = NT / ntc;
+    const int t = tid % ntc, j = tid / ntc;
+    const bool active = j < nsplit;
+    double p0 = 0, p1 = 0, p2 = 0;
+    if (active) {
+      p0 = tx[tb + c0 + t];
+      p1 = tx[n_trg + tb + c0 + t];
+      p2 = tx[2 * n_trg + tb + c0 + t];
+    }
+    double acc[K1];
+#pragma unroll
+    for (int k = 0; k < K1; k++) acc[k] = 0;
+    int it = ib, off = 0;
+    while (it < ie) {
+      __syncthreads();  // the previous chunk is consumed
+      int pos = 0;
+      while (it < ie && pos < SC) {
+        const int S = items[it];
+        const int sb = src_begin[S], ns = src_end[S] - sb;
+        const int take = min(ns - off, SC - pos);
+        const double o0 = center[3 * S] - ct0, o1 = center[3 * S + 1] - ct1, o2 = center[3 * S + 2] - ct2;
+        for (int q = tid; q < take; q += NT) {
+          const int64_t g = sb + off + q;
+          lc[pos + q] = sx[g] + o0;
+          lc[SC + pos + q] = sx[n_src + g] + o1;
+          lc[2 * SC + pos + q] = sx[2 * n_src + g] + o2;
+#pragma unroll
+          for (int c = 0; c < K0; c++) ld[c * SC + pos + q] = sden[c * n_src + g];
+        }
+        pos += take;
+        off += take;
+        if (off == ns) {
+          it++;
+          off = 0;
+        }
+      }
+      __syncthreads();
+      if (active) {
+        Real part[K1];
+#pragma unroll
+        for (int k = 0; k < K1; k++) part[k] = 0;
+        for (int s = j; s < pos; s += nsplit) {
+          Real f[K0];
+#pragma unroll
+          for (int c = 0; c < K0; c++) f[c] = ld[c * SC + s];
+          interact<KIND, Real>((Real)(p0 - lc[s]), (Real)(p1 - lc[SC + s]), (Real)(p2 - lc[2 * SC + s]), f, part);
+        }
+#pragma unroll
+        for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K1; k++) red[k * NT + tid] = active ? acc[k] : 0.0;
+    __syncthreads();
+    if (tid < ntc) {
+#pragma unroll
+      for (int k = 0; k < K1; k++) {
+        double v = 0;
+        for (int jj = 0; jj < nsplit; jj++) v += red[k * NT + tid + jj * ntc];
+        double* d = &out[k * n_trg + tb + c0 + tid];
+        v *= kscale<KIND>();
+        *d = accumulate ? *d + v : v;
+      }
+    }
+  }
+}
+
+// Equivalent surfaces -> targets (own L2T surface and W-list M2T) with the leaf-block layout
+// of k_p2p_leaf; a staged source is the surface point o + r * size * unit_surf[p] of its node.
+template <typename Real, int KIND>
+__global__ __launch_bounds__(256) void k_far_leaf(const int32_t* __restrict__ tleaf, const int32_t* __restrict__ item_ptr,
+                                                  const int32_t* __restrict__ item_node, const int64_t* __restrict__ item_den,
+                                                  const double* __restrict__ item_rad, const int32_t* __restrict__ trg_begin,
+                                                  const int32_t* __restrict__ trg_end, const double* __restrict__ center,
+                                                  const float* __restrict__ node_size, const Real* __restrict__ unit_surf,
+                                                  int ns, const Real* __restrict__ tx, int64_t n_trg,
+                                                  const Real* __restrict__ dens, double* __restrict__ out, bool accumulate) {
+  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1, NT = 256, SC = 512;
+  __shared__ Real lc[3 * SC];
+  __shared__ Real ld[K0 * SC];
+  __shared__ double red[K1 * NT];
+  const int row = blockIdx.x, tid = threadIdx.x;
+  const int T = tleaf[row];
+  const int tb = trg_begin[T], nt = trg_end[T] - tb;
+  const double ct0 = center[3 * T], ct1 = center[3 * T + 1], ct2 = center[3 * T + 2];
+  const int ib = item_ptr[row], ie = item_ptr[row + 1];
+  for (int c0 = 0; c0 < nt; c0 += NT) {
+    const int ntc = min(NT, nt - c0), nsplit = NT / ntc;
+    const int t = tid % ntc, j = tid / ntc;
+    const bool active = j < nsplit;
+    Real p0 = 0, p1 = 0, p2 = 0;
+    if (active) {
+      p0 = tx[tb + c0 + t];
+      p1 = tx[n_trg + tb + c0 + t];
+      p2 = tx[2 * n_trg + tb + c0 + t];
+    }
+    double acc[K1];
+#pragma unroll
+    for (int k = 0; k < K1; k++) acc[k] = 0;
+    int it = ib, off = 0;
+    while (it < ie) {
+      __syncthreads();
+      int pos = 0;
+      while (it < ie && pos < SC) {
+        const int Q = item_node[it];
+        const Real* den = dens + item_den[it];
+        const int take = min(ns - off, SC - pos);
+        const Real sc = (Real)(item_rad[it] * node_size[Q]);
+        const Real o0 = (Real)(center[3 * Q] - ct0), o1 = (Real)(center[3 * Q + 1] - ct1), o2 = (Real)(center[3 * Q + 2] - ct2);
+        for (int q = tid; q < take; q += NT) {
+          const int pp = off + q;
+          lc[pos + q] = o0 + sc * unit_surf[3 * pp];
+          lc[SC + pos + q] = o1 + sc * unit_surf[3 * pp + 1];
+          lc[2 * SC + pos + q] = o2 + sc * unit_surf[3 * pp + 2];
+#pragma unroll
+          for (int c = 0; c < K0; c++) ld[c * SC + pos + q] = den[pp * K0 + c];
+        }
+        pos += take;
+        off += take;
+        if (off == ns) {
+          it++;
+          off = 0;
+        }
+      }
+      __syncthreads();
+      if (active) {
+        Real part[K1];
+#pragma unroll
+        for (int k = 0; k < K1; k++) part[k] = 0;
+        for (int s = j; s < pos; s += nsplit) {
+          Real f[K0];
+#pragma unroll
+          for (int c = 0; c < K0; c++) f[c] = ld[c * SC + s];
+          interact<KIND, Real>(p0 - lc[s], p1 - lc[SC + s], p2 - lc[2 * SC + s], f, part);
+        }
+#pragma unroll
+        for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K1; k++) red[k * NT + tid] = active ? acc[k] : 0.0;
+    __syncthreads();
+    if (tid < ntc) {
+#pragma unroll
+      for (int k = 0; k < K1; k++) {
+        double v = 0;
+        for (int jj = 0; jj < nsplit; jj++) v += red[k * NT + tid + jj * ntc];
+        double* d = &out[k * n_trg + tb + c0 + tid];
+        v *= kscale<KIND>();
+        *d = accumulate ? *d + v : v;
+      }
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------- P2P: points -> surfaces
 template <typename Real, int KIND>
 __global__ __launch_bounds__(64) void k_p2p_surface(const int32_t* __restrict__ tnode, const int32_t* __restrict__ item_ptr,
@@ -380,57 +563,85 @@ template <> struct Mfma<double> {
 // time in LDS with k contiguous for both operands so each lane feeds its four k values of an
 // MFMA group from one 4-vector read.  With ksplit > 1 the block handles one k range and writes
 // its partial tile to work[z][i][n]; k_gemm_reduce then sums the partials in a fixed order.
-template <typename Real, bool ACC, bool SPLIT>
+template <typename Real, bool ACC, bool SPLIT, bool VEC>
 __global__ __launch_bounds__(256) void k_gemm_rows(int M, int N, int K, int kchunk, const Real* __restrict__ A,
                                                    const int32_t* __restrict__ ra, int64_t lda, const Real* __restrict__ B,
                                                    Real* __restrict__ C, const int32_t* __restrict__ rc, int64_t ldc,
                                                    const Real* __restrict__ alpha, Real* __restrict__ work) {
   using MF = Mfma<Real>;
   using V4 = typename MF::V4;
-  constexpr int BM = 64, BN = 64, BK = 16, LD = BK + 4;
-  __shared__ Real As[BM * LD];
-  __shared__ Real Bt[BN * LD];
+  constexpr int BM = 64, BN = 64, BK = 32, LD = BK + 4;
+  __shared__ __align__(16) Real As[BM * LD];
+  __shared__ __align__(16) Real Bt[BN * LD];
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int kb0 = blockIdx.z * kchunk, kb1 = min(K, kb0 + kchunk);
-  // global->LDS assignments: A: 4 consecutive k of one row; B: 4 consecutive n of one k
-  const int ar = tid >> 2, ak = (tid & 3) * 4;
-  const int bk = tid >> 4, bn = (tid & 15) * 4;
+  // global -> register -> LDS assignments: A: 8 consecutive k of one row; B: 8 consecutive n of one k
+  const int ar = tid >> 2, ak = (tid & 3) * 8;
+  const int bk = tid >> 3, bn = (tid & 7) * 8;
   const bool arow_ok = m0 + ar < M;
-  const int64_t abase = arow_ok ? (int64_t)ra[m0 + ar] * lda : 0;
+  const Real* arow = A + (arow_ok ? (int64_t)ra[m0 + ar] * lda : 0);
+  Real ra_[8], rb_[8];
+  // load the k-step starting at k0 into registers (zero outside the matrix)
+  auto load = [&](int k0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int k = k0 + ak + 4 * h;
+      if (VEC && arow_ok && k + 3 < kb1) {
+        const V4 v = *reinterpret_cast<const V4*>(arow + k);
+#pragma unroll
+        for (int q = 0; q < 4; q++) ra_[4 * h + q] = v[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) ra_[4 * h + q] = (arow_ok && k + q < kb1) ? arow[k + q] : Real(0);
+      }
+    }
+    const int k = k0 + bk;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int n = n0 + bn + 4 * h;
+      if (VEC && k < kb1 && n + 3 < N) {
+        const V4 v = *reinterpret_cast<const V4*>(B + (int64_t)k * N + n);
+#pragma unroll
+        for (int q = 0; q < 4; q++) rb_[4 * h + q] = v[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) rb_[4 * h + q] = (k < kb1 && n + q < N) ? B[(int64_t)k * N + n + q] : Real(0);
+      }
+    }
+  };
   V4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = V4{0, 0, 0, 0};
+  if (kb0 < kb1) load(kb0);
   for (int k0 = kb0; k0 < kb1; k0 += BK) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = k0 + ak + q;
-      As[ar * LD + ak + q] = (arow_ok && k < kb1) ? A[abase + k] : Real(0);
-    }
+    for (int q = 0; q < 8; q++) As[ar * LD + ak + q] = ra_[q];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int k = k0 + bk, n = n0 + bn + q;
-      Bt[(bn + q) * LD + bk] = (k < kb1 && n < N) ? B[(int64_t)k * N + n] : Real(0);
-    }
+    for (int q = 0; q < 8; q++) Bt[(bn + q) * LD + bk] = rb_[q];
     __syncthreads();
-    V4 av[2], bv[2];
+    if (k0 + BK < kb1) load(k0 + BK);  // in flight during the MFMAs of this step
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      av[t] = *reinterpret_cast<const V4*>(&As[(wm * 32 + t * 16 + (l & 15)) * LD + 4 * (l >> 4)]);
-      bv[t] = *reinterpret_cast<const V4*>(&Bt[(wn * 32 + t * 16 + (l & 15)) * LD + 4 * (l >> 4)]);
-    }
+    for (int kk = 0; kk < BK / 16; kk++) {
+      V4 av[2], bv[2];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        acc[i][j] = MF::mma(av[i][0], bv[j][0], acc[i][j]);
-        acc[i][j] = MF::mma(av[i][1], bv[j][1], acc[i][j]);
-        acc[i][j] = MF::mma(av[i][2], bv[j][2], acc[i][j]);
-        acc[i][j] = MF::mma(av[i][3], bv[j][3], acc[i][j]);
+      for (int t = 0; t < 2; t++) {
+        av[t] = *reinterpret_cast<const V4*>(&As[(wm * 32 + t * 16 + (l & 15)) * LD + 16 * kk + 4 * (l >> 4)]);
+        bv[t] = *reinterpret_cast<const V4*>(&Bt[(wn * 32 + t * 16 + (l & 15)) * LD + 16 * kk + 4 * (l >> 4)]);
       }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          acc[i][j] = MF::mma(av[i][0], bv[j][0], acc[i][j]);
+          acc[i][j] = MF::mma(av[i][1], bv[j][1], acc[i][j]);
+          acc[i][j] = MF::mma(av[i][2], bv[j][2], acc[i][j]);
+          acc[i][j] = MF::mma(av[i][3], bv[j][3], acc[i][j]);
+        }
+    }
     __syncthreads();
   }
 #pragma unroll
@@ -1375,9 +1586,9 @@ void DeviceKernels<Real>::p2p_points(hipStream_t s, int kind, int n_rows, const 
                                      bool accumulate) {
   if (n_rows == 0) return;
 #define P2P_ARGS tleaf, item_ptr, items, trg_begin, trg_end, src_begin, src_end, center, tx, n_trg, sx, sden, n_src, out, accumulate
-  if (kind == 0) hipLaunchKernelGGL((k_p2p_points<Real, 0>), dim3(n_rows), dim3(64), 0, s, P2P_ARGS);
-  else if (kind == 1) hipLaunchKernelGGL((k_p2p_points<Real, 1>), dim3(n_rows), dim3(64), 0, s, P2P_ARGS);
-  else hipLaunchKernelGGL((k_p2p_points<Real, 2>), dim3(n_rows), dim3(64), 0, s, P2P_ARGS);
+  if (kind == 0) hipLaunchKernelGGL((k_p2p_leaf<Real, 0>), dim3(n_rows), dim3(256), 0, s, P2P_ARGS);
+  else if (kind == 1) hipLaunchKernelGGL((k_p2p_leaf<Real, 1>), dim3(n_rows), dim3(256), 0, s, P2P_ARGS);
+  else hipLaunchKernelGGL((k_p2p_leaf<Real, 2>), dim3(n_rows), dim3(256), 0, s, P2P_ARGS);
 #undef P2P_ARGS
   HIP_LAUNCH_CHECK();
 }
@@ -1390,9 +1601,9 @@ void DeviceKernels<Real>::far_points(hipStream_t s, int kind, int n_rows, const 
                                      const Real* dens, double* out, bool accumulate) {
   if (n_rows == 0) return;
 #define FAR_ARGS tleaf, item_ptr, item_node, item_den, item_rad, trg_begin, trg_end, center, node_size, unit_surf, ns, tx, n_trg, dens, out, accumulate
-  if (kind == 0) hipLaunchKernelGGL((k_far_points<Real, 0>), dim3(n_rows), dim3(64), 0, s, FAR_ARGS);
-  else if (kind == 1) hipLaunchKernelGGL((k_far_points<Real, 1>), dim3(n_rows), dim3(64), 0, s, FAR_ARGS);
-  else hipLaunchKernelGGL((k_far_points<Real, 2>), dim3(n_rows), dim3(64), 0, s, FAR_ARGS);
+  if (kind == 0) hipLaunchKernelGGL((k_far_leaf<Real, 0>), dim3(n_rows), dim3(256), 0, s, FAR_ARGS);
+  else if (kind == 1) hipLaunchKernelGGL((k_far_leaf<Real, 1>), dim3(n_rows), dim3(256), 0, s, FAR_ARGS);
+  else hipLaunchKernelGGL((k_far_leaf<Real, 2>), dim3(n_rows), dim3(256), 0, s, FAR_ARGS);
 #undef FAR_ARGS
   HIP_LAUNCH_CHECK();
 }
@@ -1410,11 +1621,23 @@ void DeviceKernels<Real>::gemm_rows(hipStream_t s, int M, int N, int K, const Re
     ksplit = std::min((1024 + blocks - 1) / blocks, (K + 63) / 64);
     while (ksplit > 1 && (int64_t)ksplit * M * N > work_elems) ksplit--;
   }
-  const int kchunk = ((K + ksplit - 1) / ksplit + 15) / 16 * 16;
+  const int kchunk = ((K + ksplit - 1) / ksplit + 31) / 32 * 32;
   ksplit = (K + kchunk - 1) / kchunk;
   dim3 g(gx, gy, ksplit);
+  // 4-vector loads when every row start and the B rows are 16-byte aligned
+  const bool vec = lda % 4 == 0 && N % 4 == 0 && (reinterpret_cast<uintptr_t>(A) % 32) == 0 &&
+                   (reinterpret_cast<uintptr_t>(B) % 32) == 0;
+#define GEMM_LAUNCH(ACCV, SPLITV)                                                                                     \
+  do {                                                                                                                \
+    if (vec)                                                                                                          \
+      hipLaunchKernelGGL((k_gemm_rows<Real, ACCV, SPLITV, true>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, \
+                         ldc, alpha, work);                                                                           \
+    else                                                                                                              \
+      hipLaunchKernelGGL((k_gemm_rows<Real, ACCV, SPLITV, false>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, \
+                         ldc, alpha, work);                                                                           \
+  } while (0)
   if (ksplit > 1) {
-    hipLaunchKernelGGL((k_gemm_rows<Real, false, true>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, ldc, alpha, work);
+    GEMM_LAUNCH(false, true);
     HIP_LAUNCH_CHECK();
     const int64_t tot = (int64_t)M * N;
     if (accumulate)
@@ -1422,10 +1645,11 @@ void DeviceKernels<Real>::gemm_rows(hipStream_t s, int M, int N, int K, const Re
     else
       hipLaunchKernelGGL((k_gemm_reduce<Real, false>), dim3(grid_for(tot, 256)), dim3(256), 0, s, M, N, ksplit, work, C, rc, ldc, alpha);
   } else if (accumulate) {
-    hipLaunchKernelGGL((k_gemm_rows<Real, true, false>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, ldc, alpha, work);
+    GEMM_LAUNCH(true, false);
   } else {
-    hipLaunchKernelGGL((k_gemm_rows<Real, false, false>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, ldc, alpha, work);
+    GEMM_LAUNCH(false, false);
   }
+#undef GEMM_LAUNCH
   HIP_LAUNCH_CHECK();
 }
 
