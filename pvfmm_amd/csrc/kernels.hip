@@ -17,6 +17,12 @@
 
 #include "kernels.hpp"
 
+// Hadamard kernel selection (build-time): 0 tiled with 2 accumulator sets, 1 tiled with one,
+// 2 and 3 persistent double-buffered with 2 or 4 sets.
+#ifndef PVFMM_HADAMARD_VARIANT
+#define PVFMM_HADAMARD_VARIANT 0
+#endif
+
 namespace pvfmm_amd {
 
 #define HIP_LAUNCH_CHECK()                                                       \
@@ -1269,7 +1275,7 @@ __global__ __launch_bounds__(256) void k_hadamard(int n_slots, int M_dim, int n_
 // partial sums of the waves of one frequency are added through LDS in a fixed order.  The
 // instruction stream has no data-dependent branches, so the accumulators stay in place.
 // Blocks are mapped so that all tiles of one frequency group run on one XCD and share its L2.
-template <typename Real, int E0, int E1, int FB, int UMAX>
+template <typename Real, int E0, int E1, int FB, int UMAX, int NACC>
 __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, const int32_t* __restrict__ tile_t0,
                                                         const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
                                                         const uint16_t* __restrict__ uidx, const int32_t* __restrict__ offsets,
@@ -1328,14 +1334,18 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
     for (int q = 0; q < NB; q++) asm volatile("" : "+v"(bq[p][q]));
   __syncthreads();
   const int row = l & 15, kg = l >> 4;
-  V4 acc[4][E1];
+  // NACC accumulator sets (offset j uses set j % NACC): independent MFMA chains
+  V4 accs[NACC][4][E1];
 #pragma unroll
-  for (int g = 0; g < 4; g++)
+  for (int a = 0; a < NACC; a++)
 #pragma unroll
-    for (int nb = 0; nb < E1; nb++) acc[g][nb] = V4{0, 0, 0, 0};
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int nb = 0; nb < E1; nb++) accs[a][g][nb] = V4{0, 0, 0, 0};
 #pragma unroll
   for (int j = 0; j < NOW; j++) {
     const int oi = ob + j;
+    auto& acc = accs[j % NACC];
     V4 cur[NB];
 #pragma unroll
     for (int q = 0; q < NB; q++) cur[q] = bq[j % PD][q];
@@ -1361,6 +1371,15 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
       }
     }
   }
+  V4 acc[4][E1];
+#pragma unroll
+  for (int g = 0; g < 4; g++)
+#pragma unroll
+    for (int nb = 0; nb < E1; nb++) {
+      acc[g][nb] = accs[0][g][nb];
+#pragma unroll
+      for (int a = 1; a < NACC; a++) acc[g][nb] += accs[a][g][nb];
+    }
   if constexpr (WPF > 1) {
     // partial sums of the other waves of this frequency, added in wave order
     static_assert(WPF == 2, "two waves per frequency");
@@ -1413,7 +1432,7 @@ __device__ __forceinline__ void lds_copy16(const void* g, void* lds_wave_base) {
 // the 16 rows of an MFMA operand read over the LDS banks.  WPF MFMA waves share a frequency,
 // each with TG of the 4 target groups and the 26 transfer blocks of the frequency in
 // registers, reloaded when the block moves to its next frequency group.
-template <typename Real, int FB, int UMAX>
+template <typename Real, int FB, int UMAX, int NACC>
 __global__ __launch_bounds__(320) void k_hadamard_pipe(int M_dim, int n_tiles, int n_fg, int nper,
                                                        const int32_t* __restrict__ tile_t0, const int32_t* __restrict__ tile_u0,
                                                        const int32_t* __restrict__ usrc, const uint16_t* __restrict__ uidx,
@@ -1484,11 +1503,14 @@ __global__ __launch_bounds__(320) void k_hadamard_pipe(int M_dim, int n_tiles, i
       const int t0 = tile_t0[tile], nt = tile_t0[tile + 1] - t0;
       const Real* xs = &xb[buf][0];
       const uint16_t* us = &ub[buf][0];
-      V4 acc[TG];
+      V4 accs[NACC][TG];  // offset oi uses set oi % NACC: independent MFMA chains
 #pragma unroll
-      for (int g = 0; g < TG; g++) acc[g] = V4{0, 0, 0, 0};
+      for (int a = 0; a < NACC; a++)
+#pragma unroll
+        for (int g = 0; g < TG; g++) accs[a][g] = V4{0, 0, 0, 0};
 #pragma unroll
       for (int oi = 0; oi < NOFF; oi++) {
+        auto& acc = accs[oi % NACC];
 #pragma unroll
         for (int g = 0; g < TG; g++) {
           const uint16_t u = us[((gsub * TG + g) * 16 + row) * NOFF + oi];  // rows past the tile hold NONE
@@ -1512,12 +1534,16 @@ __global__ __launch_bounds__(320) void k_hadamard_pipe(int M_dim, int n_tiles, i
       }
       if (live) {
 #pragma unroll
-        for (int g = 0; g < TG; g++)
+        for (int g = 0; g < TG; g++) {
+          V4 sum = accs[0][g];
+#pragma unroll
+          for (int a = 1; a < NACC; a++) sum += accs[a][g];
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             const int tg = (gsub * TG + g) * 16 + MF::row(l, r);
-            if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * 16 + row] = acc[g][r];
+            if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * 16 + row] = sum[r];
           }
+        }
       }
     }
   }
@@ -1711,7 +1737,12 @@ void DeviceKernels<Real>::hadamard(hipStream_t s, int n_slots, int M_dim, int e0
 }
 
 template <typename Real>
-int DeviceKernels<Real>::hadamard_union_max(int e0) { return e0 == 1 ? 256 : 80; }
+int DeviceKernels<Real>::hadamard_union_max(int e0) {
+  // FP32: 2 frequencies per block and a union of at most 250 keep a block's LDS under 40 KiB
+  // (four blocks per CU); FP64: 2 frequencies, 256; Stokes: 80
+  if (e0 != 1) return 80;
+  return (sizeof(Real) == 4 && PVFMM_HADAMARD_VARIANT < 2) ? 250 : 256;
+}
 
 template <typename Real>
 void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
@@ -1724,15 +1755,26 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
   const int64_t nb = (int64_t)n_tiles * ((n_fg + 7) / 8) * 8;
   if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   if (e0 == 1 && e1 == 1) {
+#if PVFMM_HADAMARD_VARIANT >= 2
     // persistent blocks: one per CU (the LDS double buffer fills a CU), grouped 8 per XCD round
     int dev = 0, n_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       throw_device_error(hipErrorInvalidDevice, __FILE__, __LINE__);
     const int nper = std::max(1, n_cu / 8);
-    hipLaunchKernelGGL((k_hadamard_pipe<Real, FB, 256>), dim3((unsigned)(8 * nper)), dim3(320), 0, s, M_dim, n_tiles, n_fg, nper,
+    constexpr int NA = PVFMM_HADAMARD_VARIANT == 2 ? 2 : 4;
+    hipLaunchKernelGGL((k_hadamard_pipe<Real, FB, 256, NA>), dim3((unsigned)(8 * nper)), dim3(320), 0, s, M_dim, n_tiles, n_fg, nper,
                        tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
+#else
+    constexpr int NA = PVFMM_HADAMARD_VARIANT == 1 ? 2 : 1;
+    constexpr int FBT = 2, UM = sizeof(Real) == 4 ? 250 : 256;
+    const int n_fgt = (M_dim + FBT - 1) / FBT;
+    const int64_t nbt = (int64_t)n_tiles * ((n_fgt + 7) / 8) * 8;
+    if (nbt >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+    hipLaunchKernelGGL((k_hadamard_tiled<Real, 1, 1, FBT, UM, NA>), dim3((unsigned)nbt), dim3(256), 0, s, M_dim, n_tiles,
+                       tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
+#endif
   } else if (e0 == 3 && e1 == 3) {
-    hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, FB, 80>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
+    hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, FB, 80, 1>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
                        tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
   } else {
     throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
