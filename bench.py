@@ -108,7 +108,7 @@ def roofline(ph: dict, st: dict, m: int, real_bytes: int):
     dom = max(ph, key=ph.get)
     t = ph[dom] * 1e-3
     if dom == "V-List:Hadamard":
-        name = "k_hadamard<float,1,1,2>"
+        name = "k_hadamard_tiled<float, 1, 1, 2, 250, 1>"
         bytes_alg = (16.0 * M_dim * real_bytes * st["v_pairs"] + 16.0 * M_dim * real_bytes * st["v_trg_slots"]
                      + 26.0 * M_dim * 128 * real_bytes)
         flops = 512.0 * st["v_pairs"] * M_dim
@@ -117,9 +117,10 @@ def roofline(ph: dict, st: dict, m: int, real_bytes: int):
         return {"kernel": name + " (V-list frequency-domain M2L)", "bound": "hbm", "achieved": achieved,
                 "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS, "traffic": traffic,
                 "ms_per_launch": ph[dom], "algorithmic_bytes": bytes_alg,
+                "mfma_tflops": flops / t / 1e12, "mfma_frac": flops / t / 1e12 / MI355X_FP32_MFMA_TFLOPS,
                 "work": f"{st['v_pairs']} V1 pairs x {M_dim} frequencies x 16 reals read + {st['v_trg_slots']} "
-                        f"target parents x {M_dim} x 16 reals written + 26 V1 matrices; {flops / t / 1e12:.1f} "
-                        f"TFLOP/s of {MI355X_FP32_MFMA_TFLOPS} FP32 MFMA"}
+                        f"target parents x {M_dim} x 16 reals written + 26 V1 matrices (SURVEY target-stationary "
+                        f"model); the staged kernel moves `traffic` bytes and is MFMA-issue bound"}
     if dom == "U-List":
         name = "k_p2p_points<float,1>"
         flops = 16.0 * st["u_pairs"]

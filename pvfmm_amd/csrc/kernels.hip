@@ -362,17 +362,21 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
       }
       __syncthreads();
       if (active) {
-        Real part[K1];
+        // Real partial sums over at most 64 sources, added into the FP64 accumulator
+        for (int s0 = j; s0 < pos; s0 += 64 * nsplit) {
+          Real part[K1];
 #pragma unroll
-        for (int k = 0; k < K1; k++) part[k] = 0;
-        for (int s = j; s < pos; s += nsplit) {
-          Real f[K0];
+          for (int k = 0; k < K1; k++) part[k] = 0;
+          const int s1 = min(pos, s0 + 64 * nsplit);
+          for (int s = s0; s < s1; s += nsplit) {
+            Real f[K0];
 #pragma unroll
-          for (int c = 0; c < K0; c++) f[c] = ld[c * SC + s];
-          interact<KIND, Real>((Real)(p0 - lc[s]), (Real)(p1 - lc[SC + s]), (Real)(p2 - lc[2 * SC + s]), f, part);
+            for (int c = 0; c < K0; c++) f[c] = ld[c * SC + s];
+            interact<KIND, Real>((Real)(p0 - lc[s]), (Real)(p1 - lc[SC + s]), (Real)(p2 - lc[2 * SC + s]), f, part);
+          }
+#pragma unroll
+          for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
         }
-#pragma unroll
-        for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
       }
     }
     __syncthreads();
@@ -451,17 +455,20 @@ __global__ __launch_bounds__(256) void k_far_leaf(const int32_t* __restrict__ tl
       }
       __syncthreads();
       if (active) {
-        Real part[K1];
+        for (int s0 = j; s0 < pos; s0 += 64 * nsplit) {
+          Real part[K1];
 #pragma unroll
-        for (int k = 0; k < K1; k++) part[k] = 0;
-        for (int s = j; s < pos; s += nsplit) {
-          Real f[K0];
+          for (int k = 0; k < K1; k++) part[k] = 0;
+          const int s1 = min(pos, s0 + 64 * nsplit);
+          for (int s = s0; s < s1; s += nsplit) {
+            Real f[K0];
 #pragma unroll
-          for (int c = 0; c < K0; c++) f[c] = ld[c * SC + s];
-          interact<KIND, Real>(p0 - lc[s], p1 - lc[SC + s], p2 - lc[2 * SC + s], f, part);
+            for (int c = 0; c < K0; c++) f[c] = ld[c * SC + s];
+            interact<KIND, Real>(p0 - lc[s], p1 - lc[SC + s], p2 - lc[2 * SC + s], f, part);
+          }
+#pragma unroll
+          for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
         }
-#pragma unroll
-        for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
       }
     }
     __syncthreads();
