@@ -119,7 +119,7 @@ def roofline(ph: dict, st: dict, m: int, real_bytes: int):
                 "ms_per_launch": ph[dom], "algorithmic_bytes": bytes_alg,
                 "mfma_tflops": flops / t / 1e12, "mfma_frac": flops / t / 1e12 / MI355X_FP32_MFMA_TFLOPS,
                 "work": f"{st['v_pairs']} V1 pairs x {M_dim} frequencies x 16 reals read + {st['v_trg_slots']} "
-                        f"target parents x {M_dim} x 16 reals written + 26 V1 matrices (SURVEY target-stationary "
+                        f"target parents x {M_dim} x 16 reals written + 26 V1 matrices (target-stationary "
                         f"model); the staged kernel moves `traffic` bytes and is MFMA-issue bound"}
     if dom == "U-List":
         name = "k_p2p_points<float,1>"
