@@ -144,160 +144,6 @@ void direct_sum(hipStream_t s, int kind, const double* src, int64_t n_src, const
   HIP_LAUNCH_CHECK();
 }
 
-// ----------------------------------------------------------------------------- P2P: points -> points
-// Coordinates are FP64 leaf-local offsets; the pair displacement is formed in FP64 and then
-// rounded to Real, so that near-coincident pairs keep their digits in the FP32 evaluator.
-template <typename Real, int KIND>
-__global__ __launch_bounds__(64) void k_p2p_points(const int32_t* __restrict__ tleaf, const int32_t* __restrict__ item_ptr,
-                                                   const int32_t* __restrict__ items, const int32_t* __restrict__ trg_begin,
-                                                   const int32_t* __restrict__ trg_end, const int32_t* __restrict__ src_begin,
-                                                   const int32_t* __restrict__ src_end, const double* __restrict__ center,
-                                                   const double* __restrict__ tx, int64_t n_trg, const double* __restrict__ sx,
-                                                   const Real* __restrict__ sden, int64_t n_src, double* __restrict__ out,
-                                                   bool accumulate) {
-  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1;
-  __shared__ double lc[3 * 64];
-  __shared__ Real ld[K0 * 64];
-  const int row = blockIdx.x, lane = threadIdx.x;
-  const int T = tleaf[row];
-  const int tb = trg_begin[T], nt = trg_end[T] - tb;
-  const double ct0 = center[3 * T], ct1 = center[3 * T + 1], ct2 = center[3 * T + 2];
-  const int ib = item_ptr[row], ie = item_ptr[row + 1];
-  for (int c0 = 0; c0 < nt; c0 += 64) {
-    const int ntc = min(64, nt - c0);
-    int tper = 1;
-    while (tper < ntc) tper <<= 1;
-    const int nsplit = 64 / tper;
-    const int t = lane & (tper - 1), j = lane / tper;
-    const bool valid = t < ntc;
-    double p0 = 0, p1 = 0, p2 = 0;
-    if (valid) {
-      p0 = tx[tb + c0 + t];
-      p1 = tx[n_trg + tb + c0 + t];
-      p2 = tx[2 * n_trg + tb + c0 + t];
-    }
-    double acc[K1];
-#pragma unroll
-    for (int k = 0; k < K1; k++) acc[k] = 0;
-    for (int it = ib; it < ie; it++) {
-      const int S = items[it];
-      const int sb = src_begin[S], ns = src_end[S] - sb;
-      const double o0 = center[3 * S] - ct0, o1 = center[3 * S + 1] - ct1, o2 = center[3 * S + 2] - ct2;
-      Real part[K1];
-#pragma unroll
-      for (int k = 0; k < K1; k++) part[k] = 0;
-      for (int s0 = 0; s0 < ns; s0 += 64) {
-        const int nsc = min(64, ns - s0);
-        __syncthreads();
-        if (lane < nsc) {
-          const int64_t g = sb + s0 + lane;
-          lc[lane] = sx[g] + o0;
-          lc[64 + lane] = sx[n_src + g] + o1;
-          lc[128 + lane] = sx[2 * n_src + g] + o2;
-#pragma unroll
-          for (int c = 0; c < K0; c++) ld[c * 64 + lane] = sden[c * n_src + g];
-        }
-        __syncthreads();
-        for (int s = j; s < nsc; s += nsplit) {
-          Real f[K0];
-#pragma unroll
-          for (int c = 0; c < K0; c++) f[c] = ld[c * 64 + s];
-          interact<KIND, Real>((Real)(p0 - lc[s]), (Real)(p1 - lc[64 + s]), (Real)(p2 - lc[128 + s]), f, part);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
-    }
-    for (int o = tper; o < 64; o <<= 1)
-#pragma unroll
-      for (int k = 0; k < K1; k++) acc[k] += __shfl_xor(acc[k], o);
-    if (j == 0 && valid) {
-#pragma unroll
-      for (int k = 0; k < K1; k++) {
-        double* d = &out[k * n_trg + tb + c0 + t];
-        const double v = acc[k] * kscale<KIND>();
-        *d = accumulate ? *d + v : v;
-      }
-    }
-  }
-}
-
-// ----------------------------------------------------------------------------- P2P: surfaces -> points
-template <typename Real, int KIND>
-__global__ __launch_bounds__(64) void k_far_points(const int32_t* __restrict__ tleaf, const int32_t* __restrict__ item_ptr,
-                                                   const int32_t* __restrict__ item_node, const int64_t* __restrict__ item_den,
-                                                   const double* __restrict__ item_rad, const int32_t* __restrict__ trg_begin,
-                                                   const int32_t* __restrict__ trg_end, const double* __restrict__ center,
-                                                   const float* __restrict__ node_size, const Real* __restrict__ unit_surf,
-                                                   int ns, const Real* __restrict__ tx, int64_t n_trg,
-                                                   const Real* __restrict__ dens, double* __restrict__ out, bool accumulate) {
-  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1;
-  __shared__ Real ls[(3 + K0) * 64];
-  const int row = blockIdx.x, lane = threadIdx.x;
-  const int T = tleaf[row];
-  const int tb = trg_begin[T], nt = trg_end[T] - tb;
-  const double ct0 = center[3 * T], ct1 = center[3 * T + 1], ct2 = center[3 * T + 2];
-  const int ib = item_ptr[row], ie = item_ptr[row + 1];
-  for (int c0 = 0; c0 < nt; c0 += 64) {
-    const int ntc = min(64, nt - c0);
-    int tper = 1;
-    while (tper < ntc) tper <<= 1;
-    const int nsplit = 64 / tper;
-    const int t = lane & (tper - 1), j = lane / tper;
-    const bool valid = t < ntc;
-    Real p0 = 0, p1 = 0, p2 = 0;
-    if (valid) {
-      p0 = tx[tb + c0 + t];
-      p1 = tx[n_trg + tb + c0 + t];
-      p2 = tx[2 * n_trg + tb + c0 + t];
-    }
-    double acc[K1];
-#pragma unroll
-    for (int k = 0; k < K1; k++) acc[k] = 0;
-    for (int it = ib; it < ie; it++) {
-      const int Q = item_node[it];
-      const Real* den = dens + item_den[it];
-      const Real sc = (Real)(item_rad[it] * node_size[Q]);
-      const Real o0 = (Real)(center[3 * Q] - ct0), o1 = (Real)(center[3 * Q + 1] - ct1), o2 = (Real)(center[3 * Q + 2] - ct2);
-      Real part[K1];
-#pragma unroll
-      for (int k = 0; k < K1; k++) part[k] = 0;
-      for (int s0 = 0; s0 < ns; s0 += 64) {
-        const int nsc = min(64, ns - s0);
-        __syncthreads();
-        if (lane < nsc) {
-          const int p = s0 + lane;
-          ls[lane] = o0 + sc * unit_surf[3 * p];
-          ls[64 + lane] = o1 + sc * unit_surf[3 * p + 1];
-          ls[128 + lane] = o2 + sc * unit_surf[3 * p + 2];
-#pragma unroll
-          for (int c = 0; c < K0; c++) ls[(3 + c) * 64 + lane] = den[p * K0 + c];
-        }
-        __syncthreads();
-        for (int s = j; s < nsc; s += nsplit) {
-          Real f[K0];
-#pragma unroll
-          for (int c = 0; c < K0; c++) f[c] = ls[(3 + c) * 64 + s];
-          interact<KIND, Real>(p0 - ls[s], p1 - ls[64 + s], p2 - ls[128 + s], f, part);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
-    }
-    for (int o = tper; o < 64; o <<= 1)
-#pragma unroll
-      for (int k = 0; k < K1; k++) acc[k] += __shfl_xor(acc[k], o);
-    if (j == 0 && valid) {
-#pragma unroll
-      for (int k = 0; k < K1; k++) {
-        double* d = &out[k * n_trg + tb + c0 + t];
-        const double v = acc[k] * kscale<KIND>();
-        *d = accumulate ? *d + v : v;
-      }
-    }
-  }
-}
-
 // ----------------------------------------------------------------------------- P2P, 256-thread leaf blocks
 // One target leaf per block of 256 threads.  Threads are (target t, split j) with
 // tid = t + ntc * j, j < nsplit = 256 / ntc (ntc targets in the current chunk of at most 256);
