@@ -234,6 +234,10 @@ def bench_config(name: str, steps: int, warmup: int, check: bool):
         t = timed(lambda: tree.evaluate_device(dq))
         out = tree.evaluate_device(dq)[0].cpu().numpy()
         e = err(c["kernel"], c["src"], c["den"][0], out, 1)
+        tree.set_timing(True)
+        tree.evaluate_device(dq[0].contiguous())
+        log("   phases of one density (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in tree.phase_times().items()))
+        tree.set_timing(False)
         log(f"config 4a: {1e3 * t:.2f} ms for {nd} densities ({n * nd / t / 1e6:.1f} Mtarget-densities/s), rel-L2 {e}")
         line.update(metric="laplace_poten FMM Mtarget-densities/s", value=n * nd / t / 1e6, unit="Mtargets/s",
                     ms_per_step=1e3 * t, dtype="f32", rel_l2=e,

@@ -377,13 +377,15 @@ class DevicePlan final : public DevicePlanBase {
       }
       if (n_vtrg_ > t0.back()) close_tile(n_vtrg_);
       n_vtiles_ = (int)t0.size() - 1;
-      uidx.assign((size_t)n_vtiles_ * 64 * n_off_, 0xFFFF);
+      // a missing partner points at the staging area's zero entry (index umax)
+      uidx.assign((size_t)n_vtiles_ * 64 * n_off_, (uint16_t)umax);
       for (int k = 0; k < n_vtiles_; k++) {
         for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = u - u0[k];
         for (int t = t0[k]; t < t0[k + 1]; t++)
           for (int o = 0; o < n_off_; o++) {
             const int32_t sl = v_partner[(size_t)t * n_off_ + o];
-            if (sl >= 0) uidx[((size_t)k * 64 + (t - t0[k])) * n_off_ + o] = (uint16_t)pos[sl];
+            const int tt = t - t0[k];  // target tt = 16 * group + row of the MFMA tiles
+            if (sl >= 0) uidx[(((size_t)k * n_off_ + o) * 16 + (tt & 15)) * 4 + (tt >> 4)] = (uint16_t)pos[sl];
           }
         for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = -1;
         if (u0[k + 1] - u0[k] > umax || t0[k + 1] - t0[k] > 64)

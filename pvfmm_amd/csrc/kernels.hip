@@ -17,12 +17,6 @@
 
 #include "kernels.hpp"
 
-// Hadamard kernel selection (build-time): 0 tiled with 2 accumulator sets, 1 tiled with one,
-// 2 and 3 persistent double-buffered with 2 or 4 sets.
-#ifndef PVFMM_HADAMARD_VARIANT
-#define PVFMM_HADAMARD_VARIANT 0
-#endif
-
 namespace pvfmm_amd {
 
 #define HIP_LAUNCH_CHECK()                                                       \
@@ -1048,76 +1042,10 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_ifft_down_m(cons
 // Frequency-domain M2L as a real GEMM per frequency f:
 //   Y[t-slots][f][16*E1 reals] = sum over the 26 parent offsets R of
 //   X[partner_R(t)][f][16*E0 reals] x B_R(f)[16*E0 x 16*E1],
-// the complex 8x8 (x E0,E1) transfer block in real form.  A wave owns 16*TG target slots at
-// one frequency; a block's 4 waves take 4 consecutive frequencies.  Blocks are mapped so
-// that all blocks of one frequency group run on one XCD (block b runs on XCD b % 8) and
-// share X[.][f] in that XCD's L2.  Vp holds B pre-permuted per lane: lane l's four values
-// for k-step q are B[16kb + 4(l/16) + q][16nb + l%16], so both operands load as 4-vectors.
-template <typename Real, int E0, int E1, int TG>
-__global__ __launch_bounds__(256) void k_hadamard(int n_slots, int M_dim, int n_tsg, const int32_t* __restrict__ partner,
-                                                  const int32_t* __restrict__ offsets, int n_off, const Real* __restrict__ Vp,
-                                                  const Real* __restrict__ X, Real* __restrict__ Y) {
-  using M = Mfma<Real>;
-  using V4 = typename M::V4;
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
-  const int fg = (local / n_tsg) * 8 + xcd, tsg = local % n_tsg;
-  const int f = fg * 4 + w;
-  if (f >= M_dim) return;  // whole wave
-  const int row = l & 15, kg = l >> 4;
-  V4 acc[TG][E1];
-#pragma unroll
-  for (int g = 0; g < TG; g++)
-#pragma unroll
-    for (int nb = 0; nb < E1; nb++) acc[g][nb] = V4{0, 0, 0, 0};
-  int slot[TG];
-  bool valid[TG];
-#pragma unroll
-  for (int g = 0; g < TG; g++) {
-    slot[g] = (tsg * TG + g) * 16 + row;
-    valid[g] = slot[g] < n_slots;
-  }
-  for (int oi = 0; oi < n_off; oi++) {
-    int p[TG];
-    bool mine = false;
-#pragma unroll
-    for (int g = 0; g < TG; g++) {
-      p[g] = valid[g] ? partner[(int64_t)slot[g] * n_off + oi] : -1;
-      mine |= p[g] >= 0;
-    }
-    if (!__any(mine)) continue;  // no target of this wave has a partner at offset R
-    const int R = offsets[oi];
-    const Real* vb = Vp + (int64_t)((int64_t)R * M_dim + f) * (E0 * E1 * 256);
-#pragma unroll
-    for (int kb = 0; kb < E0; kb++) {
-      V4 bq[E1];
-#pragma unroll
-      for (int nb = 0; nb < E1; nb++) bq[nb] = *reinterpret_cast<const V4*>(vb + ((kb * E1 + nb) * 64 + l) * 4);
-#pragma unroll
-      for (int g = 0; g < TG; g++) {
-        V4 a = V4{0, 0, 0, 0};
-        if (p[g] >= 0) a = *reinterpret_cast<const V4*>(X + ((int64_t)p[g] * M_dim + f) * (16 * E0) + 16 * kb + 4 * kg);
-#pragma unroll
-        for (int nb = 0; nb < E1; nb++) {
-          acc[g][nb] = M::mma(a[0], bq[nb][0], acc[g][nb]);
-          acc[g][nb] = M::mma(a[1], bq[nb][1], acc[g][nb]);
-          acc[g][nb] = M::mma(a[2], bq[nb][2], acc[g][nb]);
-          acc[g][nb] = M::mma(a[3], bq[nb][3], acc[g][nb]);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int g = 0; g < TG; g++)
-#pragma unroll
-    for (int nb = 0; nb < E1; nb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int s = (tsg * TG + g) * 16 + M::row(l, r);
-        if (s < n_slots) Y[((int64_t)s * M_dim + f) * (16 * E1) + 16 * nb + row] = acc[g][nb][r];
-      }
-}
-
+// the complex 8x8 (x E0,E1) transfer block in real form.  Vp holds B pre-permuted per lane:
+// lane l's four values for k-step q are B[16kb + 4(l/16) + q][16nb + l%16], so both MFMA
+// operands load as 4-vectors.
+//
 // Tiled frequency-domain M2L.  A block owns one tile of consecutive target parents (at most
 // 64 = 4 groups of 16, with at most UMAX distinct V1 partners) and FB consecutive
 // frequencies.  The partners' spectra for those frequencies are staged in LDS once (entry
@@ -1128,8 +1056,16 @@ __global__ __launch_bounds__(256) void k_hadamard(int n_slots, int M_dim, int n_
 // partial sums of the waves of one frequency are added through LDS in a fixed order.  The
 // instruction stream has no data-dependent branches, so the accumulators stay in place.
 // Blocks are mapped so that all tiles of one frequency group run on one XCD and share its L2.
+// Blocks of the tiled kernel that fit a CU's 160 KiB of LDS (at most 4); the register
+// budget is set to match, so the compiler keeps to it.
+template <typename Real, int E0, int FB, int UMAX>
+constexpr int hadamard_blocks_per_cu() {
+  constexpr int lds = (UMAX + 1) * (FB * 16 * E0 + 4) * (int)sizeof(Real) + 64 * 26 * 2;
+  return 163840 / lds > 4 ? 4 : 163840 / lds;
+}
+
 template <typename Real, int E0, int E1, int FB, int UMAX, int NACC>
-__global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, const int32_t* __restrict__ tile_t0,
+__global__ __launch_bounds__(256, (hadamard_blocks_per_cu<Real, E0, FB, UMAX>())) void k_hadamard_tiled(int M_dim, int n_tiles, const int32_t* __restrict__ tile_t0,
                                                         const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
                                                         const uint16_t* __restrict__ uidx, const int32_t* __restrict__ offsets,
                                                         const Real* __restrict__ Vp, const Real* __restrict__ X,
@@ -1148,9 +1084,8 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
   // fit in ~104 VGPRs); the first PD are loaded before the staging barrier
   constexpr int PD0 = 104 / (NB * (int)sizeof(V4) / 4);
   constexpr int PD = PD0 < 1 ? 1 : (PD0 > NOW ? NOW : PD0);
-  constexpr uint16_t NONE = 0xFFFF;
   __shared__ __align__(16) Real Xl[(UMAX + 1) * EST];  // entry UMAX stays zero
-  __shared__ uint16_t ul[64 * NOFF];
+  __shared__ __align__(8) uint16_t ul[64 * NOFF];
   const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
   const int fg = (local / n_tiles) * 8 + xcd, tile = local % n_tiles;
   const int f0 = fg * FB;
@@ -1159,14 +1094,6 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
   const int u0 = tile_u0[tile], nu = tile_u0[tile + 1] - u0;
   const int fn = min(FB, M_dim - f0);
   constexpr int V4PU = FB * RW / 4;  // 4-vectors per entry
-  for (int e = threadIdx.x; e < nu * V4PU; e += 256) {
-    const int u = e / V4PU, r = e - u * V4PU, fl = r / (RW / 4);
-    V4 val = V4{0, 0, 0, 0};
-    if (fl < fn) val = *reinterpret_cast<const V4*>(X + ((int64_t)usrc[u0 + u] * M_dim + f0) * RW + 4 * r);
-    *reinterpret_cast<V4*>(&Xl[u * EST + 4 * r]) = val;
-  }
-  for (int e = threadIdx.x; e < 64 * NOFF; e += 256) ul[e] = uidx[(int64_t)tile * 64 * NOFF + e];
-  for (int e = threadIdx.x; e < EST; e += 256) Xl[UMAX * EST + e] = Real(0);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int fl = w / WPF, gsub = w % WPF;
   const bool live = fl < fn;                  // waves past the last frequency compute, not store
@@ -1174,11 +1101,60 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
   const int ob = gsub * NOW;                  // this wave's first offset
   const Real* vf = Vp + (int64_t)f * (NB * 256) + l * 4;
   const int64_t vstride = (int64_t)M_dim * NB * 256;
+  // The prologue keeps its global loads in flight together (one latency each for the index
+  // and data reads instead of one per loop trip): the partner table, the staged spectra in
+  // batches of SB passes, and the transfer blocks (first when the registers allow).
+  constexpr bool V_FIRST = sizeof(Real) == 4 && E0 == 1;
   V4 bq[PD][NB];
+  auto load_v = [&]() {
 #pragma unroll
-  for (int p = 0; p < PD; p++)
+    for (int p = 0; p < PD; p++)
 #pragma unroll
-    for (int q = 0; q < NB; q++) bq[p][q] = *reinterpret_cast<const V4*>(vf + offsets[ob + p] * vstride + q * 256);
+      for (int q = 0; q < NB; q++) bq[p][q] = *reinterpret_cast<const V4*>(vf + offsets[ob + p] * vstride + q * 256);
+  };
+  if constexpr (V_FIRST) load_v();
+  constexpr int UW = 64 * NOFF / 4;  // partner table as 8-byte words
+  constexpr int UWP = (UW + 255) / 256;
+  uint2 uw[UWP];
+  const uint2* ug = reinterpret_cast<const uint2*>(uidx + (int64_t)tile * 64 * NOFF);
+#pragma unroll
+  for (int i = 0; i < UWP; i++) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < UW) uw[i] = ug[e];
+  }
+  constexpr int NPASS = (UMAX * V4PU + 255) / 256;
+  constexpr int SB = sizeof(Real) == 4 ? 8 : 4;
+#pragma unroll
+  for (int p0 = 0; p0 < NPASS; p0 += SB) {
+    if (256 * p0 >= nu * V4PU) break;  // block-uniform
+    int src[SB];
+#pragma unroll
+    for (int i = 0; i < SB; i++) {
+      const int u = (threadIdx.x + 256 * (p0 + i)) / V4PU;
+      src[i] = (p0 + i < NPASS && u < nu) ? usrc[u0 + u] : -1;
+    }
+    V4 val[SB];
+#pragma unroll
+    for (int i = 0; i < SB; i++) {
+      const int e = threadIdx.x + 256 * (p0 + i), u = e / V4PU, r = e - u * V4PU;
+      val[i] = V4{0, 0, 0, 0};
+      if (src[i] >= 0 && r / (RW / 4) < fn)
+        val[i] = *reinterpret_cast<const V4*>(X + ((int64_t)src[i] * M_dim + f0) * RW + 4 * r);
+    }
+#pragma unroll
+    for (int i = 0; i < SB; i++) {
+      const int e = threadIdx.x + 256 * (p0 + i), u = e / V4PU, r = e - u * V4PU;
+      if (p0 + i < NPASS && u < nu) *reinterpret_cast<V4*>(&Xl[u * EST + 4 * r]) = val[i];
+    }
+  }
+  // partner table [offset][row][group]: one 8-byte read per lane and offset gives its 4 groups
+#pragma unroll
+  for (int i = 0; i < UWP; i++) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < UW) reinterpret_cast<uint2*>(ul)[e] = uw[i];
+  }
+  for (int e = threadIdx.x; e < EST; e += 256) Xl[UMAX * EST + e] = Real(0);
+  if constexpr (!V_FIRST) load_v();
   // pin the blocks in registers here: otherwise the loads are sunk to their first use past
   // the barrier and the wave stalls on each of them in the offset loop
 #pragma unroll
@@ -1195,6 +1171,21 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
     for (int g = 0; g < 4; g++)
 #pragma unroll
       for (int nb = 0; nb < E1; nb++) accs[a][g][nb] = V4{0, 0, 0, 0};
+  // A operands (partner spectra of the 4 groups) of the wave's next offset, read one offset
+  // ahead so that the LDS latency overlaps the MFMAs (scalar kernels; the Stokes blocks would
+  // need 3x the registers)
+  // (partner indices two offsets ahead); scheduling barriers keep the reads ahead of the
+  // MFMAs, which the scheduler would otherwise sink next to their use
+  V4 an[4];
+  using U4 = unsigned short __attribute__((ext_vector_type(4)));
+  U4 un;
+  const int lane_off = fl * RW + 4 * kg;
+  if constexpr (E0 == 1) {
+    const U4 uu = *reinterpret_cast<const U4*>(&ul[(ob * 16 + row) * 4]);
+#pragma unroll
+    for (int g = 0; g < 4; g++) an[g] = *reinterpret_cast<const V4*>(&Xl[(int)uu[g] * EST + lane_off]);
+    if (NOW > 1) un = *reinterpret_cast<const U4*>(&ul[((ob + 1) * 16 + row) * 4]);
+  }
 #pragma unroll
   for (int j = 0; j < NOW; j++) {
     const int oi = ob + j;
@@ -1206,10 +1197,31 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
 #pragma unroll
       for (int q = 0; q < NB; q++) bq[j % PD][q] = *reinterpret_cast<const V4*>(vf + offsets[oi + PD] * vstride + q * 256);
     }
+    if constexpr (E0 == 1) {
+      V4 ac[4];
+#pragma unroll
+      for (int g = 0; g < 4; g++) ac[g] = an[g];
+      if (j + 1 < NOW) {
+#pragma unroll
+        for (int g = 0; g < 4; g++) an[g] = *reinterpret_cast<const V4*>(&Xl[(int)un[g] * EST + lane_off]);
+      }
+      if (j + 2 < NOW) un = *reinterpret_cast<const U4*>(&ul[((oi + 2) * 16 + row) * 4]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int g = 0; g < 4; g++)
+#pragma unroll
+        for (int nb = 0; nb < E1; nb++) {
+          acc[g][nb] = MF::mma(ac[g][0], cur[nb][0], acc[g][nb]);
+          acc[g][nb] = MF::mma(ac[g][1], cur[nb][1], acc[g][nb]);
+          acc[g][nb] = MF::mma(ac[g][2], cur[nb][2], acc[g][nb]);
+          acc[g][nb] = MF::mma(ac[g][3], cur[nb][3], acc[g][nb]);
+        }
+      __builtin_amdgcn_sched_barrier(0);
+      continue;
+    }
 #pragma unroll
     for (int g = 0; g < 4; g++) {
-      const uint16_t u = ul[(g * 16 + row) * NOFF + oi];  // rows past the tile's targets hold NONE
-      const int ue = (u == NONE) ? UMAX : (int)u;          // missing partner -> the zero entry
+      const int ue = ul[(oi * 16 + row) * 4 + g];  // missing partners point at the zero entry
 #pragma unroll
       for (int kb = 0; kb < E0; kb++) {
         const V4 a = *reinterpret_cast<const V4*>(&Xl[ue * EST + fl * RW + 16 * kb + 4 * kg]);
@@ -1262,145 +1274,6 @@ __global__ __launch_bounds__(256) void k_hadamard_tiled(int M_dim, int n_tiles, 
         const int tg = g * 16 + MF::row(l, r);
         if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * (16 * E1) + 16 * nb + row] = acc[g][nb][r];
       }
-}
-
-// 16-byte direct global -> LDS copy of one lane (gfx950 global_load_lds_dwordx4): the wave's
-// lanes write 16 bytes each at lds_wave_base + 16 * lane.  The builtin exists only in the
-// device compilation pass; the host pass sees an empty body.
-__device__ __forceinline__ void lds_copy16(const void* g, void* lds_wave_base) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  __builtin_amdgcn_global_load_lds(g, lds_wave_base, 16, 0, 0);
-#endif
-}
-
-// Persistent, double-buffered variant of the tiled M2L for scalar kernels (e0 = e1 = 1).
-// One block per CU, 8 * nper blocks; block b serves XCD x = b % 8, which owns the frequency
-// groups fg = x, x+8, ...  The block walks its work items (tile, fg) in fg-major order,
-// item = b/8 + nper*j.  Four MFMA waves compute one item from one LDS buffer while a fifth
-// wave copies the next item's partner spectra and partner table global -> LDS into the
-// other buffer with the direct LDS loads of gfx950 (global_load_lds, 16 bytes per lane:
-// lane i of a wave writes 16 bytes at the wave's LDS base + 16 i).  Keeping the copies in
-// their own wave keeps the compiler from making the MFMA waves wait on them.  A staged entry
-// is 16 chunks of 16 bytes; chunk q of entry u holds global chunk q ^ (u & 15), which spreads
-// the 16 rows of an MFMA operand read over the LDS banks.  WPF MFMA waves share a frequency,
-// each with TG of the 4 target groups and the 26 transfer blocks of the frequency in
-// registers, reloaded when the block moves to its next frequency group.
-template <typename Real, int FB, int UMAX, int NACC>
-__global__ __launch_bounds__(320) void k_hadamard_pipe(int M_dim, int n_tiles, int n_fg, int nper,
-                                                       const int32_t* __restrict__ tile_t0, const int32_t* __restrict__ tile_u0,
-                                                       const int32_t* __restrict__ usrc, const uint16_t* __restrict__ uidx,
-                                                       const int32_t* __restrict__ offsets, const Real* __restrict__ Vp,
-                                                       const Real* __restrict__ X, Real* __restrict__ Y) {
-  using MF = Mfma<Real>;
-  using V4 = typename MF::V4;
-  constexpr int NOFF = 26;
-  constexpr int RW = 16;                        // reals of one (slot, frequency) spectrum row
-  constexpr int ENT = FB * RW;                  // reals per staged entry
-  static_assert(ENT * (int)sizeof(Real) == 256, "an entry is 16 chunks of 16 bytes");
-  constexpr int CPE = 16, RPC = 16 / (int)sizeof(Real);  // chunks per entry, reals per chunk
-  constexpr int CPF = CPE / FB;                 // chunks per frequency
-  constexpr int NMW = 4, WPF = NMW / FB, TG = 4 / WPF;     // MFMA waves; wave 4 copies
-  static_assert(WPF * FB == NMW && TG * WPF == 4, "wave split");
-  constexpr int UTB = 64 * NOFF * 2;            // partner table bytes per tile
-  static_assert(UTB % 16 == 0, "partner table is whole chunks");
-  constexpr uint16_t NONE = 0xFFFF;
-  __shared__ __align__(16) Real xb[2][(UMAX + 1) * ENT];  // entry UMAX stays zero
-  __shared__ __align__(16) uint16_t ub[2][64 * NOFF];
-  const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
-  const int fgx = n_fg > x ? (n_fg - x + 7) / 8 : 0;
-  const int n_items = fgx * n_tiles;
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  for (int e = tid; e < ENT; e += 5 * 64) {
-    xb[0][UMAX * ENT + e] = Real(0);
-    xb[1][UMAX * ENT + e] = Real(0);
-  }
-  // copy wave: item `it` into buffer `buf`
-  auto copy = [&](int it, int buf) __attribute__((always_inline)) {
-    const int tile = it % n_tiles, f0 = (x + 8 * (it / n_tiles)) * FB, fn = min(FB, M_dim - f0);
-    const int u0 = tile_u0[tile], nu = tile_u0[tile + 1] - u0;
-    for (int c0 = 0; c0 < nu * CPE; c0 += 64) {
-      const int c = c0 + l, u = c / CPE, q = c % CPE, gq = q ^ (u & 15);
-      if (u < nu && gq / CPF < fn)
-        lds_copy16(X + ((int64_t)usrc[u0 + u] * M_dim + f0) * RW + gq * RPC, &xb[buf][(c0 / CPE) * ENT]);
-    }
-    for (int c0 = 0; c0 < UTB / 16; c0 += 64) {
-      const int c = c0 + l;
-      if (c < UTB / 16)
-        lds_copy16(reinterpret_cast<const unsigned char*>(uidx) + (int64_t)tile * UTB + c * 16,
-                   reinterpret_cast<unsigned char*>(&ub[buf][0]) + c0 * 16);
-    }
-  };
-  const int fl = w / WPF, gsub = w % WPF, row = l & 15, kg = l >> 4;
-  int item = k, j = 0;
-  if (w == NMW && item < n_items) copy(item, 0);
-  for (int fgi = 0; fgi < fgx; fgi++) {
-    const int f0 = (x + 8 * fgi) * FB, fn = min(FB, M_dim - f0);
-    const bool live = w < NMW && fl < fn;  // waves past the last frequency compute, not store
-    const int f = f0 + ((w < NMW && fl < fn) ? fl : 0);
-    // the 26 transfer blocks of this wave's frequency (lane layout of the B operand)
-    V4 bq[NOFF];
-    if (w < NMW) {
-      const Real* vf = Vp + (int64_t)f * 256 + l * 4;
-#pragma unroll
-      for (int o = 0; o < NOFF; o++) bq[o] = *reinterpret_cast<const V4*>(vf + (int64_t)offsets[o] * M_dim * 256);
-    }
-    for (; item < (fgi + 1) * n_tiles; item += nper, j++) {
-      const int buf = j & 1;
-      __builtin_amdgcn_s_waitcnt(0);  // the copy wave's copies of `item` landed
-      __syncthreads();                // ... for everyone; the other buffer is free
-      if (w == NMW) {
-        if (item + nper < n_items) copy(item + nper, buf ^ 1);
-        continue;
-      }
-      const int tile = item % n_tiles;
-      const int t0 = tile_t0[tile], nt = tile_t0[tile + 1] - t0;
-      const Real* xs = &xb[buf][0];
-      const uint16_t* us = &ub[buf][0];
-      V4 accs[NACC][TG];  // offset oi uses set oi % NACC: independent MFMA chains
-#pragma unroll
-      for (int a = 0; a < NACC; a++)
-#pragma unroll
-        for (int g = 0; g < TG; g++) accs[a][g] = V4{0, 0, 0, 0};
-#pragma unroll
-      for (int oi = 0; oi < NOFF; oi++) {
-        auto& acc = accs[oi % NACC];
-#pragma unroll
-        for (int g = 0; g < TG; g++) {
-          const uint16_t u = us[((gsub * TG + g) * 16 + row) * NOFF + oi];  // rows past the tile hold NONE
-          const int ue = (u == NONE) ? UMAX : (int)u;
-          const Real* ent = xs + ue * ENT;
-          V4 a;
-          if constexpr (sizeof(Real) == 4) {
-            a = *reinterpret_cast<const V4*>(ent + ((fl * CPF + kg) ^ (ue & 15)) * RPC);
-          } else {
-            using D2 = double __attribute__((ext_vector_type(2)));
-            const int gq = fl * CPF + 2 * kg;
-            const D2 lo = *reinterpret_cast<const D2*>(ent + (gq ^ (ue & 15)) * RPC);
-            const D2 hi = *reinterpret_cast<const D2*>(ent + ((gq + 1) ^ (ue & 15)) * RPC);
-            a = V4{lo[0], lo[1], hi[0], hi[1]};
-          }
-          acc[g] = MF::mma(a[0], bq[oi][0], acc[g]);
-          acc[g] = MF::mma(a[1], bq[oi][1], acc[g]);
-          acc[g] = MF::mma(a[2], bq[oi][2], acc[g]);
-          acc[g] = MF::mma(a[3], bq[oi][3], acc[g]);
-        }
-      }
-      if (live) {
-#pragma unroll
-        for (int g = 0; g < TG; g++) {
-          V4 sum = accs[0][g];
-#pragma unroll
-          for (int a = 1; a < NACC; a++) sum += accs[a][g];
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int tg = (gsub * TG + g) * 16 + MF::row(l, r);
-            if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * 16 + row] = sum[r];
-          }
-        }
-      }
-    }
-  }
-  __builtin_amdgcn_s_waitcnt(0);
 }
 
 // ----------------------------------------------------------------------------- launchers
@@ -1570,31 +1443,11 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
 }
 
 template <typename Real>
-void DeviceKernels<Real>::hadamard(hipStream_t s, int n_slots, int M_dim, int e0, int e1, const int32_t* partner,
-                                   const int32_t* offsets, int n_off, const Real* Vp, const Real* X, Real* Y) {
-  if (n_slots == 0) return;
-  constexpr int TG = 2;
-  const int n_tsg = (n_slots + 16 * TG - 1) / (16 * TG);
-  const int n_fg = (M_dim + 3) / 4;
-  const int64_t nb = (int64_t)n_tsg * ((n_fg + 7) / 8) * 8;
-  if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-  if (e0 == 1 && e1 == 1)
-    hipLaunchKernelGGL((k_hadamard<Real, 1, 1, TG>), dim3((unsigned)nb), dim3(256), 0, s, n_slots, M_dim, n_tsg, partner, offsets,
-                       n_off, Vp, X, Y);
-  else if (e0 == 3 && e1 == 3)
-    hipLaunchKernelGGL((k_hadamard<Real, 3, 3, 1>), dim3((unsigned)((int64_t)((n_slots + 15) / 16) * ((n_fg + 7) / 8) * 8)),
-                       dim3(256), 0, s, n_slots, M_dim, (n_slots + 15) / 16, partner, offsets, n_off, Vp, X, Y);
-  else
-    throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-  HIP_LAUNCH_CHECK();
-}
-
-template <typename Real>
 int DeviceKernels<Real>::hadamard_union_max(int e0) {
   // FP32: 2 frequencies per block and a union of at most 250 keep a block's LDS under 40 KiB
   // (four blocks per CU); FP64: 2 frequencies, 256; Stokes: 80
   if (e0 != 1) return 80;
-  return (sizeof(Real) == 4 && PVFMM_HADAMARD_VARIANT < 2) ? 250 : 256;
+  return sizeof(Real) == 4 ? 250 : 256;
 }
 
 template <typename Real>
@@ -1608,24 +1461,13 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
   const int64_t nb = (int64_t)n_tiles * ((n_fg + 7) / 8) * 8;
   if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   if (e0 == 1 && e1 == 1) {
-#if PVFMM_HADAMARD_VARIANT >= 2
-    // persistent blocks: one per CU (the LDS double buffer fills a CU), grouped 8 per XCD round
-    int dev = 0, n_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      throw_device_error(hipErrorInvalidDevice, __FILE__, __LINE__);
-    const int nper = std::max(1, n_cu / 8);
-    constexpr int NA = PVFMM_HADAMARD_VARIANT == 2 ? 2 : 4;
-    hipLaunchKernelGGL((k_hadamard_pipe<Real, FB, 256, NA>), dim3((unsigned)(8 * nper)), dim3(320), 0, s, M_dim, n_tiles, n_fg, nper,
-                       tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
-#else
-    constexpr int NA = PVFMM_HADAMARD_VARIANT == 1 ? 2 : 1;
+    constexpr int NA = 1;
     constexpr int FBT = 2, UM = sizeof(Real) == 4 ? 250 : 256;
     const int n_fgt = (M_dim + FBT - 1) / FBT;
     const int64_t nbt = (int64_t)n_tiles * ((n_fgt + 7) / 8) * 8;
     if (nbt >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
     hipLaunchKernelGGL((k_hadamard_tiled<Real, 1, 1, FBT, UM, NA>), dim3((unsigned)nbt), dim3(256), 0, s, M_dim, n_tiles,
                        tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
-#endif
   } else if (e0 == 3 && e1 == 3) {
     hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, FB, 80, 1>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
                        tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);

@@ -56,13 +56,12 @@ struct DeviceKernels {
   static void fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0,
                      int m, const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X);
   // Y[slot][f][8*e1] = sum_R V[R][f] X[partner[slot][R]][f] on MFMA; Vp is the transfer block
-  // in real form, pre-permuted per lane (layout [R][f][kb][nb][lane 64][4]).
-  static void hadamard(hipStream_t s, int n_slots, int M_dim, int e0, int e1, const int32_t* partner,
-                       const int32_t* offsets, int n_off, const Real* Vp, const Real* X, Real* Y);
-  // Tiled variant: tiles of at most 64 consecutive target slots [tile_t0[k], tile_t0[k+1])
+  // in real form, pre-permuted per lane (layout [R][f][kb][nb][lane 64][4]).  The target slots
+  // form tiles of at most 64 consecutive slots [tile_t0[k], tile_t0[k+1])
   // whose distinct partner source slots usrc[tile_u0[k] .. tile_u0[k+1]) (at most
-  // hadamard_union_max(e0)) are staged in LDS; uidx[k][64][n_off] is the partner's position
-  // in that list, 0xFFFF for none.
+  // hadamard_union_max(e0)) are staged in LDS; uidx[k][n_off][16][4] holds, for offset o and
+  // tile target 16 * g + r at [k][o][r][g], the partner's position in that list, or
+  // hadamard_union_max(e0) (the zero entry) for none.
   static int hadamard_union_max(int e0);
   static void hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
                              const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx, const int32_t* offsets,
