@@ -393,6 +393,7 @@ class DevicePlan final : public DevicePlanBase {
       }
       stats.v_tiles = n_vtiles_;
       stats.v_tile_partners = (int64_t)usrc.size();
+      usrc.push_back(0);  // one readable slot past the end: the kernels read ahead branch-free
       vt_t0_.upload(t0, &total); vt_u0_.upload(u0, &total); vt_usrc_.upload(usrc, &total); vt_uidx_.upload(uidx, &total);
     }
     X_.alloc((int64_t)n_vsrc_ * M_dim_ * 16 * e0_, &total);
