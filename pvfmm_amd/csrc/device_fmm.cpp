@@ -106,6 +106,7 @@ class DevicePlan final : public DevicePlanBase {
   // V-list target tiles for the LDS-staged Hadamard
   DBuf<int32_t> vt_t0_, vt_u0_, vt_usrc_;
   DBuf<uint16_t> vt_uidx_;
+  DBuf<uint32_t> vt_gmask_;
   int n_vtiles_ = 0;
   // L2T
   DBuf<int32_t> l2t_ra_, l2t_rc_, l2t_seq_;
@@ -379,13 +380,17 @@ class DevicePlan final : public DevicePlanBase {
       n_vtiles_ = (int)t0.size() - 1;
       // a missing partner points at the staging area's zero entry (index umax)
       uidx.assign((size_t)n_vtiles_ * 64 * n_off_, (uint16_t)umax);
+      std::vector<uint32_t> gmask((size_t)n_vtiles_ * 4, 0u);  // offsets present per 16-target group
       for (int k = 0; k < n_vtiles_; k++) {
         for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = u - u0[k];
         for (int t = t0[k]; t < t0[k + 1]; t++)
           for (int o = 0; o < n_off_; o++) {
             const int32_t sl = v_partner[(size_t)t * n_off_ + o];
             const int tt = t - t0[k];  // target tt = 16 * group + row of the MFMA tiles
-            if (sl >= 0) uidx[(((size_t)k * n_off_ + o) * 16 + (tt & 15)) * 4 + (tt >> 4)] = (uint16_t)pos[sl];
+            if (sl >= 0) {
+              uidx[(((size_t)k * n_off_ + o) * 16 + (tt & 15)) * 4 + (tt >> 4)] = (uint16_t)pos[sl];
+              gmask[(size_t)k * 4 + (tt >> 4)] |= 1u << o;
+            }
           }
         for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = -1;
         if (u0[k + 1] - u0[k] > umax || t0[k + 1] - t0[k] > 64)
@@ -395,6 +400,7 @@ class DevicePlan final : public DevicePlanBase {
       stats.v_tile_partners = (int64_t)usrc.size();
       usrc.push_back(0);  // one readable slot past the end: the kernels read ahead branch-free
       vt_t0_.upload(t0, &total); vt_u0_.upload(u0, &total); vt_usrc_.upload(usrc, &total); vt_uidx_.upload(uidx, &total);
+      vt_gmask_.upload(gmask, &total);
     }
     X_.alloc((int64_t)n_vsrc_ * M_dim_ * 16 * e0_, &total);
     Y_.alloc((int64_t)n_vtrg_ * M_dim_ * 16 * e1_, &total);
@@ -481,7 +487,7 @@ class DevicePlan final : public DevicePlanBase {
     // V list
     K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p, X_.p);
     mark(6);
-    K::hadamard_tiled(st_, n_vtiles_, M_dim_, e0_, e1_, n_off_, vt_t0_.p, vt_u0_.p, vt_usrc_.p, vt_uidx_.p, v_offsets_.p, V_.p,
+    K::hadamard_tiled(st_, n_vtiles_, M_dim_, e0_, e1_, n_off_, vt_t0_.p, vt_u0_.p, vt_usrc_.p, vt_uidx_.p, vt_gmask_.p, v_offsets_.p, V_.p,
                       X_.p, Y_.p);
     mark(7);
     K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p, K1_, e1_, m_, lattice_.p, lattice_inv_.p, ns_, v_trg_scal_.p);

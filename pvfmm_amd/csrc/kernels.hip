@@ -1289,8 +1289,8 @@ template <int UMAX>
 __global__ __launch_bounds__(256, 2) void k_hadamard_stream(int M_dim, int n_tiles, int tiles_per_block, int n_chunks,
                                                            const int32_t* __restrict__ tile_t0,
                                                            const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
-                                                           const uint16_t* __restrict__ uidx, const int32_t* __restrict__ offsets,
-                                                           const float* __restrict__ Vp, const float* __restrict__ X,
+                                                           const uint16_t* __restrict__ uidx, const uint32_t* __restrict__ gmask,
+                                                           const int32_t* __restrict__ offsets, const float* __restrict__ Vp, const float* __restrict__ X,
                                                            float* __restrict__ Y) {
   using MF = Mfma<float>;
   using V4 = MF::V4;
@@ -1390,6 +1390,8 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_stream(int M_dim, int n_til
   for (int k = k0; k < k1; k++) {
     const int buf = (k - k0) & 1;
     const int nu1 = nu_of(k + 1), nu2 = nu_of(k + 2), u02 = u0_of(k + 2);
+    // offsets at which a target of the wave's two groups has a partner (others are skipped)
+    const uint32_t wm = __builtin_amdgcn_readfirstlane(gmask[4 * k + 2 * gh] | gmask[4 * k + 2 * gh + 1]);
     const float* xb = Xl[buf];
     const uint16_t* ub = ul[buf];
     // partner positions of the wave's 2 groups at offset o: one 4-byte LDS read
@@ -1441,11 +1443,14 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_stream(int M_dim, int n_til
       if (j == NPASS) load_uw(k + 1);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int q = 0; q < 4; q++)
+      for (int a = 0; a < 2; a++) {
+        if ((wm >> (2 * j + a)) & 1u) {
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+          for (int q = 0; q < 4; q++)
 #pragma unroll
-          for (int g = 0; g < 2; g++) acc[a][g] = MF::mma(ac[a][g][q], bq[2 * j + a][q], acc[a][g]);
+            for (int g = 0; g < 2; g++) acc[a][g] = MF::mma(ac[a][g][q], bq[2 * j + a][q], acc[a][g]);
+        }
+      }
       __builtin_amdgcn_sched_barrier(0);
     }
     store_lds(buf ^ 1, nu1);  // tile k+1 (the other buffer is free since the last barrier)
@@ -1642,7 +1647,7 @@ int DeviceKernels<Real>::hadamard_union_max(int e0) {
 template <typename Real>
 void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
                                          const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx,
-                                         const int32_t* offsets, const Real* Vp, const Real* X, Real* Y) {
+                                         const uint32_t* gmask, const int32_t* offsets, const Real* Vp, const Real* X, Real* Y) {
   if (n_tiles == 0) return;
   if (n_off != 26) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // the 26 V1 parent offsets
   constexpr int FB = sizeof(Real) == 4 ? 4 : 2;
@@ -1659,7 +1664,7 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
     const int64_t nbs = (int64_t)n_chunks * ((n_fgs + 7) / 8) * 8;
     if (nbs >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
     hipLaunchKernelGGL(k_hadamard_stream<250>, dim3((unsigned)nbs), dim3(256), 0, s, M_dim, n_tiles, TPB, n_chunks, tile_t0,
-                       tile_u0, usrc, uidx, offsets, reinterpret_cast<const float*>(Vp), reinterpret_cast<const float*>(X),
+                       tile_u0, usrc, uidx, gmask, offsets, reinterpret_cast<const float*>(Vp), reinterpret_cast<const float*>(X),
                        reinterpret_cast<float*>(Y));
   } else if (e0 == 1 && e1 == 1) {
     constexpr int NA = 1;

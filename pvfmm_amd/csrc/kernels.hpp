@@ -61,10 +61,12 @@ struct DeviceKernels {
   // whose distinct partner source slots usrc[tile_u0[k] .. tile_u0[k+1]) (at most
   // hadamard_union_max(e0)) are staged in LDS; uidx[k][n_off][16][4] holds, for offset o and
   // tile target 16 * g + r at [k][o][r][g], the partner's position in that list, or
-  // hadamard_union_max(e0) (the zero entry) for none.
+  // hadamard_union_max(e0) (the zero entry) for none; bit o of gmask[k][g] is set when a
+  // target of group g of tile k has a partner at offset o.
   static int hadamard_union_max(int e0);
   static void hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
-                             const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx, const int32_t* offsets,
+                             const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx, const uint32_t* gmask,
+                             const int32_t* offsets,
                              const Real* Vp, const Real* X, Real* Y);
   // inverse DFT, pick the check-surface points, dn[child] += scal * value
   static void ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn,
