@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MI355X_FP32_MFMA_TFLOPS = 157.3
+MI355X_FP32_VALU_TFLOPS = 78.6  # non-packed FP32 FMA (packed FP32 is not used)
 MI355X_FP64_TFLOPS = 78.6
 MI355X_HBM_GBS = 8000.0
 
@@ -97,36 +98,36 @@ def read_traffic(kernel: str):
 def roofline(ph: dict, st: dict, m: int, real_bytes: int):
     """Roofline of the dominant kernel of the mixed evaluator from algorithmic counts.
 
-    V-list Hadamard (target-stationary model): per (V1 pair, frequency) the source spectrum
-    16*k0 reals are read, per (target parent, frequency) 16*k1 reals are written, and the 26 V1
-    matrices (128*k0*k1 reals per frequency) are read once; it is HBM-bound in that model
-    (512*k0*k1 flop per pair and frequency is ~2.5 flop/byte).  U-list P2P: 16 flop per pair
-    for laplace_grad, VALU-bound; its bytes are each target leaf's points and outputs plus the
-    U sources it streams."""
+    V-list Hadamard: 512*k0*k1 flop per (V1 pair, frequency) (the reference's own count,
+    include/fmm_pts.txx:3336) against the FP32 MFMA peak.  The kernel stages each tile's
+    partner spectra in LDS once, so it moves far fewer bytes than the target-stationary model
+    (per pair and frequency 16*k0 reals read, per target parent 16*k1 written, the 26 V1
+    matrices read once), which it would otherwise be bound by; that model's GB/s is reported
+    beside it, and `traffic` is the measured HBM bytes per launch.  U-list P2P: 16 flop per
+    pair for laplace_grad, VALU-bound."""
     n1 = 2 * m
     M_dim = n1 * n1 * (n1 // 2 + 1)
     dom = max(ph, key=ph.get)
     t = ph[dom] * 1e-3
     if dom == "V-List:Hadamard":
-        name = "k_hadamard_tiled<float, 1, 1, 2, 250, 1>"
-        bytes_alg = (16.0 * M_dim * real_bytes * st["v_pairs"] + 16.0 * M_dim * real_bytes * st["v_trg_slots"]
-                     + 26.0 * M_dim * 128 * real_bytes)
+        name = "k_hadamard_stream<250>"
+        bytes_model = (16.0 * M_dim * real_bytes * st["v_pairs"] + 16.0 * M_dim * real_bytes * st["v_trg_slots"]
+                       + 26.0 * M_dim * 128 * real_bytes)
         flops = 512.0 * st["v_pairs"] * M_dim
-        achieved = bytes_alg / t / 1e9
-        traffic = read_traffic(name)
-        return {"kernel": name + " (V-list frequency-domain M2L)", "bound": "hbm", "achieved": achieved,
-                "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS, "traffic": traffic,
-                "ms_per_launch": ph[dom], "algorithmic_bytes": bytes_alg,
-                "mfma_tflops": flops / t / 1e12, "mfma_frac": flops / t / 1e12 / MI355X_FP32_MFMA_TFLOPS,
-                "work": f"{st['v_pairs']} V1 pairs x {M_dim} frequencies x 16 reals read + {st['v_trg_slots']} "
-                        f"target parents x {M_dim} x 16 reals written + 26 V1 matrices (target-stationary "
-                        f"model); the staged kernel moves `traffic` bytes and is MFMA-issue bound"}
+        achieved = flops / t / 1e12
+        return {"kernel": name + " (V-list frequency-domain M2L)", "bound": "mfma", "achieved": achieved,
+                "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_MFMA_TFLOPS,
+                "traffic": read_traffic(name), "ms_per_launch": ph[dom], "algorithmic_flops": flops,
+                "target_stationary_model_bytes": bytes_model,
+                "target_stationary_model_gbs": bytes_model / t / 1e9,
+                "work": f"512 flop x {st['v_pairs']} V1 pairs x {M_dim} frequencies; the kernel runs dense "
+                        f"16-target MFMA groups (offsets without partners skipped) from LDS-staged spectra"}
     if dom == "U-List":
-        name = "k_p2p_points<float,1>"
+        name = "k_p2p_leaf<float, 1>"
         flops = 16.0 * st["u_pairs"]
         achieved = flops / t / 1e12
-        return {"kernel": name + " (U-list P2P)", "bound": "valu", "achieved": achieved, "peak": MI355X_FP32_MFMA_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_MFMA_TFLOPS, "traffic": read_traffic(name),
+        return {"kernel": name + " (U-list P2P)", "bound": "valu", "achieved": achieved, "peak": MI355X_FP32_VALU_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_VALU_TFLOPS, "traffic": read_traffic(name),
                 "ms_per_launch": ph[dom], "work": f"16 flop x {st['u_pairs']} pairs"}
     return {"kernel": dom, "bound": "hbm", "achieved": None, "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": None,
             "traffic": None, "ms_per_launch": ph[dom]}
