@@ -428,7 +428,13 @@ __global__ __launch_bounds__(256) void k_gemm_rows(int M, int N, int K, int kchu
   __shared__ __align__(16) Real Bt[BN * LD];
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  // tiles in row-major order, cut into 8 contiguous ranges, one per XCD (block b runs on XCD
+  // b mod 8): the column tiles of a row tile run together on one XCD and share its A rows in L2
+  const int gx = (N + BN - 1) / BN, n_tiles = gx * ((M + BM - 1) / BM);
+  const int per = (n_tiles + 7) / 8;
+  const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (tile >= n_tiles) return;  // whole block
+  const int m0 = (tile / gx) * BM, n0 = (tile % gx) * BN;
   const int kb0 = blockIdx.z * kchunk, kb1 = min(K, kb0 + kchunk);
   // global -> register -> LDS assignments: A: 8 consecutive k of one row; B: 8 consecutive n of one k
   const int ar = tid >> 2, ak = (tid & 3) * 8;
@@ -1563,13 +1569,14 @@ void DeviceKernels<Real>::gemm_rows(hipStream_t s, int M, int N, int K, const Re
   // split K when the tile grid cannot fill the 256 CUs, in chunks of whole 16-wide k steps
   int ksplit = 1;
   const int blocks = gx * gy;
+  if ((int64_t)blocks + 8 >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   if (work && blocks < 512) {
     ksplit = std::min((1024 + blocks - 1) / blocks, (K + 63) / 64);
     while (ksplit > 1 && (int64_t)ksplit * M * N > work_elems) ksplit--;
   }
   const int kchunk = ((K + ksplit - 1) / ksplit + 31) / 32 * 32;
   ksplit = (K + kchunk - 1) / kchunk;
-  dim3 g(gx, gy, ksplit);
+  dim3 g((unsigned)((blocks + 7) / 8 * 8), 1, ksplit);
   // 4-vector loads when every row start and the B rows are 16-byte aligned
   const bool vec = lda % 4 == 0 && N % 4 == 0 && (reinterpret_cast<uintptr_t>(A) % 32) == 0 &&
                    (reinterpret_cast<uintptr_t>(B) % 32) == 0;
