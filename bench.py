@@ -246,26 +246,38 @@ def bench_config(name: str, steps: int, warmup: int, check: bool):
                                         f"max_pts={c['max_pts']}, mixed", "config": "4a"})
     elif name == "4b":
         c = W.config4b()
-        trees, dens = [], []
-        for x, q in c["clouds"]:
-            trees.append(P.PtFMMTree(x, x, c["max_pts"]).setup(P.PtFMM(c["kernel"], c["m"], "mixed")))
-            dens.append(torch.from_numpy(q).to("cuda:0"))
+        fmm = P.PtFMM(c["kernel"], c["m"], "mixed")
+        trees = [P.PtFMMTree(x, x, c["max_pts"]).setup(fmm) for x, _ in c["clouds"]]
+        dens = [torch.from_numpy(q).to("cuda:0") for _, q in c["clouds"]]
         outs = [torch.empty_like(d) for d in dens]
         ntot = sum(len(q) for _, q in c["clouds"])
 
-        def run_all():
+        def run_streams():
             for tr, d, o in zip(trees, dens, outs):
                 tr.evaluate_device(d, out=o, sync=False)  # each tree has its own stream
             for tr in trees:
                 torch.cuda.ExternalStream(tr.stream_handle()).synchronize()
 
-        t = timed(run_all)
+        t_streams = timed(run_streams)
+        # launch shapes per cloud from the autotuner, then one graph launch for all clouds
+        t0 = time.perf_counter()
+        tunes = [tr.autotune(d) for tr, d in zip(trees, dens)]
+        t_tune = time.perf_counter() - t0
+        batch = P.PtFMMBatch(trees)
+        t = timed(lambda: batch.evaluate_device(dens, outs))
         e = err(c["kernel"], c["clouds"][0][0], c["clouds"][0][1], outs[0].cpu().numpy(), 1)
-        log(f"config 4b: {1e3 * t:.2f} ms for {len(trees)} clouds, {ntot} targets ({ntot / t / 1e6:.1f} Mtargets/s)")
+        shapes = {}
+        for tu in tunes:
+            key = f"{tu['hadamard_tiles_per_block']}/{tu['gemm_split_blocks']}"
+            shapes[key] = shapes.get(key, 0) + 1
+        log(f"config 4b: {1e3 * t:.2f} ms for {len(trees)} clouds, {ntot} targets ({ntot / t / 1e6:.1f} Mtargets/s) "
+            f"as one graph launch; {1e3 * t_streams:.2f} ms with one stream per cloud before tuning; "
+            f"autotune {t_tune:.1f} s, shapes (tiles per block / split blocks: clouds) {shapes}")
         line.update(metric="laplace_poten FMM Mtargets/s over 64 clouds", value=ntot / t / 1e6, unit="Mtargets/s",
-                    ms_per_step=1e3 * t, dtype="f32", rel_l2_cloud0=e,
+                    ms_per_step=1e3 * t, dtype="f32", rel_l2_cloud0=e, streams_ms=1e3 * t_streams,
+                    autotune_s=t_tune, autotuned_shapes=shapes, graph_records=batch.records(),
                     config={"workload": f"64 uniform clouds, {ntot} points, m={c['m']}, max_pts={c['max_pts']}, mixed, "
-                                        "one stream per cloud", "config": "4b"})
+                                        "autotuned launch shapes, one HIP graph launch", "config": "4b"})
     elif name == "5":
         bf = bwd_fwd(steps, warmup, 1048576)
         line.update(metric="laplace energy t_bwd / t_fwd", value=bf["ratio"], unit="ratio", higher_is_better=False,

@@ -146,6 +146,29 @@ int pvfmm_amd_tree_evaluate_device(void* tree, const double* d_den, double* d_ou
 /* The HIP stream (hipStream_t) the tree's kernels run on. */
 void* pvfmm_amd_tree_stream(void* tree);
 
+/* Launch-shape tuning of a set-up tree; no counterpart in the reference (its CPU kernels have
+ * no launch shapes).  params = {hadamard_tiles_per_block, gemm_split_blocks}.
+ * autotune times the candidates on this tree with one device-resident density vector d_den
+ * (as pvfmm_amd_tree_evaluate_device, n_vec = 1; d_out is overwritten), keeps the fastest and
+ * writes the chosen values to params[0..cap).  Returns 0 or an error code. */
+int pvfmm_amd_tree_autotune(void* tree, const double* d_den, double* d_out, int* params, int cap);
+int pvfmm_amd_tree_get_tuning(void* tree, int* params, int cap);
+int pvfmm_amd_tree_set_tuning(void* tree, const int* params, int n);
+
+/* Many set-up trees evaluated together (independent point clouds); no counterpart in the
+ * reference, whose callers keep one context per cloud.  The first evaluation records every
+ * tree's evaluation (one density vector each, device-resident) into one HIP graph with one
+ * branch per tree; later calls with the same device pointers replay it with a single graph
+ * launch on the batch's stream.  Other pointers, or changed launch shapes of a tree, record
+ * the graph again.  The trees must outlive the batch and must not be set up again while it
+ * exists. */
+void* pvfmm_amd_batch_create(void* const* trees, int n_trees);
+int pvfmm_amd_batch_evaluate_device(void* batch, const double* const* d_den, double* const* d_out, int async);
+void* pvfmm_amd_batch_stream(void* batch);
+/* number of times the graph was recorded (1 while the pointers stay the same) */
+int pvfmm_amd_batch_records(void* batch);
+void pvfmm_amd_batch_destroy(void* batch);
+
 /* Introspection for parity checks. */
 int64_t pvfmm_amd_tree_num_nodes(void* tree);
 /* per node (level-major order): x, y, z (units 2^-30), depth, is_leaf, parent, n_src, n_trg,

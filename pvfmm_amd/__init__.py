@@ -10,6 +10,7 @@ from .wrapper import (  # noqa: F401
     FMMKernel,
     FMMParticleContext,
     PtFMM,
+    PtFMMBatch,
     PtFMMTree,
     device_count,
     direct,
@@ -23,6 +24,7 @@ __all__ = [
     "KERNEL_DIMS",
     "SUPPORTED_KERNELS",
     "PtFMM",
+    "PtFMMBatch",
     "PtFMMTree",
     "device_count",
     "direct",

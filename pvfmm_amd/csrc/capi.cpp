@@ -106,6 +106,95 @@ void eval_device(TreeHandle* t, const double* d_den, double* d_out, int n_vec, d
   t->plan->evaluate(d_den, d_out, n_vec, den_scale, out_scale);
 }
 
+// Independent trees evaluated as one HIP graph: a fork event on the batch stream, each
+// tree's evaluation on its own stream, a join back.  Recorded once per set of device pointers.
+struct BatchHandle {
+  std::vector<TreeHandle*> trees;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const double*> den_key;
+  std::vector<double*> out_key;
+  std::vector<std::pair<int, int>> tune_key;  // launch shapes baked into the graph
+  int records = 0;
+  std::vector<std::pair<int, int>> tunings() const {
+    std::vector<std::pair<int, int>> v;
+    for (TreeHandle* t : trees) v.emplace_back(t->plan->tuning.hadamard_tpb, t->plan->tuning.gemm_split_blocks);
+    return v;
+  }
+  ~BatchHandle() {
+    drop_graph();
+    for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  void record(const double* const* d_den, double* const* d_out) {
+    drop_graph();
+    std::vector<bool> timing(trees.size());
+    HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      HIP_CHECK(hipEventRecord(fork, stream));
+      for (size_t i = 0; i < trees.size(); i++) {
+        TreeHandle* t = trees[i];
+        HIP_CHECK(hipStreamWaitEvent(t->stream, fork, 0));
+        t->plan->evaluate(d_den[i], d_out[i], 1, 1.0, 1.0);
+        HIP_CHECK(hipEventRecord(joins[i], t->stream));
+        HIP_CHECK(hipStreamWaitEvent(stream, joins[i], 0));
+      }
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(stream, &graph));
+    HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    den_key.assign(d_den, d_den + trees.size());
+    out_key.assign(d_out, d_out + trees.size());
+    tune_key = tunings();
+    records++;
+  }
+};
+
+BatchHandle* make_batch(void* const* trees, int n) {
+  if (n <= 0 || !trees) throw Error(kErrInvalidArgument, "a batch needs at least one tree");
+  auto b = std::make_unique<BatchHandle>();
+  for (int i = 0; i < n; i++) {
+    TreeHandle* t = (TreeHandle*)trees[i];
+    if (!t || !t->plan) throw Error(kErrState, "every tree of a batch must be set up");
+    b->trees.push_back(t);
+  }
+  HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  b->joins.resize(n, nullptr);
+  for (int i = 0; i < n; i++) HIP_CHECK(hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming));
+  return b.release();
+}
+
+void batch_eval(BatchHandle* b, const double* const* d_den, double* const* d_out, bool async) {
+  if (!b) throw Error(kErrInvalidArgument, "null batch");
+  if (!d_den || !d_out) throw Error(kErrInvalidArgument, "null pointer array");
+  const size_t n = b->trees.size();
+  for (size_t i = 0; i < n; i++) {
+    if (!d_den[i] || !d_out[i]) throw Error(kErrInvalidArgument, "null device array");
+    if (!b->trees[i]->plan) throw Error(kErrState, "a tree of the batch lost its plan");
+    b->trees[i]->plan->set_timing(false);  // event records stay out of the graph
+  }
+  const bool same = b->exec && std::equal(b->den_key.begin(), b->den_key.end(), d_den) &&
+                    std::equal(b->out_key.begin(), b->out_key.end(), d_out) && b->tune_key == b->tunings();
+  if (!same) b->record(d_den, d_out);
+  HIP_CHECK(hipGraphLaunch(b->exec, b->stream));
+  if (!async) HIP_CHECK(hipStreamSynchronize(b->stream));
+}
+
 void eval_host(TreeHandle* t, const double* den, double* out, int n_vec, double den_scale, double out_scale) {
   if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
   const KernelInfo& ki = t->fmm->O.ker;
@@ -329,6 +418,48 @@ int pvfmm_amd_tree_evaluate_device(void* tree, const double* d_den, double* d_ou
   });
 }
 void* pvfmm_amd_tree_stream(void* tree) { return tree ? (void*)((TreeHandle*)tree)->stream : nullptr; }
+
+int pvfmm_amd_tree_autotune(void* tree, const double* d_den, double* d_out, int* params, int cap) {
+  return guarded([&] {
+    TreeHandle* t = (TreeHandle*)tree;
+    if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
+    if (!d_den || !d_out) throw Error(kErrInvalidArgument, "null device array");
+    const Tuning tu = t->plan->autotune(d_den, d_out);
+    HIP_CHECK(hipStreamSynchronize(t->stream));
+    const int v[2] = {tu.hadamard_tpb, tu.gemm_split_blocks};
+    for (int i = 0; i < std::min(cap, 2); i++) params[i] = v[i];
+  });
+}
+int pvfmm_amd_tree_get_tuning(void* tree, int* params, int cap) {
+  return guarded([&] {
+    TreeHandle* t = (TreeHandle*)tree;
+    if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
+    const int v[2] = {t->plan->tuning.hadamard_tpb, t->plan->tuning.gemm_split_blocks};
+    for (int i = 0; i < std::min(cap, 2); i++) params[i] = v[i];
+  });
+}
+int pvfmm_amd_tree_set_tuning(void* tree, const int* params, int n) {
+  return guarded([&] {
+    TreeHandle* t = (TreeHandle*)tree;
+    if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
+    if (n < 2 || !params) throw Error(kErrInvalidArgument, "two tuning values expected");
+    if (params[0] < 1 || params[1] < 0) throw Error(kErrInvalidArgument, "tiles per block >= 1, split blocks >= 0");
+    t->plan->tuning.hadamard_tpb = params[0];
+    t->plan->tuning.gemm_split_blocks = params[1];
+  });
+}
+
+void* pvfmm_amd_batch_create(void* const* trees, int n_trees) {
+  BatchHandle* b = nullptr;
+  guarded([&] { b = make_batch(trees, n_trees); });
+  return b;
+}
+int pvfmm_amd_batch_evaluate_device(void* batch, const double* const* d_den, double* const* d_out, int async) {
+  return guarded([&] { batch_eval((BatchHandle*)batch, d_den, d_out, async != 0); });
+}
+void* pvfmm_amd_batch_stream(void* batch) { return batch ? (void*)((BatchHandle*)batch)->stream : nullptr; }
+int pvfmm_amd_batch_records(void* batch) { return batch ? ((BatchHandle*)batch)->records : -1; }
+void pvfmm_amd_batch_destroy(void* batch) { delete (BatchHandle*)batch; }
 
 int64_t pvfmm_amd_tree_num_nodes(void* tree) { return tree ? ((TreeHandle*)tree)->T.n_nodes() : -1; }
 int pvfmm_amd_tree_nodes(void* tree, int64_t* out) {

@@ -65,6 +65,41 @@ class DevicePlan final : public DevicePlanBase {
       evaluate_one(d_den + (int64_t)v * k0_ * n_src_, d_out + (int64_t)v * k1_ * n_trg_, den_scale, out_scale);
   }
 
+  Tuning autotune(const double* d_den, double* d_out) override {
+    const bool was = timing_;
+    timing_ = true;
+    // fastest per-phase times over a few evaluations with the current tuning
+    auto measure = [&]() {
+      std::vector<double> best(kPhaseCount, 1e300);
+      for (int r = 0; r < 3; r++) {
+        evaluate_one(d_den, d_out, 1.0, 1.0);
+        for (int p = 0; p < kPhaseCount; p++) best[p] = std::min(best[p], phase_ms[p]);
+      }
+      return best;
+    };
+    auto pick = [&](int& knob, std::initializer_list<int> cands, std::initializer_list<int> phases) {
+      int best_v = knob;
+      double best_t = 1e300;
+      for (int v : cands) {
+        knob = v;
+        const std::vector<double> ph = measure();
+        double t = 0;
+        for (int p : phases) t += ph[p];
+        if (t < best_t) { best_t = t; best_v = v; }
+      }
+      knob = best_v;
+    };
+    try {
+      if (sizeof(Real) == 4 && e0_ == 1 && n_vtiles_ > 0) pick(tuning.hadamard_tpb, {4, 8, 16, 32, 64}, {kPhHadamard});
+      pick(tuning.gemm_split_blocks, {0, 256, 512, 1024, 2048}, {kPhS2UGemm, kPhU2U, kPhD2D, kPhD2TGemm});
+    } catch (...) {
+      timing_ = was;
+      throw;
+    }
+    timing_ = was;
+    return tuning;
+  }
+
  private:
   using K = DeviceKernels<Real>;
   const Operators& O_;
@@ -471,13 +506,13 @@ class DevicePlan final : public DevicePlanBase {
     K::p2p_surface(st_, ek, n_s2m_, s2m_node_.p, s2m_ptr_.p, s2m_items_.p, src_begin_.p, src_end_.p, center_.p, node_size_.p,
                    kRad1, unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, chk_.p, s2m_orow_.p, K1_, false);
     mark(2);
-    K::gemm_rows(st_, n_s2m_, ru_, K1_, chk_.p, s2m_orow_.p, K1_, uc2ue0_.p, ctmp_.p, s2m_orow_.p, ru_, s2m_alpha_.p, false, gemm_work_.p, kGemmWork);
-    K::gemm_rows(st_, n_s2m_, K0_, ru_, ctmp_.p, s2m_orow_.p, ru_, uc2ue1_.p, eq_.p, s2m_node_.p, K0_, nullptr, false, gemm_work_.p, kGemmWork);
+    K::gemm_rows(st_, n_s2m_, ru_, K1_, chk_.p, s2m_orow_.p, K1_, uc2ue0_.p, ctmp_.p, s2m_orow_.p, ru_, s2m_alpha_.p, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
+    K::gemm_rows(st_, n_s2m_, K0_, ru_, ctmp_.p, s2m_orow_.p, ru_, uc2ue1_.p, eq_.p, s2m_node_.p, K0_, nullptr, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
     mark(3);
     // U2U, bottom-up: parent row = [8 children] x stacked U2U
     for (int l = max_depth_ - 1; l >= 0; l--) {
       const LevelRows& L = *m2m_[l];
-      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false, gemm_work_.p, kGemmWork);
+      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
     }
     mark(4);
     // X list: source points -> downward check surface (RAD0)
@@ -488,19 +523,19 @@ class DevicePlan final : public DevicePlanBase {
     K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p, X_.p);
     mark(6);
     K::hadamard_tiled(st_, n_vtiles_, M_dim_, e0_, e1_, n_off_, vt_t0_.p, vt_u0_.p, vt_usrc_.p, vt_uidx_.p, vt_gmask_.p, v_offsets_.p, V_.p,
-                      X_.p, Y_.p);
+                      X_.p, Y_.p, tuning.hadamard_tpb);
     mark(7);
     K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p, K1_, e1_, m_, lattice_.p, lattice_inv_.p, ns_, v_trg_scal_.p);
     mark(8);
     // D2D, top-down: 8 children rows += parent x [D2D_0 .. D2D_7]
     for (int l = 1; l <= max_depth_; l++) {
       const LevelRows& L = *l2l_[l - 1];
-      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true, gemm_work_.p, kGemmWork);
+      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
     }
     mark(9);
     // D2T: check -> equivalent
-    K::gemm_rows(st_, n_l2t_, rd_, K1_, dn_.p, l2t_ra_.p, K1_, dc2de0_.p, ctmp_.p, l2t_seq_.p, rd_, l2t_alpha_.p, false, gemm_work_.p, kGemmWork);
-    K::gemm_rows(st_, n_l2t_, K0_, rd_, ctmp_.p, l2t_seq_.p, rd_, dc2de1_.p, eq_.p, l2t_rc_.p, K0_, nullptr, false, gemm_work_.p, kGemmWork);
+    K::gemm_rows(st_, n_l2t_, rd_, K1_, dn_.p, l2t_ra_.p, K1_, dc2de0_.p, ctmp_.p, l2t_seq_.p, rd_, l2t_alpha_.p, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
+    K::gemm_rows(st_, n_l2t_, K0_, rd_, ctmp_.p, l2t_seq_.p, rd_, dc2de1_.p, eq_.p, l2t_rc_.p, K0_, nullptr, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
     mark(10);
     // equivalent surfaces -> targets (own downward equivalent + W list)
     K::far_points(st_, pk, n_tleaf_, t_leaf_.p, far_ptr_.p, far_node_.p, far_den_.p, far_rad_.p, trg_begin_.p, trg_end_.p,

@@ -38,9 +38,20 @@ struct PlanStats {
   int64_t v_tiles = 0, v_tile_partners = 0;
 };
 
+// Launch shapes an evaluation uses; the defaults suit large trees, and autotune() picks them
+// per tree by timing the candidates.
+struct Tuning {
+  int hadamard_tpb = 16;        // tiles per block run of the FP32 scalar Hadamard
+  int gemm_split_blocks = 512;  // GEMMs with fewer tile blocks than this split K
+};
+
 class DevicePlanBase {
  public:
   virtual ~DevicePlanBase() = default;
+  // Times each candidate launch shape on this tree (3 evaluations each, the fastest counts)
+  // and keeps the best; d_den / d_out as for evaluate (one vector), d_out is overwritten.
+  virtual Tuning autotune(const double* d_den, double* d_out) = 0;
+  Tuning tuning;
   // d_den: user layout (k0 per source, input order) for n_vec vectors with stride k0*n_src;
   // d_out: user layout (k1 per target, input order), overwritten.
   virtual void evaluate(const double* d_den, double* d_out, int n_vec, double den_scale, double out_scale) = 0;

@@ -1563,15 +1563,15 @@ void DeviceKernels<Real>::far_points(hipStream_t s, int kind, int n_rows, const 
 template <typename Real>
 void DeviceKernels<Real>::gemm_rows(hipStream_t s, int M, int N, int K, const Real* A, const int32_t* ra, int64_t lda,
                                     const Real* B, Real* C, const int32_t* rc, int64_t ldc, const Real* alpha, bool accumulate,
-                                    Real* work, int64_t work_elems) {
+                                    Real* work, int64_t work_elems, int split_blocks) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   const int gx = (N + 63) / 64, gy = (M + 63) / 64;
   // split K when the tile grid cannot fill the 256 CUs, in chunks of whole 16-wide k steps
   int ksplit = 1;
   const int blocks = gx * gy;
   if ((int64_t)blocks + 8 >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-  if (work && blocks < 512) {
-    ksplit = std::min((1024 + blocks - 1) / blocks, (K + 63) / 64);
+  if (work && blocks < split_blocks) {
+    ksplit = std::min((2 * split_blocks + blocks - 1) / blocks, (K + 63) / 64);
     while (ksplit > 1 && (int64_t)ksplit * M * N > work_elems) ksplit--;
   }
   const int kchunk = ((K + ksplit - 1) / ksplit + 31) / 32 * 32;
@@ -1654,7 +1654,7 @@ int DeviceKernels<Real>::hadamard_union_max(int e0) {
 template <typename Real>
 void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
                                          const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx,
-                                         const uint32_t* gmask, const int32_t* offsets, const Real* Vp, const Real* X, Real* Y) {
+                                         const uint32_t* gmask, const int32_t* offsets, const Real* Vp, const Real* X, Real* Y, int tiles_per_block) {
   if (n_tiles == 0) return;
   if (n_off != 26) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // the 26 V1 parent offsets
   constexpr int FB = sizeof(Real) == 4 ? 4 : 2;
@@ -1662,9 +1662,9 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
   const int64_t nb = (int64_t)n_tiles * ((n_fg + 7) / 8) * 8;
   if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   if (e0 == 1 && e1 == 1 && std::is_same<Real, float>::value) {
-    // runs of 16 tiles per block: the transfer blocks are read once per run, and the grid
-    // (n_fg * runs blocks) is many times the 2 resident blocks per CU
-    constexpr int TPB = 16;
+    // runs of tiles_per_block tiles per block: the transfer blocks are read once per run, and
+    // the grid (n_fg * runs blocks) stays many times the 2 resident blocks per CU
+    const int TPB = std::max(1, tiles_per_block);
     const int n_chunks = (n_tiles + TPB - 1) / TPB;
     constexpr int FBS = 2;
     const int n_fgs = (M_dim + FBS - 1) / FBS;

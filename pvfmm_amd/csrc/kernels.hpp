@@ -46,9 +46,11 @@ struct DeviceKernels {
 
   // C[rc[i]*ldc : +N] (+)= alpha[i] * A[ra[i]*lda : +K] * B[K x N] on MFMA; `work` (work_elems
   // Reals, may be null) enables a deterministic split-K for grids too small to fill the GPU.
+  // split_blocks: K is split (through work[], work_elems reals) when the tile grid has fewer
+  // blocks than this, so that about twice as many run
   static void gemm_rows(hipStream_t s, int M, int N, int K, const Real* A, const int32_t* ra, int64_t lda,
                         const Real* B, Real* C, const int32_t* rc, int64_t ldc, const Real* alpha,
-                        bool accumulate, Real* work = nullptr, int64_t work_elems = 0);
+                        bool accumulate, Real* work = nullptr, int64_t work_elems = 0, int split_blocks = 512);
 
   // V-list: forward DFT of the 8 children's equivalent densities of each source parent.
   // X layout [slot][f][8*e0] complex.  lattice[3p..3p+2] is the m^3 lattice cell of surface
@@ -62,12 +64,13 @@ struct DeviceKernels {
   // hadamard_union_max(e0)) are staged in LDS; uidx[k][n_off][16][4] holds, for offset o and
   // tile target 16 * g + r at [k][o][r][g], the partner's position in that list, or
   // hadamard_union_max(e0) (the zero entry) for none; bit o of gmask[k][g] is set when a
-  // target of group g of tile k has a partner at offset o.
+  // target of group g of tile k has a partner at offset o.  tiles_per_block: length of the
+  // run of tiles one block of the FP32 scalar kernel handles.
   static int hadamard_union_max(int e0);
   static void hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
                              const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx, const uint32_t* gmask,
                              const int32_t* offsets,
-                             const Real* Vp, const Real* X, Real* Y);
+                             const Real* Vp, const Real* X, Real* Y, int tiles_per_block = 16);
   // inverse DFT, pick the check-surface points, dn[child] += scal * value
   static void ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn,
                         int K1, int e1, int m, const int32_t* lattice, const int32_t* lat_inv, int ns,

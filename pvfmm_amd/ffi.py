@@ -61,6 +61,14 @@ def _declare(L):
         "pvfmm_amd_tree_evaluate": (i, [vp, c_double_p, c_double_p, i]),
         "pvfmm_amd_tree_evaluate_device": (i, [vp, vp, vp, i, i]),
         "pvfmm_amd_tree_stream": (vp, [vp]),
+        "pvfmm_amd_tree_autotune": (i, [vp, vp, vp, c_int32_p, i]),
+        "pvfmm_amd_tree_get_tuning": (i, [vp, c_int32_p, i]),
+        "pvfmm_amd_tree_set_tuning": (i, [vp, c_int32_p, i]),
+        "pvfmm_amd_batch_create": (vp, [ctypes.POINTER(vp), i]),
+        "pvfmm_amd_batch_evaluate_device": (i, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i]),
+        "pvfmm_amd_batch_stream": (vp, [vp]),
+        "pvfmm_amd_batch_records": (i, [vp]),
+        "pvfmm_amd_batch_destroy": (None, [vp]),
         "pvfmm_amd_tree_num_nodes": (i64, [vp]),
         "pvfmm_amd_tree_nodes": (i, [vp, c_int64_p]),
         "pvfmm_amd_tree_point_order": (i, [vp, i, c_int64_p]),

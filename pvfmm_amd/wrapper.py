@@ -275,6 +275,42 @@ class PtFMMTree:
     def stream_handle(self) -> int:
         return ffi.lib().pvfmm_amd_tree_stream(self._h) or 0
 
+    # -- launch-shape tuning
+    TUNING_KEYS = ("hadamard_tiles_per_block", "gemm_split_blocks")
+
+    def autotune(self, den=None) -> dict:
+        """Times the launch-shape candidates on this tree with one density vector on the GPU
+        (seeded N(0,1) when none is given), keeps the fastest and returns it."""
+        import torch
+
+        k0, k1 = self.dims()
+        if den is None:
+            g = torch.Generator(device="cuda:0").manual_seed(0)
+            den = torch.randn(k0 * self.n_src, dtype=torch.float64, device="cuda:0", generator=g)
+        if not (den.is_cuda and den.dtype == torch.float64 and den.is_contiguous() and den.numel() == k0 * self.n_src):
+            raise ValueError("den must be one contiguous float64 GPU density vector")
+        out = torch.empty(k1 * self.n_trg, dtype=torch.float64, device=den.device)
+        torch.cuda.current_stream(den.device).synchronize()
+        p = (ctypes.c_int32 * 2)()
+        ffi.check(ffi.lib().pvfmm_amd_tree_autotune(self._h, ctypes.c_void_p(den.data_ptr()),
+                                                    ctypes.c_void_p(out.data_ptr()), p, 2))
+        return dict(zip(self.TUNING_KEYS, list(p)))
+
+    def tuning(self) -> dict:
+        p = (ctypes.c_int32 * 2)()
+        ffi.check(ffi.lib().pvfmm_amd_tree_get_tuning(self._h, p, 2))
+        return dict(zip(self.TUNING_KEYS, list(p)))
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        for k, v in kw.items():
+            if k not in cur:
+                raise ValueError(f"unknown tuning key {k}")
+            cur[k] = int(v)
+        p = (ctypes.c_int32 * 2)(*[cur[k] for k in self.TUNING_KEYS])
+        ffi.check(ffi.lib().pvfmm_amd_tree_set_tuning(self._h, p, 2))
+        return self
+
     def set_timing(self, on: bool = True):
         ffi.check(ffi.lib().pvfmm_amd_tree_set_timing(self._h, int(on)))
 
@@ -290,6 +326,57 @@ class PtFMMTree:
                  "x_items", "v_src_slots", "v_trg_slots", "v_pairs", "u_rows", "u_items", "u_pairs", "far_items",
                  "far_pairs", "s2m_pairs", "u_src_points_streamed", "device_bytes", "v_tiles", "v_tile_partners"]
         return dict(zip(names, out.tolist()))
+
+
+class PtFMMBatch:
+    """Set-up trees of independent point clouds evaluated together: the first evaluation
+    records every tree's evaluation into one HIP graph (one branch per tree), later calls with
+    the same tensors replay it with one graph launch.  The trees must stay alive and set up."""
+
+    def __init__(self, trees):
+        self.trees = list(trees)
+        if not self.trees:
+            raise ValueError("a batch needs at least one tree")
+        arr = (ctypes.c_void_p * len(self.trees))(*[t._h for t in self.trees])
+        self._h = ffi.lib().pvfmm_amd_batch_create(arr, len(self.trees))
+        if not self._h:
+            ffi.check()
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                ffi.lib().pvfmm_amd_batch_destroy(self._h)
+                self._h = None
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def evaluate_device(self, dens, outs=None, sync: bool = True):
+        """dens: one contiguous float64 GPU tensor per tree (k0*n_src); returns the outputs."""
+        import torch
+
+        if len(dens) != len(self.trees):
+            raise ValueError("one density tensor per tree")
+        if outs is None:
+            outs = [torch.empty(t.dims()[1] * t.n_trg, dtype=torch.float64, device=d.device)
+                    for t, d in zip(self.trees, dens)]
+        for t, d, o in zip(self.trees, dens, outs):
+            k0, k1 = t.dims()
+            if not (d.is_cuda and d.dtype == torch.float64 and d.is_contiguous() and d.numel() == k0 * t.n_src):
+                raise ValueError("each density must be a contiguous float64 GPU tensor of k0*n_src values")
+            if not (o.is_cuda and o.dtype == torch.float64 and o.is_contiguous() and o.numel() == k1 * t.n_trg):
+                raise ValueError("each output must be a contiguous float64 GPU tensor of k1*n_trg values")
+        n = len(self.trees)
+        dp = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dens])
+        op = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+        torch.cuda.current_stream(dens[0].device).synchronize()
+        ffi.check(ffi.lib().pvfmm_amd_batch_evaluate_device(self._h, dp, op, 0 if sync else 1))
+        return outs
+
+    def stream_handle(self) -> int:
+        return ffi.lib().pvfmm_amd_batch_stream(self._h) or 0
+
+    def records(self) -> int:
+        return ffi.lib().pvfmm_amd_batch_records(self._h)
 
 
 def device_count() -> int:

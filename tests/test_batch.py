@@ -1,0 +1,77 @@
+"""GPU: independent clouds evaluated as one recorded HIP graph (PtFMMBatch), and the per-tree
+launch-shape autotuner.  The graph replays the same kernels on the same data as the eager
+evaluations, so replays must equal them bit for bit; the graph is replayed several times with
+eager work, in-place density changes and synchronizations between the replays, and every
+replay is checked.  Retuned launch shapes change only the work split (the Hadamard run length
+keeps every summation order; a different K split of a GEMM reorders its sums), so results
+after tuning match to rounding."""
+import numpy as np
+import pytest
+
+import pvfmm_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def clouds(sizes, seed):
+    rng = np.random.default_rng(seed)
+    return [(rng.random((n, 3)), rng.standard_normal(n)) for n in sizes]
+
+
+def test_batch_graph_replays_match_eager():
+    import torch
+
+    fmm = P.PtFMM("laplace_poten", 6, "mixed")
+    cs = clouds([3000, 9000, 20000], 7)
+    trees = [P.PtFMMTree(x, x, 64).setup(fmm) for x, _ in cs]
+    dens = [torch.tensor(q, device="cuda:0") for _, q in cs]
+    outs = [torch.empty_like(d) for d in dens]
+    batch = P.PtFMMBatch(trees)
+    for rep in range(4):
+        print(f"replay {rep}", flush=True)
+        batch.evaluate_device(dens, outs)
+        ref = [t.evaluate_device(d.clone()) for t, d in zip(trees, dens)]
+        for i, (o, r) in enumerate(zip(outs, ref)):
+            assert torch.equal(o, r), f"replay {rep} cloud {i}: max diff {float((o - r).abs().max())}"
+        # eager work on a tree of the batch, then new density values in the same buffers
+        trees[0].evaluate_device(torch.ones_like(dens[0]))
+        torch.cuda.synchronize()
+        for d in dens:
+            d.mul_(-0.5).add_(0.25)
+    assert batch.records() == 1
+    # other output buffers: the graph is recorded again
+    outs2 = [torch.empty_like(d) for d in dens]
+    batch.evaluate_device(dens, outs2)
+    assert batch.records() == 2
+    for t, d, o in zip(trees, dens, outs2):
+        assert torch.equal(o, t.evaluate_device(d.clone()))
+    # accuracy of a batched cloud against the direct sum
+    x, _ = cs[1]
+    ud = P.direct("laplace_poten", x, dens[1].cpu().numpy(), x)
+    err = float(np.linalg.norm(outs2[1].cpu().numpy() - ud) / np.linalg.norm(ud))
+    print(f"cloud 1 rel-L2 vs direct {err:.2e}")
+    assert err < 1e-4
+
+
+@pytest.mark.parametrize("prec,tol", [("mixed", 1e-6), ("fp64", 1e-12)])
+def test_autotune_keeps_results(prec, tol):
+    import torch
+
+    rng = np.random.default_rng(3)
+    x = rng.random((40000, 3))
+    q = torch.tensor(rng.standard_normal(len(x)), device="cuda:0")
+    t = P.PtFMMTree(x, x, 64).setup(P.PtFMM("laplace_grad", 6, prec))
+    assert t.tuning() == {"hadamard_tiles_per_block": 16, "gemm_split_blocks": 512}
+    before = t.evaluate_device(q)
+    chosen = t.autotune(q)
+    print(f"{prec}: autotuned {chosen}")
+    assert chosen["hadamard_tiles_per_block"] in (4, 8, 16, 32, 64)
+    assert chosen["gemm_split_blocks"] in (0, 256, 512, 1024, 2048)
+    assert t.tuning() == chosen
+    after = t.evaluate_device(q)
+    r = float(torch.linalg.norm(after - before) / torch.linalg.norm(before))
+    assert r < tol, r
+    # explicit extreme shapes: one tile per block run, no K split
+    t.set_tuning(hadamard_tiles_per_block=1, gemm_split_blocks=0)
+    r = float(torch.linalg.norm(t.evaluate_device(q) - before) / torch.linalg.norm(before))
+    assert r < tol, r

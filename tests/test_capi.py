@@ -46,6 +46,15 @@ def test_error_codes_without_abort():
     t = P.PtFMMTree(x, x, 4)
     out = np.zeros(10)
     assert L.pvfmm_amd_tree_evaluate(t._h, ffi.dptr(np.ones(10)), ffi.dptr(out), 1) == 5
+    # tuning and batching need a set-up tree: state errors, no HIP work
+    p2 = (ctypes.c_int32 * 2)(16, 512)
+    assert L.pvfmm_amd_tree_set_tuning(t._h, p2, 2) == 5
+    assert L.pvfmm_amd_tree_get_tuning(t._h, p2, 2) == 5
+    arr = (ctypes.c_void_p * 1)(t._h)
+    assert not L.pvfmm_amd_batch_create(arr, 1)
+    assert L.PVFMMGetLastError() == 5
+    assert not L.pvfmm_amd_batch_create(arr, 0)
+    assert L.PVFMMGetLastError() == 1
     # destroy on NULL is a no-op and sets the handle to NULL
     p = ctypes.c_void_p(None)
     L.PVFMMDestroyContextD(ctypes.byref(p))
