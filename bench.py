@@ -301,6 +301,7 @@ def main():
                     help="evaluators to time (profiling runs); the JSON line needs both")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
     ap.add_argument("--no-bwd", action="store_true", help="skip the energy forward/backward (config 5) timing")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--config", default="2", help="2 (headline, default), or a secondary workload: 1, 3, 4a, 4b, 5")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -342,14 +343,17 @@ def main():
         t0 = time.perf_counter()
         tree.setup(fmm)
         t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        shapes = tree.tuning() if args.no_tune else tree.autotune(dq)
+        t_tune = time.perf_counter() - t0
         step, out, phases = time_steps(tree, dq, args.steps, args.warmup, timing=True)
         # un-instrumented timing of the same steps
         step_plain, out, _ = time_steps(tree, dq, args.steps, 0, timing=False)
         u = out.cpu().numpy().reshape(n, 3)[idx]
         res[prec] = dict(step=step_plain, step_timed=step, phases=phases, err=None if ud is None else rel_l2(u, ud), t_ops=t_ops,
-                         t_setup=t_setup, stats=tree.stats())
+                         t_setup=t_setup, t_tune=t_tune, shapes=shapes, stats=tree.stats())
         log(f"{prec}: {1e3 * step_plain:.2f} ms/eval ({n / step_plain / 1e6:.1f} Mtargets/s), rel-L2 {res[prec]['err'] if res[prec]['err'] is None else format(res[prec]['err'], '.3e')}, "
-            f"operators {t_ops:.1f}s, setup {t_setup:.1f}s")
+            f"operators {t_ops:.1f}s, setup {t_setup:.1f}s, launch shapes {shapes} ({'default' if args.no_tune else f'autotuned in {t_tune:.1f}s'})")
         log("   phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
 
     if args.prec != "both":
@@ -396,7 +400,9 @@ def main():
         "rel_l2_fp64": fp64["err"],
         "bwd_fwd_ratio": None if bf is None else bf["ratio"],
         "bwd_fwd": bf,
-        "setup_s": {"tree": t_tree, "operators_mixed": mixed["t_ops"], "plan_mixed": mixed["t_setup"]},
+        "setup_s": {"tree": t_tree, "operators_mixed": mixed["t_ops"], "plan_mixed": mixed["t_setup"],
+                    "autotune_mixed": mixed["t_tune"]},
+        "launch_shapes": {"mixed": mixed["shapes"], "fp64": fp64["shapes"], "autotuned": not args.no_tune},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
