@@ -75,6 +75,18 @@ void PVFMMEvalF(const float* src_pos, const float* sl_den, const float* dl_den, 
 void PVFMMDestroyContextD(void** ctx);
 void PVFMMDestroyContextF(void** ctx);
 
+/* Fortran-callable particle context (src/pvfmm-wrapper.cpp:831-876): every argument by
+ * reference, the communicator as a Fortran integer handle (ignored: one GPU).  Errors are
+ * recorded as for the C calls; a failed create stores NULL in *ctx. */
+void pvfmmcreatecontextd_(void** ctx, double* box_size, int32_t* n, int32_t* m, int32_t* kernel, int32_t* fcomm);
+void pvfmmcreatecontextf_(void** ctx, float* box_size, int32_t* n, int32_t* m, int32_t* kernel, int32_t* fcomm);
+void pvfmmevald_(const double* src_pos, const double* sl_den, int64_t* n_src, const double* trg_pos, double* trg_val,
+                 int64_t* n_trg, void** ctx, int32_t* setup);
+void pvfmmevalf_(const float* src_pos, const float* sl_den, int64_t* n_src, const float* trg_pos, float* trg_val,
+                 int64_t* n_trg, void** ctx, int32_t* setup);
+void pvfmmdestroycontextd_(void** ctx);
+void pvfmmdestroycontextf_(void** ctx);
+
 /* Error reporting; no counterpart in the reference (it aborts).  The state is per thread. */
 int PVFMMGetLastError(void);
 const char* PVFMMErrorString(int code);

@@ -355,6 +355,40 @@ void PVFMMDestroyContextD(void** ctx) {
 }
 void PVFMMDestroyContextF(void** ctx) { PVFMMDestroyContextD(ctx); }
 
+// Fortran symbols: arguments by reference (src/pvfmm-wrapper.cpp:831-876)
+void pvfmmcreatecontextd_(void** ctx, double* box_size, int32_t* n, int32_t* m, int32_t* kernel, int32_t*) {
+  if (!ctx) return;
+  *ctx = nullptr;
+  guarded([&] {
+    if (!box_size || !n || !m || !kernel) throw Error(kErrInvalidArgument, "null argument");
+    *ctx = create_context<double>(*box_size, *n, *m, *kernel, false);
+  });
+}
+void pvfmmcreatecontextf_(void** ctx, float* box_size, int32_t* n, int32_t* m, int32_t* kernel, int32_t*) {
+  if (!ctx) return;
+  *ctx = nullptr;
+  guarded([&] {
+    if (!box_size || !n || !m || !kernel) throw Error(kErrInvalidArgument, "null argument");
+    *ctx = create_context<float>(*box_size, *n, *m, *kernel, true);
+  });
+}
+void pvfmmevald_(const double* src_pos, const double* sl_den, int64_t* n_src, const double* trg_pos, double* trg_val,
+                 int64_t* n_trg, void** ctx, int32_t* setup) {
+  guarded([&] {
+    if (!n_src || !n_trg || !ctx || !setup) throw Error(kErrInvalidArgument, "null argument");
+    context_eval<double>((Context*)*ctx, src_pos, sl_den, nullptr, (long)*n_src, trg_pos, trg_val, (long)*n_trg, *setup);
+  });
+}
+void pvfmmevalf_(const float* src_pos, const float* sl_den, int64_t* n_src, const float* trg_pos, float* trg_val,
+                 int64_t* n_trg, void** ctx, int32_t* setup) {
+  guarded([&] {
+    if (!n_src || !n_trg || !ctx || !setup) throw Error(kErrInvalidArgument, "null argument");
+    context_eval<float>((Context*)*ctx, src_pos, sl_den, nullptr, (long)*n_src, trg_pos, trg_val, (long)*n_trg, *setup);
+  });
+}
+void pvfmmdestroycontextd_(void** ctx) { PVFMMDestroyContextD(ctx); }
+void pvfmmdestroycontextf_(void** ctx) { PVFMMDestroyContextD(ctx); }
+
 int PVFMMGetLastError(void) { return g_err; }
 const char* PVFMMGetLastErrorMessage(void) { return g_msg.c_str(); }
 const char* PVFMMErrorString(int code) {

@@ -48,6 +48,14 @@ def _declare(L):
         "PVFMMEvalF": (None, [c_float_p, c_float_p, c_float_p, l, c_float_p, c_float_p, l, vp, i]),
         "PVFMMDestroyContextD": (None, [ctypes.POINTER(vp)]),
         "PVFMMDestroyContextF": (None, [ctypes.POINTER(vp)]),
+        "pvfmmcreatecontextd_": (None, [ctypes.POINTER(vp), c_double_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),
+        "pvfmmcreatecontextf_": (None, [ctypes.POINTER(vp), c_float_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),
+        "pvfmmevald_": (None, [c_double_p, c_double_p, c_int64_p, c_double_p, c_double_p, c_int64_p, ctypes.POINTER(vp),
+                               c_int32_p]),
+        "pvfmmevalf_": (None, [c_float_p, c_float_p, c_int64_p, c_float_p, c_float_p, c_int64_p, ctypes.POINTER(vp),
+                               c_int32_p]),
+        "pvfmmdestroycontextd_": (None, [ctypes.POINTER(vp)]),
+        "pvfmmdestroycontextf_": (None, [ctypes.POINTER(vp)]),
         "PVFMMGetLastError": (i, []),
         "PVFMMErrorString": (ctypes.c_char_p, [i]),
         "PVFMMGetLastErrorMessage": (ctypes.c_char_p, []),

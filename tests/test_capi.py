@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "pvfmm.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:PVFMM|pvfmm_amd_)\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:PVFMM|pvfmm_amd_|pvfmm)\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -59,6 +59,27 @@ def test_error_codes_without_abort():
     p = ctypes.c_void_p(None)
     L.PVFMMDestroyContextD(ctypes.byref(p))
     assert p.value is None
+
+
+def test_fortran_context_create_and_errors():
+    """Fortran-callable create/destroy (host work only: operators, no evaluation)."""
+    L = ffi.lib()
+    i32 = ctypes.c_int32
+    ctx = ctypes.c_void_p()
+    box = ctypes.c_double(-1.0)
+    L.pvfmmcreatecontextd_(ctypes.byref(ctx), ctypes.byref(box), ctypes.byref(i32(100)), ctypes.byref(i32(4)),
+                           ctypes.byref(i32(0)), ctypes.byref(i32(0)))
+    assert L.PVFMMGetLastError() == 0 and ctx.value
+    L.pvfmmdestroycontextd_(ctypes.byref(ctx))
+    assert ctx.value is None
+    # StokesPressure is unsupported: NULL context and the error code, no abort
+    L.pvfmmcreatecontextd_(ctypes.byref(ctx), ctypes.byref(box), ctypes.byref(i32(100)), ctypes.byref(i32(4)),
+                           ctypes.byref(i32(2)), ctypes.byref(i32(0)))
+    assert ctx.value is None and L.PVFMMGetLastError() == 2
+    # evaluation on a NULL context is an argument error
+    z = ctypes.c_int64(1)
+    L.pvfmmevald_(None, None, ctypes.byref(z), None, None, ctypes.byref(z), ctypes.byref(ctx), ctypes.byref(i32(1)))
+    assert L.PVFMMGetLastError() == 1
 
 
 def test_list_counts_match_reference_tables():

@@ -171,6 +171,47 @@ def test_c_abi_context_roundtrip(gpu):
         assert p.value is None
 
 
+def test_fortran_symbols_match_c_context(gpu):
+    """pvfmmcreatecontextd_ / pvfmmevald_ / pvfmmdestroycontextd_ (arguments by reference,
+    src/pvfmm-wrapper.cpp:831-876) give the C context's values bit for bit; the F variants too."""
+    L = ffi.lib()
+    rng = np.random.default_rng(23)
+    n = 4000
+    src = rng.uniform(-1, 2, (n, 3))
+    q = rng.standard_normal(n)
+    for real, ev, evc, cr, crc, dsf in ((np.float64, L.pvfmmevald_, L.PVFMMEvalD, L.pvfmmcreatecontextd_,
+                                        L.PVFMMCreateContextD, L.pvfmmdestroycontextd_),
+                                       (np.float32, L.pvfmmevalf_, L.PVFMMEvalF, L.pvfmmcreatecontextf_,
+                                        L.PVFMMCreateContextF, L.pvfmmdestroycontextf_)):
+        cptr = ctypes.POINTER(ctypes.c_double if real == np.float64 else ctypes.c_float)
+        xs = np.ascontiguousarray(src.ravel(), dtype=real)
+        qs = np.ascontiguousarray(q, dtype=real)
+        out_f = np.zeros(3 * n, dtype=real)
+        out_c = np.zeros(3 * n, dtype=real)
+        fctx = ctypes.c_void_p()
+        box = (ctypes.c_double if real == np.float64 else ctypes.c_float)(-1.0)
+        i32 = ctypes.c_int32
+        cr(ctypes.byref(fctx), ctypes.byref(box), ctypes.byref(i32(64)), ctypes.byref(i32(6)), ctypes.byref(i32(1)),
+           ctypes.byref(i32(0)))
+        ffi.check()
+        assert fctx.value
+        ns, nt, setup = ctypes.c_int64(n), ctypes.c_int64(n), i32(1)
+        ev(xs.ctypes.data_as(cptr), qs.ctypes.data_as(cptr), ctypes.byref(ns), xs.ctypes.data_as(cptr),
+           out_f.ctypes.data_as(cptr), ctypes.byref(nt), ctypes.byref(fctx), ctypes.byref(setup))
+        ffi.check()
+        cctx = crc(-1.0, 64, 6, 1, 0)
+        ffi.check()
+        evc(xs.ctypes.data_as(cptr), qs.ctypes.data_as(cptr), None, n, xs.ctypes.data_as(cptr), out_c.ctypes.data_as(cptr),
+            n, cctx, 1)
+        ffi.check()
+        assert np.array_equal(out_f, out_c)
+        assert rel(out_f, O.direct(1, src, q, src)) < (1e-3 if real == np.float32 else 1e-4)
+        dsf(ctypes.byref(fctx))
+        assert fctx.value is None
+        p = ctypes.c_void_p(cctx)
+        L.PVFMMDestroyContextD(ctypes.byref(p))
+
+
 def test_python_context_float_and_mixed(gpu):
     rng = np.random.default_rng(18)
     n = 4000
