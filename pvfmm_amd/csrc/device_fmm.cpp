@@ -121,7 +121,7 @@ class DevicePlan final : public DevicePlanBase {
   // operators
   DBuf<Real> uc2ue0_, uc2ue1_, dc2de0_, dc2de1_, u2u_, d2d_, V_, unit_surf_, ctmp_;
   DBuf<Real> gemm_work_;  // split-K partial tiles of the small coarse-level GEMMs
-  static constexpr int64_t kGemmWork = int64_t(1) << 22;
+  int64_t gemm_work_elems_ = 0;  // room for 4 K-split partials of the largest GEMM output, at most 2^25
   int ru_ = 0, rd_ = 0;
   DBuf<int32_t> lattice_, lattice_inv_;
   // S2M
@@ -302,7 +302,6 @@ class DevicePlan final : public DevicePlanBase {
     l2t_seq_.upload(l2t_seq, &total);
     ctmp_.alloc((int64_t)std::max(1, std::max(n_s2m_, n_l2t_)) * std::max(1, std::max(ru_, rd_)), &total);
     t_leaf_.upload(tleaf, &total);
-    gemm_work_.alloc(kGemmWork, &total);
     eq_.alloc((int64_t)(N + n_l2t_) * K0_, &total);
     dn_.alloc((int64_t)N * K1_, &total);
 
@@ -321,6 +320,13 @@ class DevicePlan final : public DevicePlanBase {
       b->n = (int)ra2.size(); b->ra.upload(ra2, &total); b->rc.upload(rc2, &total);
       m2m_rows += a->n; l2l_rows += b->n;
       m2m_.push_back(std::move(a)); l2l_.push_back(std::move(b));
+    }
+    {
+      int64_t out = std::max((int64_t)n_s2m_ * std::max(ru_, K0_), (int64_t)n_l2t_ * std::max(rd_, K0_));
+      for (auto& L : m2m_) out = std::max(out, (int64_t)L->n * K0_);
+      for (auto& L : l2l_) out = std::max(out, (int64_t)L->n * 8 * K1_);
+      gemm_work_elems_ = std::min<int64_t>(int64_t(1) << 25, std::max<int64_t>(1, 4 * out));
+      gemm_work_.alloc(gemm_work_elems_, &total);
     }
 
     const int nl_x = T.list_count(kX), nl_w = T.list_count(kW);
@@ -506,13 +512,13 @@ class DevicePlan final : public DevicePlanBase {
     K::p2p_surface(st_, ek, n_s2m_, s2m_node_.p, s2m_ptr_.p, s2m_items_.p, src_begin_.p, src_end_.p, center_.p, node_size_.p,
                    kRad1, unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, chk_.p, s2m_orow_.p, K1_, false);
     mark(2);
-    K::gemm_rows(st_, n_s2m_, ru_, K1_, chk_.p, s2m_orow_.p, K1_, uc2ue0_.p, ctmp_.p, s2m_orow_.p, ru_, s2m_alpha_.p, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
-    K::gemm_rows(st_, n_s2m_, K0_, ru_, ctmp_.p, s2m_orow_.p, ru_, uc2ue1_.p, eq_.p, s2m_node_.p, K0_, nullptr, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
+    K::gemm_rows(st_, n_s2m_, ru_, K1_, chk_.p, s2m_orow_.p, K1_, uc2ue0_.p, ctmp_.p, s2m_orow_.p, ru_, s2m_alpha_.p, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
+    K::gemm_rows(st_, n_s2m_, K0_, ru_, ctmp_.p, s2m_orow_.p, ru_, uc2ue1_.p, eq_.p, s2m_node_.p, K0_, nullptr, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
     mark(3);
     // U2U, bottom-up: parent row = [8 children] x stacked U2U
     for (int l = max_depth_ - 1; l >= 0; l--) {
       const LevelRows& L = *m2m_[l];
-      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
+      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
     }
     mark(4);
     // X list: source points -> downward check surface (RAD0)
@@ -530,12 +536,12 @@ class DevicePlan final : public DevicePlanBase {
     // D2D, top-down: 8 children rows += parent x [D2D_0 .. D2D_7]
     for (int l = 1; l <= max_depth_; l++) {
       const LevelRows& L = *l2l_[l - 1];
-      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
+      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
     }
     mark(9);
     // D2T: check -> equivalent
-    K::gemm_rows(st_, n_l2t_, rd_, K1_, dn_.p, l2t_ra_.p, K1_, dc2de0_.p, ctmp_.p, l2t_seq_.p, rd_, l2t_alpha_.p, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
-    K::gemm_rows(st_, n_l2t_, K0_, rd_, ctmp_.p, l2t_seq_.p, rd_, dc2de1_.p, eq_.p, l2t_rc_.p, K0_, nullptr, false, gemm_work_.p, kGemmWork, tuning.gemm_split_blocks);
+    K::gemm_rows(st_, n_l2t_, rd_, K1_, dn_.p, l2t_ra_.p, K1_, dc2de0_.p, ctmp_.p, l2t_seq_.p, rd_, l2t_alpha_.p, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
+    K::gemm_rows(st_, n_l2t_, K0_, rd_, ctmp_.p, l2t_seq_.p, rd_, dc2de1_.p, eq_.p, l2t_rc_.p, K0_, nullptr, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
     mark(10);
     // equivalent surfaces -> targets (own downward equivalent + W list)
     K::far_points(st_, pk, n_tleaf_, t_leaf_.p, far_ptr_.p, far_node_.p, far_den_.p, far_rad_.p, trg_begin_.p, trg_end_.p,

@@ -42,7 +42,7 @@ struct PlanStats {
 // per tree by timing the candidates.
 struct Tuning {
   int hadamard_tpb = 16;        // tiles per block run of the FP32 scalar Hadamard
-  int gemm_split_blocks = 512;  // GEMMs with fewer tile blocks than this split K
+  int gemm_split_blocks = 1024;  // GEMMs with fewer tile blocks than this split K
 };
 
 class DevicePlanBase {

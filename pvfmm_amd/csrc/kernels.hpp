@@ -50,7 +50,7 @@ struct DeviceKernels {
   // blocks than this, so that about twice as many run
   static void gemm_rows(hipStream_t s, int M, int N, int K, const Real* A, const int32_t* ra, int64_t lda,
                         const Real* B, Real* C, const int32_t* rc, int64_t ldc, const Real* alpha,
-                        bool accumulate, Real* work = nullptr, int64_t work_elems = 0, int split_blocks = 512);
+                        bool accumulate, Real* work = nullptr, int64_t work_elems = 0, int split_blocks = 1024);
 
   // V-list: forward DFT of the 8 children's equivalent densities of each source parent.
   // X layout [slot][f][8*e0] complex.  lattice[3p..3p+2] is the m^3 lattice cell of surface

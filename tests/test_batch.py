@@ -61,7 +61,7 @@ def test_autotune_keeps_results(prec, tol):
     x = rng.random((40000, 3))
     q = torch.tensor(rng.standard_normal(len(x)), device="cuda:0")
     t = P.PtFMMTree(x, x, 64).setup(P.PtFMM("laplace_grad", 6, prec))
-    assert t.tuning() == {"hadamard_tiles_per_block": 16, "gemm_split_blocks": 512}
+    assert t.tuning() == {"hadamard_tiles_per_block": 16, "gemm_split_blocks": 1024}
     before = t.evaluate_device(q)
     chosen = t.autotune(q)
     print(f"{prec}: autotuned {chosen}")
