@@ -61,8 +61,11 @@ class DevicePlan final : public DevicePlanBase {
 
   void evaluate(const double* d_den, double* d_out, int n_vec, double den_scale, double out_scale) override {
     if (n_vec < 1) throw Error(kErrInvalidArgument, "n_vec must be >= 1");
-    for (int v = 0; v < n_vec; v++)
-      evaluate_one(d_den + (int64_t)v * k0_ * n_src_, d_out + (int64_t)v * k1_ * n_trg_, den_scale, out_scale);
+    // passes of up to n_vec_max vectors: they share the pair geometry of every P2P and run as
+    // the rows of one GEMM per up/down step
+    for (int v = 0; v < n_vec; v += nvb_)
+      evaluate_one(d_den + (int64_t)v * k0_ * n_src_, d_out + (int64_t)v * k1_ * n_trg_, den_scale, out_scale,
+                   std::min(nvb_, n_vec - v));
   }
 
   Tuning autotune(const double* d_den, double* d_out) override {
@@ -72,7 +75,7 @@ class DevicePlan final : public DevicePlanBase {
     auto measure = [&]() {
       std::vector<double> best(kPhaseCount, 1e300);
       for (int r = 0; r < 3; r++) {
-        evaluate_one(d_den, d_out, 1.0, 1.0);
+        evaluate_one(d_den, d_out, 1.0, 1.0, 1);
         for (int p = 0; p < kPhaseCount; p++) best[p] = std::min(best[p], phase_ms[p]);
       }
       return best;
@@ -121,7 +124,9 @@ class DevicePlan final : public DevicePlanBase {
   // operators
   DBuf<Real> uc2ue0_, uc2ue1_, dc2de0_, dc2de1_, u2u_, d2d_, V_, unit_surf_, ctmp_;
   DBuf<Real> gemm_work_;  // split-K partial tiles of the small coarse-level GEMMs
-  int64_t gemm_work_elems_ = 0;  // room for 4 K-split partials of the largest GEMM output, at most 2^25
+  int64_t gemm_work_elems_ = 0;
+  int nvb_ = 1;  // density vectors per evaluation pass (buffers hold this many)
+  int64_t eq_vs_ = 0, dn_vs_ = 0, chk_vs_ = 0, ctmp_vs_ = 0;  // per-vector strides  // room for 4 K-split partials of the largest GEMM output, at most 2^25
   int ru_ = 0, rd_ = 0;
   DBuf<int32_t> lattice_, lattice_inv_;
   // S2M
@@ -153,15 +158,24 @@ class DevicePlan final : public DevicePlanBase {
   DBuf<double> far_rad_;
   int n_tleaf_ = 0;
   std::vector<hipEvent_t> ev_;
+  std::vector<hipEvent_t> vev_;  // per-vector V-list sub-phase events (timing of multi-vector passes)
 
  public:
   ~DevicePlan() override {
     for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : vev_) (void)hipEventDestroy(e);
   }
 
  private:
   void mark(int phase) {
     if (timing_) HIP_CHECK(hipEventRecord(ev_[phase], st_));
+  }
+  void ensure_vlist_events(int nv) {
+    while ((int)vev_.size() < 4 * nv) {
+      hipEvent_t e = nullptr;
+      HIP_CHECK(hipEventCreate(&e));
+      vev_.push_back(e);
+    }
   }
 
   void build(const Octree& T, const double* src_xyz, const double* trg_xyz) {
@@ -208,8 +222,10 @@ class DevicePlan final : public DevicePlanBase {
     tx64_.upload(tx, &total);
     sx_.upload(std::vector<Real>(sx.begin(), sx.end()), &total);
     tx_.upload(std::vector<Real>(tx.begin(), tx.end()), &total);
-    sden_.alloc((int64_t)k0_ * n_src_, &total);
-    tval_.alloc((int64_t)k1_ * n_trg_, &total);
+    // per-vector buffers hold n_vec_max density vectors, one after another
+    nvb_ = n_vec_max;
+    sden_.alloc((int64_t)k0_ * n_src_ * nvb_, &total);
+    tval_.alloc((int64_t)k1_ * n_trg_ * nvb_, &total);
 
     // ---- operators in the working precision
     auto conv = [](const std::vector<double>& v) { return std::vector<Real>(v.begin(), v.end()); };
@@ -285,7 +301,8 @@ class DevicePlan final : public DevicePlanBase {
     n_s2m_ = (int)s2m_node.size();
     s2m_node_.upload(s2m_node, &total); s2m_ptr_.upload(s2m_ptr, &total); s2m_items_.upload(s2m_items, &total);
     s2m_orow_.upload(s2m_orow, &total); s2m_alpha_.upload(s2m_alpha, &total);
-    chk_.alloc((int64_t)std::max(1, n_s2m_) * K1_, &total);
+    chk_vs_ = (int64_t)std::max(1, n_s2m_) * K1_;
+    chk_.alloc(chk_vs_ * nvb_, &total);
 
     // ---- target leaves and L2T rows (Down2TargetSetup, include/fmm_pts.txx:5427-5442)
     std::vector<int32_t> tleaf, l2t_seq;
@@ -300,10 +317,13 @@ class DevicePlan final : public DevicePlanBase {
     n_l2t_ = n_tleaf_ = (int)tleaf.size();
     l2t_ra_.upload(l2t_ra, &total); l2t_rc_.upload(l2t_rc, &total); l2t_alpha_.upload(l2t_alpha, &total);
     l2t_seq_.upload(l2t_seq, &total);
-    ctmp_.alloc((int64_t)std::max(1, std::max(n_s2m_, n_l2t_)) * std::max(1, std::max(ru_, rd_)), &total);
+    ctmp_vs_ = (int64_t)std::max(1, std::max(n_s2m_, n_l2t_)) * std::max(1, std::max(ru_, rd_));
+    ctmp_.alloc(ctmp_vs_ * nvb_, &total);
     t_leaf_.upload(tleaf, &total);
-    eq_.alloc((int64_t)(N + n_l2t_) * K0_, &total);
-    dn_.alloc((int64_t)N * K1_, &total);
+    eq_vs_ = (int64_t)(N + n_l2t_) * K0_;
+    dn_vs_ = (int64_t)N * K1_;
+    eq_.alloc(eq_vs_ * nvb_, &total);
+    dn_.alloc(dn_vs_ * nvb_, &total);
 
     // ---- M2M / L2L rows per level
     m2m_.clear(); l2l_.clear();
@@ -325,7 +345,7 @@ class DevicePlan final : public DevicePlanBase {
       int64_t out = std::max((int64_t)n_s2m_ * std::max(ru_, K0_), (int64_t)n_l2t_ * std::max(rd_, K0_));
       for (auto& L : m2m_) out = std::max(out, (int64_t)L->n * K0_);
       for (auto& L : l2l_) out = std::max(out, (int64_t)L->n * 8 * K1_);
-      gemm_work_elems_ = std::min<int64_t>(int64_t(1) << 25, std::max<int64_t>(1, 4 * out));
+      gemm_work_elems_ = std::min<int64_t>(int64_t(1) << 25, std::max<int64_t>(1, 4 * out * nvb_));
       gemm_work_.alloc(gemm_work_elems_, &total);
     }
 
@@ -501,58 +521,80 @@ class DevicePlan final : public DevicePlanBase {
     stats.device_bytes = total;
   }
 
-  void evaluate_one(const double* d_den, double* d_out, double den_scale, double out_scale) {
+  void evaluate_one(const double* d_den, double* d_out, double den_scale, double out_scale, int nv) {
     const int pk = O_.ker.pt_kind, ek = O_.ker.exp_kind;
+    const int64_t den_vs = (int64_t)k0_ * n_src_, val_vs = (int64_t)k1_ * n_trg_;
+    const int sp = tuning.gemm_split_blocks;
     mark(0);
-    K::gather_density(st_, n_src_, k0_, d_den, src_perm_.p, den_scale, sden_.p, 1, 0);
-    K::fill(st_, eq_.p, eq_.n);
-    K::fill(st_, dn_.p, dn_.n);
+    K::gather_density(st_, n_src_, k0_, d_den, src_perm_.p, den_scale, sden_.p, nv, den_vs);
+    K::fill(st_, eq_.p, eq_vs_ * nv);
+    K::fill(st_, dn_.p, dn_vs_ * nv);
     mark(1);
     // S2U: sources -> upward check surface (RAD1), then check -> equivalent
     K::p2p_surface(st_, ek, n_s2m_, s2m_node_.p, s2m_ptr_.p, s2m_items_.p, src_begin_.p, src_end_.p, center_.p, node_size_.p,
-                   kRad1, unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, chk_.p, s2m_orow_.p, K1_, false);
+                   kRad1, unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, chk_.p, s2m_orow_.p, K1_, false, nv, den_vs, chk_vs_);
     mark(2);
-    K::gemm_rows(st_, n_s2m_, ru_, K1_, chk_.p, s2m_orow_.p, K1_, uc2ue0_.p, ctmp_.p, s2m_orow_.p, ru_, s2m_alpha_.p, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
-    K::gemm_rows(st_, n_s2m_, K0_, ru_, ctmp_.p, s2m_orow_.p, ru_, uc2ue1_.p, eq_.p, s2m_node_.p, K0_, nullptr, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
+    K::gemm_rows(st_, n_s2m_, ru_, K1_, chk_.p, s2m_orow_.p, K1_, uc2ue0_.p, ctmp_.p, s2m_orow_.p, ru_, s2m_alpha_.p, false,
+                 gemm_work_.p, gemm_work_elems_, sp, nv, chk_vs_, ctmp_vs_);
+    K::gemm_rows(st_, n_s2m_, K0_, ru_, ctmp_.p, s2m_orow_.p, ru_, uc2ue1_.p, eq_.p, s2m_node_.p, K0_, nullptr, false,
+                 gemm_work_.p, gemm_work_elems_, sp, nv, ctmp_vs_, eq_vs_);
     mark(3);
     // U2U, bottom-up: parent row = [8 children] x stacked U2U
     for (int l = max_depth_ - 1; l >= 0; l--) {
       const LevelRows& L = *m2m_[l];
-      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
+      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false, gemm_work_.p,
+                   gemm_work_elems_, sp, nv, eq_vs_, eq_vs_);
     }
     mark(4);
     // X list: source points -> downward check surface (RAD0)
     K::p2p_surface(st_, ek, n_x_, x_node_.p, x_ptr_.p, x_items_.p, src_begin_.p, src_end_.p, center_.p, node_size_.p, kRad0,
-                   unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, dn_.p, x_orow_.p, K1_, true);
+                   unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, dn_.p, x_orow_.p, K1_, true, nv, den_vs, dn_vs_);
     mark(5);
-    // V list
-    K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p, X_.p);
-    mark(6);
-    K::hadamard_tiled(st_, n_vtiles_, M_dim_, e0_, e1_, n_off_, vt_t0_.p, vt_u0_.p, vt_usrc_.p, vt_uidx_.p, vt_gmask_.p, v_offsets_.p, V_.p,
-                      X_.p, Y_.p, tuning.hadamard_tpb);
-    mark(7);
-    K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p, K1_, e1_, m_, lattice_.p, lattice_inv_.p, ns_, v_trg_scal_.p);
+    // V list, one vector at a time through the spectra buffers (with several vectors the three
+    // sub-phases are timed per vector and summed)
+    if (timing_ && nv > 1) ensure_vlist_events(nv);
+    for (int v = 0; v < nv; v++) {
+      const bool sub = timing_ && nv > 1;
+      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v], st_));
+      K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p + v * eq_vs_, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p, X_.p);
+      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 1], st_));
+      if (nv == 1) mark(6);
+      K::hadamard_tiled(st_, n_vtiles_, M_dim_, e0_, e1_, n_off_, vt_t0_.p, vt_u0_.p, vt_usrc_.p, vt_uidx_.p, vt_gmask_.p,
+                        v_offsets_.p, V_.p, X_.p, Y_.p, tuning.hadamard_tpb);
+      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 2], st_));
+      if (nv == 1) mark(7);
+      K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p + v * dn_vs_, K1_, e1_, m_, lattice_.p, lattice_inv_.p, ns_,
+                   v_trg_scal_.p);
+      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 3], st_));
+    }
+    if (nv > 1) {
+      mark(6);
+      mark(7);
+    }
     mark(8);
     // D2D, top-down: 8 children rows += parent x [D2D_0 .. D2D_7]
     for (int l = 1; l <= max_depth_; l++) {
       const LevelRows& L = *l2l_[l - 1];
-      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
+      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true, gemm_work_.p,
+                   gemm_work_elems_, sp, nv, dn_vs_, dn_vs_);
     }
     mark(9);
     // D2T: check -> equivalent
-    K::gemm_rows(st_, n_l2t_, rd_, K1_, dn_.p, l2t_ra_.p, K1_, dc2de0_.p, ctmp_.p, l2t_seq_.p, rd_, l2t_alpha_.p, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
-    K::gemm_rows(st_, n_l2t_, K0_, rd_, ctmp_.p, l2t_seq_.p, rd_, dc2de1_.p, eq_.p, l2t_rc_.p, K0_, nullptr, false, gemm_work_.p, gemm_work_elems_, tuning.gemm_split_blocks);
+    K::gemm_rows(st_, n_l2t_, rd_, K1_, dn_.p, l2t_ra_.p, K1_, dc2de0_.p, ctmp_.p, l2t_seq_.p, rd_, l2t_alpha_.p, false,
+                 gemm_work_.p, gemm_work_elems_, sp, nv, dn_vs_, ctmp_vs_);
+    K::gemm_rows(st_, n_l2t_, K0_, rd_, ctmp_.p, l2t_seq_.p, rd_, dc2de1_.p, eq_.p, l2t_rc_.p, K0_, nullptr, false,
+                 gemm_work_.p, gemm_work_elems_, sp, nv, ctmp_vs_, eq_vs_);
     mark(10);
     // equivalent surfaces -> targets (own downward equivalent + W list)
     K::far_points(st_, pk, n_tleaf_, t_leaf_.p, far_ptr_.p, far_node_.p, far_den_.p, far_rad_.p, trg_begin_.p, trg_end_.p,
-                  center_.p, node_size_.p, unit_surf_.p, ns_, tx_.p, n_trg_, eq_.p, tval_.p, false);
+                  center_.p, node_size_.p, unit_surf_.p, ns_, tx_.p, n_trg_, eq_.p, tval_.p, false, nv, eq_vs_, val_vs);
     mark(11);
     // U list: near-field P2P
     K::p2p_points(st_, pk, n_tleaf_, t_leaf_.p, u_ptr_.p, u_items_.p, trg_begin_.p, trg_end_.p, src_begin_.p, src_end_.p,
-                  center_.p, tx64_.p, n_trg_, sx64_.p, sden_.p, n_src_, tval_.p, true);
+                  center_.p, tx64_.p, n_trg_, sx64_.p, sden_.p, n_src_, tval_.p, true, nv, den_vs, val_vs);
     mark(12);
     // targets not in any leaf with targets are impossible; every target lies in exactly one leaf
-    K::scatter_output(st_, n_trg_, k1_, tval_.p, trg_perm_.p, out_scale, d_out, 1, 0);
+    K::scatter_output(st_, n_trg_, k1_, tval_.p, trg_perm_.p, out_scale, d_out, nv, val_vs);
     mark(13);
     if (timing_) {
       HIP_CHECK(hipEventSynchronize(ev_[kPhaseCount]));
@@ -560,6 +602,15 @@ class DevicePlan final : public DevicePlanBase {
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, ev_[p], ev_[p + 1]));
         phase_ms[p] = ms;
+      }
+      if (nv > 1) {
+        phase_ms[kPhFFT] = phase_ms[kPhHadamard] = phase_ms[kPhIFFT] = 0;
+        for (int v = 0; v < nv; v++)
+          for (int q = 0; q < 3; q++) {
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, vev_[4 * v + q], vev_[4 * v + q + 1]));
+            phase_ms[kPhFFT + q] += ms;
+          }
       }
     }
   }

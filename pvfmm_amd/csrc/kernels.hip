@@ -68,6 +68,36 @@ __device__ __forceinline__ void interact(const Real r0, const Real r1, const Rea
   }
 }
 
+// The same for VB density vectors sharing the geometry (one rsqrt per pair); with VB = 1 it
+// rounds exactly like interact.
+template <int KIND, int VB, typename Real>
+__device__ __forceinline__ void interact_v(const Real r0, const Real r1, const Real r2_, const Real (*f)[KDims<KIND>::k0],
+                                           Real (*u)[KDims<KIND>::k1]) {
+  const Real rr = r0 * r0 + r1 * r1 + r2_ * r2_;
+  const Real rinv = rsqrt_masked<Real>(rr);
+  if constexpr (KIND == 0) {
+#pragma unroll
+    for (int v = 0; v < VB; v++) u[v][0] = fma(rinv, f[v][0], u[v][0]);
+  } else if constexpr (KIND == 1) {
+    const Real rinv3 = rinv * rinv * rinv;
+#pragma unroll
+    for (int v = 0; v < VB; v++) {
+      const Real f3 = rinv3 * f[v][0];
+      u[v][0] = fma(r0, f3, u[v][0]);
+      u[v][1] = fma(r1, f3, u[v][1]);
+      u[v][2] = fma(r2_, f3, u[v][2]);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < VB; v++) {
+      const Real ip = (f[v][0] * r0 + f[v][1] * r1 + f[v][2] * r2_) * rinv * rinv;
+      u[v][0] = fma(rinv, f[v][0] + r0 * ip, u[v][0]);
+      u[v][1] = fma(rinv, f[v][1] + r1 * ip, u[v][1]);
+      u[v][2] = fma(rinv, f[v][2] + r2_ * ip, u[v][2]);
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------- small kernels
 template <typename Real>
 __global__ void k_gather_density(int64_t n, int k0, const double* __restrict__ in, const int64_t* __restrict__ perm,
@@ -146,17 +176,17 @@ void direct_sum(hipStream_t s, int kind, const double* src, int64_t n_src, const
 // small leaves and one barrier pair serves them all); the partial sums of a chunk are added
 // into an FP64 accumulator, and the nsplit partials of a target are added in split order at
 // the end (deterministic).
-template <typename Real, int KIND>
+template <typename Real, int KIND, int VB>
 __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tleaf, const int32_t* __restrict__ item_ptr,
                                                   const int32_t* __restrict__ items, const int32_t* __restrict__ trg_begin,
                                                   const int32_t* __restrict__ trg_end, const int32_t* __restrict__ src_begin,
                                                   const int32_t* __restrict__ src_end, const double* __restrict__ center,
                                                   const double* __restrict__ tx, int64_t n_trg, const double* __restrict__ sx,
-                                                  const Real* __restrict__ sden, int64_t n_src, double* __restrict__ out,
-                                                  bool accumulate) {
-  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1, NT = 256, SC = 512;
+                                                  const Real* __restrict__ sden, int64_t n_src, int64_t den_vs,
+                                                  double* __restrict__ out, int64_t out_vs, bool accumulate) {
+  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1, NT = 256, SC = VB * K0 >= 8 ? 256 : 512;
   __shared__ double lc[3 * SC];  // FP64 source coordinates in the target leaf's frame
-  __shared__ Real ld[K0 * SC];
+  __shared__ Real ld[VB * K0 * SC];
   __shared__ double red[K1 * NT];
   const int row = blockIdx.x, tid = threadIdx.x;
   const int T = tleaf[row];
@@ -173,9 +203,11 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
       p1 = tx[n_trg + tb + c0 + t];
       p2 = tx[2 * n_trg + tb + c0 + t];
     }
-    double acc[K1];
+    double acc[VB][K1];
 #pragma unroll
-    for (int k = 0; k < K1; k++) acc[k] = 0;
+    for (int v = 0; v < VB; v++)
+#pragma unroll
+      for (int k = 0; k < K1; k++) acc[v][k] = 0;
     int it = ib, off = 0;
     while (it < ie) {
       __syncthreads();  // the previous chunk is consumed
@@ -191,7 +223,9 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
           lc[SC + pos + q] = sx[n_src + g] + o1;
           lc[2 * SC + pos + q] = sx[2 * n_src + g] + o2;
 #pragma unroll
-          for (int c = 0; c < K0; c++) ld[c * SC + pos + q] = sden[c * n_src + g];
+          for (int v = 0; v < VB; v++)
+#pragma unroll
+            for (int c = 0; c < K0; c++) ld[(v * K0 + c) * SC + pos + q] = sden[v * den_vs + c * n_src + g];
         }
         pos += take;
         off += take;
@@ -204,33 +238,42 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
       if (active) {
         // Real partial sums over at most 64 sources, added into the FP64 accumulator
         for (int s0 = j; s0 < pos; s0 += 64 * nsplit) {
-          Real part[K1];
+          Real part[VB][K1];
 #pragma unroll
-          for (int k = 0; k < K1; k++) part[k] = 0;
+          for (int v = 0; v < VB; v++)
+#pragma unroll
+            for (int k = 0; k < K1; k++) part[v][k] = 0;
           const int s1 = min(pos, s0 + 64 * nsplit);
           for (int s = s0; s < s1; s += nsplit) {
-            Real f[K0];
+            Real f[VB][K0];
 #pragma unroll
-            for (int c = 0; c < K0; c++) f[c] = ld[c * SC + s];
-            interact<KIND, Real>((Real)(p0 - lc[s]), (Real)(p1 - lc[SC + s]), (Real)(p2 - lc[2 * SC + s]), f, part);
+            for (int v = 0; v < VB; v++)
+#pragma unroll
+              for (int c = 0; c < K0; c++) f[v][c] = ld[(v * K0 + c) * SC + s];
+            interact_v<KIND, VB, Real>((Real)(p0 - lc[s]), (Real)(p1 - lc[SC + s]), (Real)(p2 - lc[2 * SC + s]), f, part);
           }
 #pragma unroll
-          for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
+          for (int v = 0; v < VB; v++)
+#pragma unroll
+            for (int k = 0; k < K1; k++) acc[v][k] += (double)part[v][k];
         }
       }
     }
-    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K1; k++) red[k * NT + tid] = active ? acc[k] : 0.0;
-    __syncthreads();
-    if (tid < ntc) {
+    for (int v = 0; v < VB; v++) {
+      __syncthreads();
 #pragma unroll
-      for (int k = 0; k < K1; k++) {
-        double v = 0;
-        for (int jj = 0; jj < nsplit; jj++) v += red[k * NT + tid + jj * ntc];
-        double* d = &out[k * n_trg + tb + c0 + tid];
-        v *= kscale<KIND>();
-        *d = accumulate ? *d + v : v;
+      for (int k = 0; k < K1; k++) red[k * NT + tid] = active ? acc[v][k] : 0.0;
+      __syncthreads();
+      if (tid < ntc) {
+#pragma unroll
+        for (int k = 0; k < K1; k++) {
+          double sum = 0;
+          for (int jj = 0; jj < nsplit; jj++) sum += red[k * NT + tid + jj * ntc];
+          double* d = &out[v * out_vs + k * n_trg + tb + c0 + tid];
+          sum *= kscale<KIND>();
+          *d = accumulate ? *d + sum : sum;
+        }
       }
     }
   }
@@ -238,17 +281,18 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
 
 // Equivalent surfaces -> targets (own L2T surface and W-list M2T) with the leaf-block layout
 // of k_p2p_leaf; a staged source is the surface point o + r * size * unit_surf[p] of its node.
-template <typename Real, int KIND>
+template <typename Real, int KIND, int VB>
 __global__ __launch_bounds__(256) void k_far_leaf(const int32_t* __restrict__ tleaf, const int32_t* __restrict__ item_ptr,
                                                   const int32_t* __restrict__ item_node, const int64_t* __restrict__ item_den,
                                                   const double* __restrict__ item_rad, const int32_t* __restrict__ trg_begin,
                                                   const int32_t* __restrict__ trg_end, const double* __restrict__ center,
                                                   const float* __restrict__ node_size, const Real* __restrict__ unit_surf,
                                                   int ns, const Real* __restrict__ tx, int64_t n_trg,
-                                                  const Real* __restrict__ dens, double* __restrict__ out, bool accumulate) {
-  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1, NT = 256, SC = 512;
+                                                  const Real* __restrict__ dens, int64_t den_vs, double* __restrict__ out,
+                                                  int64_t out_vs, bool accumulate) {
+  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1, NT = 256, SC = VB * K0 >= 8 ? 256 : 512;
   __shared__ Real lc[3 * SC];
-  __shared__ Real ld[K0 * SC];
+  __shared__ Real ld[VB * K0 * SC];
   __shared__ double red[K1 * NT];
   const int row = blockIdx.x, tid = threadIdx.x;
   const int T = tleaf[row];
@@ -265,9 +309,11 @@ __global__ __launch_bounds__(256) void k_far_leaf(const int32_t* __restrict__ tl
       p1 = tx[n_trg + tb + c0 + t];
       p2 = tx[2 * n_trg + tb + c0 + t];
     }
-    double acc[K1];
+    double acc[VB][K1];
 #pragma unroll
-    for (int k = 0; k < K1; k++) acc[k] = 0;
+    for (int v = 0; v < VB; v++)
+#pragma unroll
+      for (int k = 0; k < K1; k++) acc[v][k] = 0;
     int it = ib, off = 0;
     while (it < ie) {
       __syncthreads();
@@ -284,7 +330,9 @@ __global__ __launch_bounds__(256) void k_far_leaf(const int32_t* __restrict__ tl
           lc[SC + pos + q] = o1 + sc * unit_surf[3 * pp + 1];
           lc[2 * SC + pos + q] = o2 + sc * unit_surf[3 * pp + 2];
 #pragma unroll
-          for (int c = 0; c < K0; c++) ld[c * SC + pos + q] = den[pp * K0 + c];
+          for (int v = 0; v < VB; v++)
+#pragma unroll
+            for (int c = 0; c < K0; c++) ld[(v * K0 + c) * SC + pos + q] = den[v * den_vs + pp * K0 + c];
         }
         pos += take;
         off += take;
@@ -296,49 +344,59 @@ __global__ __launch_bounds__(256) void k_far_leaf(const int32_t* __restrict__ tl
       __syncthreads();
       if (active) {
         for (int s0 = j; s0 < pos; s0 += 64 * nsplit) {
-          Real part[K1];
+          Real part[VB][K1];
 #pragma unroll
-          for (int k = 0; k < K1; k++) part[k] = 0;
+          for (int v = 0; v < VB; v++)
+#pragma unroll
+            for (int k = 0; k < K1; k++) part[v][k] = 0;
           const int s1 = min(pos, s0 + 64 * nsplit);
           for (int s = s0; s < s1; s += nsplit) {
-            Real f[K0];
+            Real f[VB][K0];
 #pragma unroll
-            for (int c = 0; c < K0; c++) f[c] = ld[c * SC + s];
-            interact<KIND, Real>(p0 - lc[s], p1 - lc[SC + s], p2 - lc[2 * SC + s], f, part);
+            for (int v = 0; v < VB; v++)
+#pragma unroll
+              for (int c = 0; c < K0; c++) f[v][c] = ld[(v * K0 + c) * SC + s];
+            interact_v<KIND, VB, Real>(p0 - lc[s], p1 - lc[SC + s], p2 - lc[2 * SC + s], f, part);
           }
 #pragma unroll
-          for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
+          for (int v = 0; v < VB; v++)
+#pragma unroll
+            for (int k = 0; k < K1; k++) acc[v][k] += (double)part[v][k];
         }
       }
     }
-    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K1; k++) red[k * NT + tid] = active ? acc[k] : 0.0;
-    __syncthreads();
-    if (tid < ntc) {
+    for (int v = 0; v < VB; v++) {
+      __syncthreads();
 #pragma unroll
-      for (int k = 0; k < K1; k++) {
-        double v = 0;
-        for (int jj = 0; jj < nsplit; jj++) v += red[k * NT + tid + jj * ntc];
-        double* d = &out[k * n_trg + tb + c0 + tid];
-        v *= kscale<KIND>();
-        *d = accumulate ? *d + v : v;
+      for (int k = 0; k < K1; k++) red[k * NT + tid] = active ? acc[v][k] : 0.0;
+      __syncthreads();
+      if (tid < ntc) {
+#pragma unroll
+        for (int k = 0; k < K1; k++) {
+          double sum = 0;
+          for (int jj = 0; jj < nsplit; jj++) sum += red[k * NT + tid + jj * ntc];
+          double* d = &out[v * out_vs + k * n_trg + tb + c0 + tid];
+          sum *= kscale<KIND>();
+          *d = accumulate ? *d + sum : sum;
+        }
       }
     }
   }
 }
 
 // ----------------------------------------------------------------------------- P2P: points -> surfaces
-template <typename Real, int KIND>
+template <typename Real, int KIND, int VB>
 __global__ __launch_bounds__(64) void k_p2p_surface(const int32_t* __restrict__ tnode, const int32_t* __restrict__ item_ptr,
                                                     const int32_t* __restrict__ items, const int32_t* __restrict__ src_begin,
                                                     const int32_t* __restrict__ src_end, const double* __restrict__ center,
                                                     const float* __restrict__ node_size, Real rad,
                                                     const Real* __restrict__ unit_surf, int ns, const Real* __restrict__ sx,
-                                                    const Real* __restrict__ sden, int64_t n_src, Real* __restrict__ out,
-                                                    const int32_t* __restrict__ orow, int64_t ld, bool accumulate) {
+                                                    const Real* __restrict__ sden, int64_t n_src, int64_t den_vs,
+                                                    Real* __restrict__ out, int64_t out_vs, const int32_t* __restrict__ orow,
+                                                    int64_t ld, bool accumulate) {
   constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1;
-  __shared__ Real ls[(3 + K0) * 64];
+  __shared__ Real ls[(3 + VB * K0) * 64];
   const int row = blockIdx.x, lane = threadIdx.x;
   const int p = blockIdx.y * 64 + lane;
   const int T = tnode[row];
@@ -350,17 +408,21 @@ __global__ __launch_bounds__(64) void k_p2p_surface(const int32_t* __restrict__ 
     p1 = sc * unit_surf[3 * p + 1];
     p2 = sc * unit_surf[3 * p + 2];
   }
-  double acc[K1];
+  double acc[VB][K1];
 #pragma unroll
-  for (int k = 0; k < K1; k++) acc[k] = 0;
+  for (int v = 0; v < VB; v++)
+#pragma unroll
+    for (int k = 0; k < K1; k++) acc[v][k] = 0;
   const int ib = item_ptr[row], ie = item_ptr[row + 1];
   for (int it = ib; it < ie; it++) {
     const int S = items[it];
     const int sb = src_begin[S], nsrc = src_end[S] - sb;
     const Real o0 = (Real)(center[3 * S] - ct0), o1 = (Real)(center[3 * S + 1] - ct1), o2 = (Real)(center[3 * S + 2] - ct2);
-    Real part[K1];
+    Real part[VB][K1];
 #pragma unroll
-    for (int k = 0; k < K1; k++) part[k] = 0;
+    for (int v = 0; v < VB; v++)
+#pragma unroll
+      for (int k = 0; k < K1; k++) part[v][k] = 0;
     for (int s0 = 0; s0 < nsrc; s0 += 64) {
       const int nsc = min(64, nsrc - s0);
       __syncthreads();
@@ -370,25 +432,34 @@ __global__ __launch_bounds__(64) void k_p2p_surface(const int32_t* __restrict__ 
         ls[64 + lane] = sx[n_src + g] + o1;
         ls[128 + lane] = sx[2 * n_src + g] + o2;
 #pragma unroll
-        for (int c = 0; c < K0; c++) ls[(3 + c) * 64 + lane] = sden[c * n_src + g];
+        for (int v = 0; v < VB; v++)
+#pragma unroll
+          for (int c = 0; c < K0; c++) ls[(3 + v * K0 + c) * 64 + lane] = sden[v * den_vs + c * n_src + g];
       }
       __syncthreads();
       for (int s = 0; s < nsc; s++) {
-        Real f[K0];
+        Real f[VB][K0];
 #pragma unroll
-        for (int c = 0; c < K0; c++) f[c] = ls[(3 + c) * 64 + s];
-        interact<KIND, Real>(p0 - ls[s], p1 - ls[64 + s], p2 - ls[128 + s], f, part);
+        for (int v = 0; v < VB; v++)
+#pragma unroll
+          for (int c = 0; c < K0; c++) f[v][c] = ls[(3 + v * K0 + c) * 64 + s];
+        interact_v<KIND, VB, Real>(p0 - ls[s], p1 - ls[64 + s], p2 - ls[128 + s], f, part);
       }
     }
 #pragma unroll
-    for (int k = 0; k < K1; k++) acc[k] += (double)part[k];
+    for (int v = 0; v < VB; v++)
+#pragma unroll
+      for (int k = 0; k < K1; k++) acc[v][k] += (double)part[v][k];
   }
   if (p < ns) {
-    Real* o = out + (int64_t)orow[row] * ld + p * K1;
 #pragma unroll
-    for (int k = 0; k < K1; k++) {
-      const Real v = (Real)(acc[k] * kscale<KIND>());
-      o[k] = accumulate ? o[k] + v : v;
+    for (int v = 0; v < VB; v++) {
+      Real* o = out + v * out_vs + (int64_t)orow[row] * ld + p * K1;
+#pragma unroll
+      for (int k = 0; k < K1; k++) {
+        const Real val = (Real)(acc[v][k] * kscale<KIND>());
+        o[k] = accumulate ? o[k] + val : val;
+      }
     }
   }
 }
@@ -411,7 +482,9 @@ template <> struct Mfma<double> {
 };
 
 // ----------------------------------------------------------------------------- row-gathered GEMM
-// C[rc[i]*ldc + n] (+)= alpha[i] * sum_k A[ra[i]*lda + k] * B[k*N + n] on MFMA.
+// C[rc[i]*ldc + n] (+)= alpha[i] * sum_k A[ra[i]*lda + k] * B[k*N + n] on MFMA, for nv
+// density vectors at once: the M = nv * Mv rows are (vector v, index row i), with A and C of
+// vector v offset by v * avs and v * cvs.
 // Block tile 64x64 (4 waves, each a 32x32 quadrant = 2x2 MFMA 16x16 tiles), k staged 16 at a
 // time in LDS with k contiguous for both operands so each lane feeds its four k values of an
 // MFMA group from one 4-vector read.  With ksplit > 1 the block handles one k range and writes
@@ -420,7 +493,8 @@ template <typename Real, bool ACC, bool SPLIT, bool VEC>
 __global__ __launch_bounds__(256) void k_gemm_rows(int M, int N, int K, int kchunk, const Real* __restrict__ A,
                                                    const int32_t* __restrict__ ra, int64_t lda, const Real* __restrict__ B,
                                                    Real* __restrict__ C, const int32_t* __restrict__ rc, int64_t ldc,
-                                                   const Real* __restrict__ alpha, Real* __restrict__ work) {
+                                                   const Real* __restrict__ alpha, Real* __restrict__ work, int Mv,
+                                                   int64_t avs, int64_t cvs) {
   using MF = Mfma<Real>;
   using V4 = typename MF::V4;
   constexpr int BM = 64, BN = 64, BK = 32, LD = BK + 4;
@@ -440,7 +514,8 @@ __global__ __launch_bounds__(256) void k_gemm_rows(int M, int N, int K, int kchu
   const int ar = tid >> 2, ak = (tid & 3) * 8;
   const int bk = tid >> 3, bn = (tid & 7) * 8;
   const bool arow_ok = m0 + ar < M;
-  const Real* arow = A + (arow_ok ? (int64_t)ra[m0 + ar] * lda : 0);
+  const int av = arow_ok ? (m0 + ar) / Mv : 0, ai = arow_ok ? m0 + ar - av * Mv : 0;
+  const Real* arow = A + (arow_ok ? av * avs + (int64_t)ra[ai] * lda : 0);
   Real ra_[8], rb_[8];
   // load the k-step starting at k0 into registers (zero outside the matrix)
   auto load = [&](int k0) __attribute__((always_inline)) {
@@ -516,8 +591,9 @@ __global__ __launch_bounds__(256) void k_gemm_rows(int M, int N, int K, int kchu
         if (SPLIT) {
           work[((int64_t)blockIdx.z * M + gm) * N + gn] = v;
         } else {
-          Real* crow = C + (int64_t)rc[gm] * ldc;
-          const Real al = alpha ? alpha[gm] : Real(1);
+          const int cv = gm / Mv, ci = gm - cv * Mv;
+          Real* crow = C + cv * cvs + (int64_t)rc[ci] * ldc;
+          const Real al = alpha ? alpha[ci] : Real(1);
           crow[gn] = ACC ? fma(al, v, crow[gn]) : al * v;
         }
       }
@@ -525,14 +601,16 @@ __global__ __launch_bounds__(256) void k_gemm_rows(int M, int N, int K, int kchu
 
 template <typename Real, bool ACC>
 __global__ void k_gemm_reduce(int M, int N, int ksplit, const Real* __restrict__ work, Real* __restrict__ C,
-                              const int32_t* __restrict__ rc, int64_t ldc, const Real* __restrict__ alpha) {
+                              const int32_t* __restrict__ rc, int64_t ldc, const Real* __restrict__ alpha, int Mv,
+                              int64_t cvs) {
   const int64_t total = (int64_t)M * N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(e / N), n = (int)(e % N);
     Real v = 0;
     for (int z = 0; z < ksplit; z++) v += work[((int64_t)z * M + i) * N + n];
-    Real* crow = C + (int64_t)rc[i] * ldc;
-    const Real al = alpha ? alpha[i] : Real(1);
+    const int cv = i / Mv, ci = i - cv * Mv;
+    Real* crow = C + cv * cvs + (int64_t)rc[ci] * ldc;
+    const Real al = alpha ? alpha[ci] : Real(1);
     crow[n] = ACC ? fma(al, v, crow[n]) : al * v;
   }
 }
@@ -1511,23 +1589,44 @@ void DeviceKernels<Real>::fill64(hipStream_t s, double* p, int64_t n) {
   HIP_LAUNCH_CHECK();
 }
 
+// Density vectors in chunks of 8, 4, 2 and 1 per launch (the chunk shares the geometry).
+template <typename F>
+static void for_density_chunks(int nv, F&& launch) {
+  int v0 = 0;
+  while (v0 < nv) {
+    const int left = nv - v0, vb = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    launch(v0, vb);
+    v0 += vb;
+  }
+}
+
 template <typename Real>
 void DeviceKernels<Real>::p2p_surface(hipStream_t s, int kind, int n_rows, const int32_t* tnode, const int32_t* item_ptr,
                                       const int32_t* items, const int32_t* src_begin, const int32_t* src_end,
                                       const double* center, const float* node_size, double rad, const Real* unit_surf, int ns,
                                       const Real* sx, const Real* sden, int64_t n_src, Real* out, const int32_t* orow,
-                                      int64_t ld, bool accumulate) {
-  if (n_rows == 0) return;
-  dim3 g(n_rows, (ns + 63) / 64);
-  if (kind == 0)
-    hipLaunchKernelGGL((k_p2p_surface<Real, 0>), g, dim3(64), 0, s, tnode, item_ptr, items, src_begin, src_end, center, node_size,
-                       (Real)rad, unit_surf, ns, sx, sden, n_src, out, orow, ld, accumulate);
-  else if (kind == 2)
-    hipLaunchKernelGGL((k_p2p_surface<Real, 2>), g, dim3(64), 0, s, tnode, item_ptr, items, src_begin, src_end, center, node_size,
-                       (Real)rad, unit_surf, ns, sx, sden, n_src, out, orow, ld, accumulate);
-  else
-    throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-  HIP_LAUNCH_CHECK();
+                                      int64_t ld, bool accumulate, int nv, int64_t den_vs, int64_t out_vs) {
+  if (n_rows == 0 || nv <= 0) return;
+  if (kind != 0 && kind != 2) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+  const dim3 g(n_rows, (ns + 63) / 64);
+  for_density_chunks(nv, [&](int v0, int vb) {
+    const Real* d = sden + v0 * den_vs;
+    Real* o = out + v0 * out_vs;
+#define SURF_LAUNCH(KD, VB)                                                                                                 \
+  hipLaunchKernelGGL((k_p2p_surface<Real, KD, VB>), g, dim3(64), 0, s, tnode, item_ptr, items, src_begin, src_end, center, \
+                     node_size, (Real)rad, unit_surf, ns, sx, d, n_src, den_vs, o, out_vs, orow, ld, accumulate)
+#define SURF_VB(KD)                   \
+  switch (vb) {                       \
+    case 8: SURF_LAUNCH(KD, 8); break; \
+    case 4: SURF_LAUNCH(KD, 4); break; \
+    case 2: SURF_LAUNCH(KD, 2); break; \
+    default: SURF_LAUNCH(KD, 1);       \
+  }
+    if (kind == 0) { SURF_VB(0) } else { SURF_VB(2) }
+#undef SURF_VB
+#undef SURF_LAUNCH
+    HIP_LAUNCH_CHECK();
+  });
 }
 
 template <typename Real>
@@ -1535,14 +1634,26 @@ void DeviceKernels<Real>::p2p_points(hipStream_t s, int kind, int n_rows, const 
                                      const int32_t* items, const int32_t* trg_begin, const int32_t* trg_end,
                                      const int32_t* src_begin, const int32_t* src_end, const double* center, const double* tx,
                                      int64_t n_trg, const double* sx, const Real* sden, int64_t n_src, double* out,
-                                     bool accumulate) {
-  if (n_rows == 0) return;
-#define P2P_ARGS tleaf, item_ptr, items, trg_begin, trg_end, src_begin, src_end, center, tx, n_trg, sx, sden, n_src, out, accumulate
-  if (kind == 0) hipLaunchKernelGGL((k_p2p_leaf<Real, 0>), dim3(n_rows), dim3(256), 0, s, P2P_ARGS);
-  else if (kind == 1) hipLaunchKernelGGL((k_p2p_leaf<Real, 1>), dim3(n_rows), dim3(256), 0, s, P2P_ARGS);
-  else hipLaunchKernelGGL((k_p2p_leaf<Real, 2>), dim3(n_rows), dim3(256), 0, s, P2P_ARGS);
-#undef P2P_ARGS
-  HIP_LAUNCH_CHECK();
+                                     bool accumulate, int nv, int64_t den_vs, int64_t out_vs) {
+  if (n_rows == 0 || nv <= 0) return;
+  for_density_chunks(nv, [&](int v0, int vb) {
+    const Real* d = sden + v0 * den_vs;
+    double* o = out + v0 * out_vs;
+#define LEAF_LAUNCH(KD, VB)                                                                                                 \
+  hipLaunchKernelGGL((k_p2p_leaf<Real, KD, VB>), dim3(n_rows), dim3(256), 0, s, tleaf, item_ptr, items, trg_begin, trg_end, \
+                     src_begin, src_end, center, tx, n_trg, sx, d, n_src, den_vs, o, out_vs, accumulate)
+#define LEAF_VB(KD)                   \
+  switch (vb) {                       \
+    case 8: LEAF_LAUNCH(KD, 8); break; \
+    case 4: LEAF_LAUNCH(KD, 4); break; \
+    case 2: LEAF_LAUNCH(KD, 2); break; \
+    default: LEAF_LAUNCH(KD, 1);       \
+  }
+    if (kind == 0) { LEAF_VB(0) } else if (kind == 1) { LEAF_VB(1) } else { LEAF_VB(2) }
+#undef LEAF_VB
+#undef LEAF_LAUNCH
+    HIP_LAUNCH_CHECK();
+  });
 }
 
 template <typename Real>
@@ -1550,21 +1661,36 @@ void DeviceKernels<Real>::far_points(hipStream_t s, int kind, int n_rows, const 
                                      const int32_t* item_node, const int64_t* item_den, const double* item_rad,
                                      const int32_t* trg_begin, const int32_t* trg_end, const double* center,
                                      const float* node_size, const Real* unit_surf, int ns, const Real* tx, int64_t n_trg,
-                                     const Real* dens, double* out, bool accumulate) {
-  if (n_rows == 0) return;
-#define FAR_ARGS tleaf, item_ptr, item_node, item_den, item_rad, trg_begin, trg_end, center, node_size, unit_surf, ns, tx, n_trg, dens, out, accumulate
-  if (kind == 0) hipLaunchKernelGGL((k_far_leaf<Real, 0>), dim3(n_rows), dim3(256), 0, s, FAR_ARGS);
-  else if (kind == 1) hipLaunchKernelGGL((k_far_leaf<Real, 1>), dim3(n_rows), dim3(256), 0, s, FAR_ARGS);
-  else hipLaunchKernelGGL((k_far_leaf<Real, 2>), dim3(n_rows), dim3(256), 0, s, FAR_ARGS);
-#undef FAR_ARGS
-  HIP_LAUNCH_CHECK();
+                                     const Real* dens, double* out, bool accumulate, int nv, int64_t den_vs, int64_t out_vs) {
+  if (n_rows == 0 || nv <= 0) return;
+  for_density_chunks(nv, [&](int v0, int vb) {
+    const Real* d = dens + v0 * den_vs;
+    double* o = out + v0 * out_vs;
+#define FAR_LAUNCH(KD, VB)                                                                                                   \
+  hipLaunchKernelGGL((k_far_leaf<Real, KD, VB>), dim3(n_rows), dim3(256), 0, s, tleaf, item_ptr, item_node, item_den, item_rad, \
+                     trg_begin, trg_end, center, node_size, unit_surf, ns, tx, n_trg, d, den_vs, o, out_vs, accumulate)
+#define FAR_VB(KD)                   \
+  switch (vb) {                      \
+    case 8: FAR_LAUNCH(KD, 8); break; \
+    case 4: FAR_LAUNCH(KD, 4); break; \
+    case 2: FAR_LAUNCH(KD, 2); break; \
+    default: FAR_LAUNCH(KD, 1);       \
+  }
+    if (kind == 0) { FAR_VB(0) } else if (kind == 1) { FAR_VB(1) } else { FAR_VB(2) }
+#undef FAR_VB
+#undef FAR_LAUNCH
+    HIP_LAUNCH_CHECK();
+  });
 }
 
 template <typename Real>
 void DeviceKernels<Real>::gemm_rows(hipStream_t s, int M, int N, int K, const Real* A, const int32_t* ra, int64_t lda,
                                     const Real* B, Real* C, const int32_t* rc, int64_t ldc, const Real* alpha, bool accumulate,
-                                    Real* work, int64_t work_elems, int split_blocks) {
-  if (M <= 0 || N <= 0 || K <= 0) return;
+                                    Real* work, int64_t work_elems, int split_blocks, int nv, int64_t avs, int64_t cvs) {
+  if (M <= 0 || N <= 0 || K <= 0 || nv <= 0) return;
+  const int Mv = M;
+  if ((int64_t)M * nv >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+  M *= nv;
   const int gx = (N + 63) / 64, gy = (M + 63) / 64;
   // split K when the tile grid cannot fill the 256 CUs, in chunks of whole 16-wide k steps
   int ksplit = 1;
@@ -1578,25 +1704,27 @@ void DeviceKernels<Real>::gemm_rows(hipStream_t s, int M, int N, int K, const Re
   ksplit = (K + kchunk - 1) / kchunk;
   dim3 g((unsigned)((blocks + 7) / 8 * 8), 1, ksplit);
   // 4-vector loads when every row start and the B rows are 16-byte aligned
-  const bool vec = lda % 4 == 0 && N % 4 == 0 && (reinterpret_cast<uintptr_t>(A) % 32) == 0 &&
+  const bool vec = lda % 4 == 0 && avs % 4 == 0 && N % 4 == 0 && (reinterpret_cast<uintptr_t>(A) % 32) == 0 &&
                    (reinterpret_cast<uintptr_t>(B) % 32) == 0;
 #define GEMM_LAUNCH(ACCV, SPLITV)                                                                                     \
   do {                                                                                                                \
     if (vec)                                                                                                          \
       hipLaunchKernelGGL((k_gemm_rows<Real, ACCV, SPLITV, true>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, \
-                         ldc, alpha, work);                                                                           \
+                         ldc, alpha, work, Mv, avs, cvs);                                                             \
     else                                                                                                              \
       hipLaunchKernelGGL((k_gemm_rows<Real, ACCV, SPLITV, false>), g, dim3(256), 0, s, M, N, K, kchunk, A, ra, lda, B, C, rc, \
-                         ldc, alpha, work);                                                                           \
+                         ldc, alpha, work, Mv, avs, cvs);                                                             \
   } while (0)
   if (ksplit > 1) {
     GEMM_LAUNCH(false, true);
     HIP_LAUNCH_CHECK();
     const int64_t tot = (int64_t)M * N;
     if (accumulate)
-      hipLaunchKernelGGL((k_gemm_reduce<Real, true>), dim3(grid_for(tot, 256)), dim3(256), 0, s, M, N, ksplit, work, C, rc, ldc, alpha);
+      hipLaunchKernelGGL((k_gemm_reduce<Real, true>), dim3(grid_for(tot, 256)), dim3(256), 0, s, M, N, ksplit, work, C, rc, ldc, alpha,
+                         Mv, cvs);
     else
-      hipLaunchKernelGGL((k_gemm_reduce<Real, false>), dim3(grid_for(tot, 256)), dim3(256), 0, s, M, N, ksplit, work, C, rc, ldc, alpha);
+      hipLaunchKernelGGL((k_gemm_reduce<Real, false>), dim3(grid_for(tot, 256)), dim3(256), 0, s, M, N, ksplit, work, C, rc, ldc, alpha,
+                         Mv, cvs);
   } else if (accumulate) {
     GEMM_LAUNCH(true, false);
   } else {

@@ -26,15 +26,17 @@ struct DeviceKernels {
                           const int32_t* items, const int32_t* src_begin, const int32_t* src_end,
                           const double* center, const float* node_size, double rad, const Real* unit_surf, int ns,
                           const Real* sx, const Real* sden, int64_t n_src, Real* out, const int32_t* orow,
-                          int64_t ld, bool accumulate);
+                          int64_t ld, bool accumulate, int nv = 1, int64_t den_vs = 0, int64_t out_vs = 0);
 
+  // All P2P launchers take nv density vectors: vector v reads densities at + v * den_vs and
+  // writes values at + v * out_vs; up to 8 vectors share one pass over the pair geometry.
   // P2P into target points of leaves: point-range sources (U list).  tx/sx are FP64
   // leaf-local coordinates; the displacement is formed in FP64, the kernel runs in Real.
   static void p2p_points(hipStream_t s, int kind, int n_rows, const int32_t* tleaf, const int32_t* item_ptr,
                          const int32_t* items, const int32_t* trg_begin, const int32_t* trg_end,
                          const int32_t* src_begin, const int32_t* src_end, const double* center,
                          const double* tx, int64_t n_trg, const double* sx, const Real* sden, int64_t n_src,
-                         double* out, bool accumulate);
+                         double* out, bool accumulate, int nv = 1, int64_t den_vs = 0, int64_t out_vs = 0);
 
   // P2P into target points from surface sources (L2T own surface and W-list equivalent
   // surfaces).  items are pairs (node, density row pointer index): den row = dens + drow*ld.
@@ -42,15 +44,18 @@ struct DeviceKernels {
                          const int32_t* item_node, const int64_t* item_den, const double* item_rad,
                          const int32_t* trg_begin, const int32_t* trg_end, const double* center,
                          const float* node_size, const Real* unit_surf, int ns, const Real* tx, int64_t n_trg,
-                         const Real* dens, double* out, bool accumulate);
+                         const Real* dens, double* out, bool accumulate, int nv = 1, int64_t den_vs = 0,
+                         int64_t out_vs = 0);
 
   // C[rc[i]*ldc : +N] (+)= alpha[i] * A[ra[i]*lda : +K] * B[K x N] on MFMA; `work` (work_elems
   // Reals, may be null) enables a deterministic split-K for grids too small to fill the GPU.
   // split_blocks: K is split (through work[], work_elems reals) when the tile grid has fewer
-  // blocks than this, so that about twice as many run
+  // blocks than this, so that about twice as many run.  nv density vectors run as one GEMM of
+  // nv * M rows: vector v reads A + v * avs and writes C + v * cvs (same ra, rc, alpha).
   static void gemm_rows(hipStream_t s, int M, int N, int K, const Real* A, const int32_t* ra, int64_t lda,
                         const Real* B, Real* C, const int32_t* rc, int64_t ldc, const Real* alpha,
-                        bool accumulate, Real* work = nullptr, int64_t work_elems = 0, int split_blocks = 1024);
+                        bool accumulate, Real* work = nullptr, int64_t work_elems = 0, int split_blocks = 1024,
+                        int nv = 1, int64_t avs = 0, int64_t cvs = 0);
 
   // V-list: forward DFT of the 8 children's equivalent densities of each source parent.
   // X layout [slot][f][8*e0] complex.  lattice[3p..3p+2] is the m^3 lattice cell of surface

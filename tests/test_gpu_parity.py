@@ -119,6 +119,29 @@ def test_linearity_and_multiple_densities(gpu):
     assert rel(lin, 2.0 * u[0] - 0.5 * u[1]) < 1e-12
 
 
+@pytest.mark.parametrize("kernel,prec,tol", [("laplace_grad", "fp64", 1e-12), ("laplace_grad", "mixed", 1e-6),
+                                              ("stokes_vel", "mixed", 1e-6), ("laplace_poten", "fp64", 1e-12)])
+def test_batched_density_passes(gpu, kernel, prec, tol):
+    """n_vec_max = 8 batches the densities: 11 vectors run as passes of 8 and 3 (P2P chunks of
+    8, 2 and 1 sharing the pair geometry; one GEMM of 8 or 3 row blocks per up/down step).
+    Each vector must match its own single-vector evaluation on a tree set up for one vector
+    (the per-pair arithmetic is the same; a GEMM K split may differ, so to rounding)."""
+    rng = np.random.default_rng(31)
+    n = 6000
+    x = rng.random((n, 3))
+    k0, k1 = P.kernel_dims(kernel)
+    q = rng.standard_normal((11, k0 * n))
+    fmm = P.PtFMM(kernel, 6, prec)
+    tb = P.PtFMMTree(x, x, 64).setup(fmm, n_vec_max=8)
+    t1 = P.PtFMMTree(x, x, 64).setup(fmm, n_vec_max=1)
+    ub = tb.evaluate(q)
+    for i in range(11):
+        r = rel(ub[i], t1.evaluate(q[i]))
+        assert r < tol, (i, r)
+    # the batched values are the FMM's values: one vector against the direct sum
+    assert rel(ub[9], O.direct(KID[kernel], x, q[9], x)) < (1e-4 if prec == "mixed" else 1e-5)
+
+
 def test_separate_targets_and_edge_cases(gpu):
     rng = np.random.default_rng(16)
     src = rng.random((3000, 3))
