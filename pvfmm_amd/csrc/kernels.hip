@@ -943,7 +943,8 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
     __syncthreads();
     const int ntask = 8 * M * kn;
     for (int T = t; T < ntask; T += NT) {  // y and z stages
-      const int c = T % 8, ky = (T / 8) % M, kl = T / (8 * M), kx = k0 + kl;
+      // child fastest, then kx: a wave's stores of one (ky, kz) cover consecutive frequencies
+      const int c = T % 8, kl = (T / 8) % kn, ky = T / (8 * kn), kx = k0 + kl;
       const Real* pr = &s1r[c * CST + kl * M2];
       const Real* pi = &s1i[c * CST + kl * M2];
       // z stage of one y-stage output column g (z < m) for frequency row kyy
