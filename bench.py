@@ -218,10 +218,17 @@ def bench_config(name: str, steps: int, warmup: int, check: bool):
         res = {}
         for prec in (["fp64"] if name == "1" else ["mixed", "fp64"]):
             tree.setup(P.PtFMM(c["kernel"], c["m"], prec))
+            shapes = tree.autotune(dq)
             t = timed(lambda: tree.evaluate_device(dq))
             out = tree.evaluate_device(dq).cpu().numpy()
-            res[prec] = {"ms_per_step": 1e3 * t, "mtargets_s": n / t / 1e6, "rel_l2": err(c["kernel"], c["src"], c["den"], out, k1)}
+            tree.set_timing(True)
+            tree.evaluate_device(dq)
+            phases = {k: round(v, 3) for k, v in tree.phase_times().items()}
+            tree.set_timing(False)
+            res[prec] = {"ms_per_step": 1e3 * t, "mtargets_s": n / t / 1e6, "rel_l2": err(c["kernel"], c["src"], c["den"], out, k1),
+                         "launch_shapes": shapes, "phases_ms": phases}
             log(f"config {name} {prec}: {1e3 * t:.2f} ms ({n / t / 1e6:.1f} Mtargets/s), rel-L2 {res[prec]['rel_l2']}")
+            log("   phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
         main_prec = "fp64" if name == "1" else "mixed"
         line.update(metric=f"{c['kernel']} FMM Mtargets/s", value=res[main_prec]["mtargets_s"], unit="Mtargets/s",
                     ms_per_step=res[main_prec]["ms_per_step"], dtype="f64" if main_prec == "fp64" else "f32",
