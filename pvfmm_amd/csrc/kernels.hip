@@ -1152,7 +1152,8 @@ constexpr int hadamard_blocks_per_cu() {
 template <typename Real, int E0, int E1, int FB, int UMAX, int NACC>
 __global__ __launch_bounds__(256, (hadamard_blocks_per_cu<Real, E0, FB, UMAX>())) void k_hadamard_tiled(int M_dim, int n_tiles, const int32_t* __restrict__ tile_t0,
                                                         const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
-                                                        const uint16_t* __restrict__ uidx, const int32_t* __restrict__ offsets,
+                                                        const uint16_t* __restrict__ uidx, const uint32_t* __restrict__ gmask,
+                                                        const int32_t* __restrict__ offsets,
                                                         const Real* __restrict__ Vp, const Real* __restrict__ X,
                                                         Real* __restrict__ Y) {
   using MF = Mfma<Real>;
@@ -1248,6 +1249,14 @@ __global__ __launch_bounds__(256, (hadamard_blocks_per_cu<Real, E0, FB, UMAX>())
     for (int q = 0; q < NB; q++) asm volatile("" : "+v"(bq[p][q]));
   __syncthreads();
   const int row = l & 15, kg = l >> 4;
+  // offsets present per 16-target group of the tile: for the FP32 3x3 (Stokes) blocks a group's
+  // 36 MFMAs at an offset where none of its targets has a partner are skipped (block-uniform
+  // branches).  The scalar kernels keep the dense stream: there a branch guards only 4 MFMAs
+  // and makes the compiler copy the accumulators through VGPRs around it; the FP64 3x3 form
+  // already uses 252 of its 256 VGPRs and would spill.
+  uint32_t gm[4];
+#pragma unroll
+  for (int g = 0; g < 4; g++) gm[g] = gmask[4 * tile + g];
   // NACC accumulator sets (offset j uses set j % NACC): independent MFMA chains
   V4 accs[NACC][4][E1];
 #pragma unroll
@@ -1306,6 +1315,8 @@ __global__ __launch_bounds__(256, (hadamard_blocks_per_cu<Real, E0, FB, UMAX>())
     }
 #pragma unroll
     for (int g = 0; g < 4; g++) {
+      if constexpr (sizeof(Real) == 4)
+        if (!((gm[g] >> oi) & 1u)) continue;
       const int ue = ul[(oi * 16 + row) * 4 + g];  // missing partners point at the zero entry
 #pragma unroll
       for (int kb = 0; kb < E0; kb++) {
@@ -1809,10 +1820,10 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
     const int64_t nbt = (int64_t)n_tiles * ((n_fgt + 7) / 8) * 8;
     if (nbt >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
     hipLaunchKernelGGL((k_hadamard_tiled<Real, 1, 1, FBT, UM, NA>), dim3((unsigned)nbt), dim3(256), 0, s, M_dim, n_tiles,
-                       tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
+                       tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
   } else if (e0 == 3 && e1 == 3) {
     hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, FB, 80, 1>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
-                       tile_t0, tile_u0, usrc, uidx, offsets, Vp, X, Y);
+                       tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
   } else {
     throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   }
