@@ -490,7 +490,7 @@ template <> struct Mfma<double> {
 // MFMA group from one 4-vector read.  With ksplit > 1 the block handles one k range and writes
 // its partial tile to work[z][i][n]; k_gemm_reduce then sums the partials in a fixed order.
 template <typename Real, bool ACC, bool SPLIT, bool VEC>
-__global__ __launch_bounds__(256) void k_gemm_rows(int M, int N, int K, int kchunk, const Real* __restrict__ A,
+__global__ __launch_bounds__(256, (sizeof(Real) == 4 ? 8 : 4)) void k_gemm_rows(int M, int N, int K, int kchunk, const Real* __restrict__ A,
                                                    const int32_t* __restrict__ ra, int64_t lda, const Real* __restrict__ B,
                                                    Real* __restrict__ C, const int32_t* __restrict__ rc, int64_t ldc,
                                                    const Real* __restrict__ alpha, Real* __restrict__ work, int Mv,
