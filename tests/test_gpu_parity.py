@@ -82,6 +82,34 @@ def test_mixed_matches_oracle(gpu, kernel, kind, m, max_pts):
     assert e < (1e-4 if kernel == "stokes_vel" else 1e-5)
 
 
+@pytest.mark.parametrize("m", [2, 4, 12, 14])
+def test_other_orders_match_oracle(gpu, m):
+    """Multipole orders outside the headline set: m = 4 and 12 use the compile-time DFTs,
+    m = 2 and 14 the generic ones.  FP64 against the oracle at the same m (m = 14 against the
+    direct sum only: its oracle precompute is slow), mixed against the FP32 bar."""
+    rng = np.random.default_rng(40 + m)
+    n = 3000
+    x = rng.random((n, 3))
+    q = rng.standard_normal(n)
+    idx = rng.choice(n, 300, replace=False)
+    ud = O.direct("laplace_poten", x, q, x[idx])
+    u, _ = run_product(x, x, q, "laplace_poten", m, 64)
+    e_dir = rel(u[idx], ud)
+    if m <= 12:
+        uo = run_oracle(x, x, q, "laplace_poten", m, 64)
+        e_or = rel(u, uo)
+        print(f"m={m} fp64: vs oracle {e_or:.2e}, vs direct {e_dir:.2e}")
+        assert e_or < 1e-6
+    else:
+        print(f"m={m} fp64: vs direct {e_dir:.2e}")
+    # expected accuracy of the expansion order (kernel-independent FMM, max_pts = 64)
+    assert e_dir < {2: 2e-1, 4: 1e-2, 12: 1e-6, 14: 1e-7}[m]
+    um, _ = run_product(x, x, q, "laplace_poten", m, 64, "mixed")
+    e_mix = rel(um[idx], ud)
+    print(f"m={m} mixed: vs direct {e_mix:.2e}")
+    assert e_mix < max(2 * e_dir, 2e-6)
+
+
 def test_m10_fp64_and_mixed_vs_direct(gpu):
     rng = np.random.default_rng(13)
     n = 30000
