@@ -11,7 +11,7 @@ same way and the ratio reported; both errors are relative L2 against an FP64 dir
 sampled targets.  The CPU baseline is the FP64 oracle (a restatement of the reference
 algorithm) on a 2^18-point sample of the same distribution, on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--n N]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--n N] [--workload NAME]
 """
 from __future__ import annotations
 
@@ -66,7 +66,7 @@ def cpu_baseline(n_sample: int):
     import oracle
     from pvfmm_amd import workloads as W
 
-    c = W.config2(n_sample)
+    c = W.adaptive_grad_2m(n_sample)
     ops = oracle.precompute(c["kernel"], c["m"])
     tr = oracle.OracleTree(c["src"], c["trg"], c["max_pts"])
     tr.set_operators(ops)
@@ -86,7 +86,7 @@ def cpu_baseline(n_sample: int):
 def read_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary (FETCH_SIZE and
     WRITE_SIZE runs, calibrated against a copy kernel of known size), or None if absent."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_config2.txt")
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_adaptive_grad_2m_m10_mixed.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -154,7 +154,7 @@ def bwd_fwd(steps: int, warmup: int, n: int):
     from pvfmm_amd import workloads as W
     from pvfmm_amd.autograd import LaplaceFMM
 
-    c = W.config5(n)
+    c = W.uniform_energy_1m(n)
     fmm = LaplaceFMM(c["src"], m=c["m"], max_pts=c["max_pts"], precision="mixed")
     x = torch.tensor(c["src"], device="cuda:0", requires_grad=True)
     q = torch.tensor(c["den"], device="cuda:0", requires_grad=True)
@@ -178,10 +178,11 @@ def bwd_fwd(steps: int, warmup: int, n: int):
             "fwd_ms": 1e3 * tf, "bwd_ms": 1e3 * tb, "ratio": tb / tf}
 
 
-def bench_config(name: str, steps: int, warmup: int, check: bool):
-    """Secondary workloads (one JSON line each): 1 laplace_poten FP64 uniform; 3 stokes_vel on an
-    ellipsoid (mixed and FP64); 4a 32 density vectors on one tree (mixed); 4b 64 independent
-    clouds evaluated concurrently on their own streams (mixed); 5 energy forward/backward."""
+def bench_workload(name: str, steps: int, warmup: int, check: bool):
+    """Secondary workloads (one JSON line each): uniform-poten-64k (laplace_poten FP64, 65,536
+    uniform points); ellipsoid-stokes-1m (stokes_vel on an ellipsoid, mixed and FP64);
+    densities-32 (32 density vectors on one tree, mixed); clouds-64 (64 independent clouds as one
+    graph launch, mixed); energy-1m (energy forward/backward)."""
     import torch
 
     import pvfmm_amd as P
@@ -209,14 +210,14 @@ def bench_config(name: str, steps: int, warmup: int, check: bool):
 
     line = {"n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "data": "synthetic"}
-    if name in ("1", "3"):
-        c = W.config1() if name == "1" else W.config3()
+    if name in ("uniform-poten-64k", "ellipsoid-stokes-1m"):
+        c = W.uniform_poten_64k() if name == "uniform-poten-64k" else W.ellipsoid_stokes_1m()
         n = len(c["src"])
         k0, k1 = P.kernel_dims(c["kernel"])
         tree = P.PtFMMTree(c["src"], c["trg"], c["max_pts"])
         dq = torch.from_numpy(c["den"]).to("cuda:0")
         res = {}
-        for prec in (["fp64"] if name == "1" else ["mixed", "fp64"]):
+        for prec in (["fp64"] if name == "uniform-poten-64k" else ["mixed", "fp64"]):
             tree.setup(P.PtFMM(c["kernel"], c["m"], prec))
             shapes = tree.autotune(dq)
             t = timed(lambda: tree.evaluate_device(dq))
@@ -227,15 +228,15 @@ def bench_config(name: str, steps: int, warmup: int, check: bool):
             tree.set_timing(False)
             res[prec] = {"ms_per_step": 1e3 * t, "mtargets_s": n / t / 1e6, "rel_l2": err(c["kernel"], c["src"], c["den"], out, k1),
                          "launch_shapes": shapes, "phases_ms": phases}
-            log(f"config {name} {prec}: {1e3 * t:.2f} ms ({n / t / 1e6:.1f} Mtargets/s), rel-L2 {res[prec]['rel_l2']}")
+            log(f"{name} {prec}: {1e3 * t:.2f} ms ({n / t / 1e6:.1f} Mtargets/s), rel-L2 {res[prec]['rel_l2']}")
             log("   phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
-        main_prec = "fp64" if name == "1" else "mixed"
+        main_prec = "fp64" if name == "uniform-poten-64k" else "mixed"
         line.update(metric=f"{c['kernel']} FMM Mtargets/s", value=res[main_prec]["mtargets_s"], unit="Mtargets/s",
                     ms_per_step=res[main_prec]["ms_per_step"], dtype="f64" if main_prec == "fp64" else "f32",
-                    config={"workload": f"{c['kernel']}, N={n}, m={c['m']}, max_pts={c['max_pts']}", "config": name},
+                    config={"workload": f"{name}: {c['kernel']}, N={n}, m={c['m']}, max_pts={c['max_pts']}"},
                     results=res)
-    elif name == "4a":
-        c = W.config4a()
+    elif name == "densities-32":
+        c = W.uniform_poten_1m_32_densities()
         n, nd = c["den"].shape[1], c["den"].shape[0]
         tree = P.PtFMMTree(c["src"], c["trg"], c["max_pts"]).setup(P.PtFMM(c["kernel"], c["m"], "mixed"), n_vec_max=nd)
         dq = torch.from_numpy(c["den"]).to("cuda:0")
@@ -250,15 +251,15 @@ def bench_config(name: str, steps: int, warmup: int, check: bool):
         # the same densities one at a time on a tree set up for one vector
         tree1 = P.PtFMMTree(c["src"], c["trg"], c["max_pts"]).setup(P.PtFMM(c["kernel"], c["m"], "mixed"), n_vec_max=1)
         t1 = timed(lambda: tree1.evaluate_device(dq))
-        log(f"config 4a: {1e3 * t:.2f} ms for {nd} densities in one pass ({n * nd / t / 1e6:.1f} Mtarget-densities/s), "
+        log(f"{name}: {1e3 * t:.2f} ms for {nd} densities in one pass ({n * nd / t / 1e6:.1f} Mtarget-densities/s), "
             f"{1e3 * t1:.2f} ms one at a time, rel-L2 {e}")
         line.update(metric="laplace_poten FMM Mtarget-densities/s", value=n * nd / t / 1e6, unit="Mtargets/s",
                     ms_per_step=1e3 * t, dtype="f32", rel_l2=e, one_at_a_time_ms=1e3 * t1,
                     phases_ms={k: round(v, 3) for k, v in phases.items()},
-                    config={"workload": f"laplace_poten, N={n} uniform, {nd} density vectors in one batched pass, "
-                                        f"m={c['m']}, max_pts={c['max_pts']}, mixed", "config": "4a"})
-    elif name == "4b":
-        c = W.config4b()
+                    config={"workload": f"{name}: laplace_poten, N={n} uniform, {nd} density vectors in one batched "
+                                        f"pass, m={c['m']}, max_pts={c['max_pts']}, mixed"})
+    elif name == "clouds-64":
+        c = W.uniform_clouds_64()
         fmm = P.PtFMM(c["kernel"], c["m"], "mixed")
         trees = [P.PtFMMTree(x, x, c["max_pts"]).setup(fmm) for x, _ in c["clouds"]]
         dens = [torch.from_numpy(q).to("cuda:0") for _, q in c["clouds"]]
@@ -283,21 +284,21 @@ def bench_config(name: str, steps: int, warmup: int, check: bool):
         for tu in tunes:
             key = f"{tu['hadamard_tiles_per_block']}/{tu['gemm_split_blocks']}"
             shapes[key] = shapes.get(key, 0) + 1
-        log(f"config 4b: {1e3 * t:.2f} ms for {len(trees)} clouds, {ntot} targets ({ntot / t / 1e6:.1f} Mtargets/s) "
+        log(f"{name}: {1e3 * t:.2f} ms for {len(trees)} clouds, {ntot} targets ({ntot / t / 1e6:.1f} Mtargets/s) "
             f"as one graph launch; {1e3 * t_streams:.2f} ms with one stream per cloud before tuning; "
             f"autotune {t_tune:.1f} s, shapes (tiles per block / split blocks: clouds) {shapes}")
         line.update(metric="laplace_poten FMM Mtargets/s over 64 clouds", value=ntot / t / 1e6, unit="Mtargets/s",
                     ms_per_step=1e3 * t, dtype="f32", rel_l2_cloud0=e, streams_ms=1e3 * t_streams,
                     autotune_s=t_tune, autotuned_shapes=shapes, graph_records=batch.records(),
-                    config={"workload": f"64 uniform clouds, {ntot} points, m={c['m']}, max_pts={c['max_pts']}, mixed, "
-                                        "autotuned launch shapes, one HIP graph launch", "config": "4b"})
-    elif name == "5":
+                    config={"workload": f"{name}: 64 uniform clouds, {ntot} points, m={c['m']}, max_pts={c['max_pts']}, "
+                                        "mixed, autotuned launch shapes, one HIP graph launch"})
+    elif name == "energy-1m":
         bf = bwd_fwd(steps, warmup, 1048576)
         line.update(metric="laplace energy t_bwd / t_fwd", value=bf["ratio"], unit="ratio", higher_is_better=False,
-                    ms_per_step=bf["fwd_ms"] + bf["bwd_ms"], dtype="f32", config={"workload": bf["workload"], "config": "5"},
+                    ms_per_step=bf["fwd_ms"] + bf["bwd_ms"], dtype="f32", config={"workload": f"{name}: {bf['workload']}"},
                     bwd_fwd=bf)
     else:
-        raise SystemExit(f"unknown config {name}")
+        raise SystemExit(f"unknown workload {name}")
     print(json.dumps(line))
 
 
@@ -313,9 +314,11 @@ def main():
     ap.add_argument("--prec", choices=["both", "mixed", "fp64"], default="both",
                     help="evaluators to time (profiling runs); the JSON line needs both")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run a device copy of this many bytes")
-    ap.add_argument("--no-bwd", action="store_true", help="skip the energy forward/backward (config 5) timing")
+    ap.add_argument("--no-bwd", action="store_true", help="skip the energy forward/backward timing")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
-    ap.add_argument("--config", default="2", help="2 (headline, default), or a secondary workload: 1, 3, 4a, 4b, 5")
+    ap.add_argument("--workload", default="adaptive-grad-2m", choices=["adaptive-grad-2m"] + sorted(
+        k for k in ("uniform-poten-64k", "ellipsoid-stokes-1m", "densities-32", "clouds-64", "energy-1m")),
+                    help="adaptive-grad-2m (headline, default) or a secondary workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the single-GPU MPI_COMM_SELF share); --gpus {args.gpus} is not "
@@ -331,10 +334,10 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
-    if args.config != "2":
-        bench_config(args.config, args.steps, args.warmup, not args.no_check)
+    if args.workload != "adaptive-grad-2m":
+        bench_workload(args.workload, args.steps, args.warmup, not args.no_check)
         return
-    c = W.config2(args.n)
+    c = W.adaptive_grad_2m(args.n)
     n = len(c["src"])
     log(f"workload: {c['kernel']} N={n} m={c['m']} max_pts={c['max_pts']}")
     t0 = time.perf_counter()

@@ -1,8 +1,8 @@
-"""Seeded point distributions of the benchmark configurations.
+"""Seeded point distributions of the benchmark workloads.
 
 The shapes follow the reference's example generators (RandUnif, RandSphr, RandElps, RandGaus in
 examples/include/utils.txx:286-372) but use NumPy PCG64 streams so that every run, test and
-benchmark sees the same points.  Seeds are 20261201 + configuration number.
+benchmark sees the same points.  Each workload has its own stream, seed 20261201 + k.
 """
 from __future__ import annotations
 
@@ -11,8 +11,8 @@ import numpy as np
 SEED0 = 20261201
 
 
-def rng_for(config: int) -> np.random.Generator:
-    return np.random.Generator(np.random.PCG64(SEED0 + config))
+def rng_for(k: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(SEED0 + k))
 
 
 def uniform(rng, n):
@@ -52,7 +52,7 @@ def ellipsoid_surface(rng, n, axes=(0.45, 0.3, 0.15), center=(0.5, 0.5, 0.5)):
     return out[:n]
 
 
-def config1(n=65536):
+def uniform_poten_64k(n=65536):
     """laplace_poten FP64, N=65,536 uniform in [0,1)^3, densities U[-1,1], m=6, max_pts=128."""
     rng = rng_for(1)
     x = uniform(rng, n)
@@ -60,7 +60,7 @@ def config1(n=65536):
     return dict(kernel="laplace_poten", m=6, max_pts=128, src=x, trg=x, den=q)
 
 
-def config2(n=2097152):
+def adaptive_grad_2m(n=2097152):
     """laplace_grad, N=2^21 adaptive: 50% sphere |x-0.5|=0.4, 30% Gaussian (sigma 0.02 at
     (0.3,0.3,0.7)), 20% uniform; densities N(0,1); m=10, max_pts=256."""
     rng = rng_for(2)
@@ -72,7 +72,7 @@ def config2(n=2097152):
     return dict(kernel="laplace_grad", m=10, max_pts=256, src=x, trg=x, den=q)
 
 
-def config3(n=1048576):
+def ellipsoid_stokes_1m(n=1048576):
     """stokes_vel, N=2^20 on the ellipsoid (0.45,0.3,0.15) centred at 0.5; forces N(0,1); m=8, max_pts=200."""
     rng = rng_for(3)
     x = ellipsoid_surface(rng, n)
@@ -80,7 +80,7 @@ def config3(n=1048576):
     return dict(kernel="stokes_vel", m=8, max_pts=200, src=x, trg=x, den=f)
 
 
-def config4a(n=1048576, n_dens=32):
+def uniform_poten_1m_32_densities(n=1048576, n_dens=32):
     """laplace_poten on one uniform tree of 2^20 points with 32 density vectors; m=8, max_pts=128."""
     rng = rng_for(4)
     x = uniform(rng, n)
@@ -88,7 +88,7 @@ def config4a(n=1048576, n_dens=32):
     return dict(kernel="laplace_poten", m=8, max_pts=128, src=x, trg=x, den=q)
 
 
-def config4b(n_clouds=64):
+def uniform_clouds_64(n_clouds=64):
     """64 independent uniform clouds with sizes floor(2^U(12,18)); m=8, max_pts=128."""
     rng = rng_for(5)
     sizes = np.floor(2.0 ** rng.uniform(12, 18, n_clouds)).astype(np.int64)
@@ -96,7 +96,7 @@ def config4b(n_clouds=64):
     return dict(kernel="laplace_poten", m=8, max_pts=128, clouds=clouds)
 
 
-def config5(n=1048576):
+def uniform_energy_1m(n=1048576):
     """Energy E = 0.5 sum q phi for laplace_poten, N=2^20 uniform, q ~ N(0,1), m=10, max_pts=256."""
     rng = rng_for(6)
     x = uniform(rng, n)
@@ -104,4 +104,11 @@ def config5(n=1048576):
     return dict(kernel="laplace_poten", m=10, max_pts=256, src=x, trg=x, den=q)
 
 
-CONFIGS = {1: config1, 2: config2, 3: config3, "4a": config4a, "4b": config4b, 5: config5}
+WORKLOADS = {
+    "uniform-poten-64k": uniform_poten_64k,
+    "adaptive-grad-2m": adaptive_grad_2m,
+    "ellipsoid-stokes-1m": ellipsoid_stokes_1m,
+    "densities-32": uniform_poten_1m_32_densities,
+    "clouds-64": uniform_clouds_64,
+    "energy-1m": uniform_energy_1m,
+}
