@@ -554,8 +554,13 @@ __global__ __launch_bounds__(256, (sizeof(Real) == 4 ? 8 : 4)) void k_gemm_rows(
   for (int k0 = kb0; k0 < kb1; k0 += BK) {
 #pragma unroll
     for (int q = 0; q < 8; q++) As[ar * LD + ak + q] = ra_[q];
+    // B transposed into LDS with its 4-wide k groups XOR-swizzled by n/8 (the 8 n-blocks of a
+    // store hit different banks; a 4-vector read stays contiguous and aligned)
 #pragma unroll
-    for (int q = 0; q < 8; q++) Bt[(bn + q) * LD + bk] = rb_[q];
+    for (int q = 0; q < 8; q++) {
+      const int n = bn + q;
+      Bt[n * LD + 4 * ((bk >> 2) ^ ((n >> 3) & 7)) + (bk & 3)] = rb_[q];
+    }
     __syncthreads();
     if (k0 + BK < kb1) load(k0 + BK);  // in flight during the MFMAs of this step
 #pragma unroll
@@ -564,7 +569,8 @@ __global__ __launch_bounds__(256, (sizeof(Real) == 4 ? 8 : 4)) void k_gemm_rows(
 #pragma unroll
       for (int t = 0; t < 2; t++) {
         av[t] = *reinterpret_cast<const V4*>(&As[(wm * 32 + t * 16 + (l & 15)) * LD + 16 * kk + 4 * (l >> 4)]);
-        bv[t] = *reinterpret_cast<const V4*>(&Bt[(wn * 32 + t * 16 + (l & 15)) * LD + 16 * kk + 4 * (l >> 4)]);
+        const int n = wn * 32 + t * 16 + (l & 15);
+        bv[t] = *reinterpret_cast<const V4*>(&Bt[n * LD + 4 * ((4 * kk + (l >> 4)) ^ ((n >> 3) & 7))]);
       }
 #pragma unroll
       for (int i = 0; i < 2; i++)
