@@ -123,6 +123,7 @@ class DevicePlan final : public DevicePlanBase {
   DBuf<Real> eq_, dn_, chk_;
   // operators
   DBuf<Real> uc2ue0_, uc2ue1_, dc2de0_, dc2de1_, u2u_, d2d_, V_, unit_surf_, ctmp_;
+  DBuf<uint16_t> Vb_;  // transfer blocks as bf16 parts (FP32 scalar kernels)
   DBuf<Real> gemm_work_;  // split-K partial tiles of the small coarse-level GEMMs
   int64_t gemm_work_elems_ = 0;
   int nvb_ = 1;  // density vectors per evaluation pass (buffers hold this many)
@@ -272,6 +273,45 @@ class DevicePlan final : public DevicePlanBase {
               }
       }
       V_.upload(Vp, &total);
+      if (std::is_same<Real, float>::value && E0 == 1 && E1 == 1) {
+        // the same blocks as three bf16 parts for the matrix-core kernel, offsets taken in
+        // pairs in the order of v1_offsets: [pair j][part p][f][lane][i] = part p of
+        // B_{2j + lane/32}[8 ((lane/16) % 2) + i][lane % 16]; each part rounds the remainder of
+        // the FP64 value to nearest
+        auto bf16_rn = [](float x) {
+          uint32_t u;
+          std::memcpy(&u, &x, 4);
+          u += 0x7FFFu + ((u >> 16) & 1u);
+          return (uint16_t)(u >> 16);
+        };
+        auto bf16_val = [](uint16_t h) {
+          const uint32_t u = (uint32_t)h << 16;
+          float x;
+          std::memcpy(&x, &u, 4);
+          return (double)x;
+        };
+        const int n_pair = (int)O_.v1_offsets.size() / 2;
+        std::vector<uint16_t> Vb((size_t)n_pair * 3 * M_dim_ * 64 * 8);
+        #pragma omp parallel for schedule(static)
+        for (int64_t jf = 0; jf < (int64_t)n_pair * M_dim_; jf++) {
+          const int j = (int)(jf / M_dim_), f = (int)(jf % M_dim_);
+          for (int l = 0; l < 64; l++) {
+            const int R = O_.v1_offsets[2 * j + (l >> 5)];
+            const std::complex<double>* Mb = &O_.v1[((size_t)R * M_dim_ + f) * NR * NS];
+            for (int i = 0; i < 8; i++) {
+              const int k = 8 * ((l >> 4) & 1) + i, n = l & 15;
+              const std::complex<double> v = Mb[(size_t)(n >> 1) * NS + (k >> 1)];
+              double r = ((k & 1) == 0) ? ((n & 1) == 0 ? v.real() : v.imag()) : ((n & 1) == 0 ? -v.imag() : v.real());
+              for (int p = 0; p < 3; p++) {
+                const uint16_t h = bf16_rn((float)r);
+                Vb[((((size_t)j * 3 + p) * M_dim_ + f) * 64 + l) * 8 + i] = h;
+                r -= bf16_val(h);
+              }
+            }
+          }
+        }
+        Vb_.upload(Vb, &total);
+      }
     }
     unit_surf_.upload(conv(O_.unit_surf), &total);
     lattice_.upload(O_.lattice, &total);
@@ -560,7 +600,7 @@ class DevicePlan final : public DevicePlanBase {
       if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 1], st_));
       if (nv == 1) mark(6);
       K::hadamard_tiled(st_, n_vtiles_, M_dim_, e0_, e1_, n_off_, vt_t0_.p, vt_u0_.p, vt_usrc_.p, vt_uidx_.p, vt_gmask_.p,
-                        v_offsets_.p, V_.p, X_.p, Y_.p, tuning.hadamard_tpb);
+                        v_offsets_.p, V_.p, X_.p, Y_.p, tuning.hadamard_tpb, Vb_.p);
       if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 2], st_));
       if (nv == 1) mark(7);
       K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p + v * dn_vs_, K1_, e1_, m_, lattice_.p, lattice_inv_.p, ns_,

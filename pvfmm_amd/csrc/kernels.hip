@@ -1572,6 +1572,194 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_stream(int M_dim, int n_til
   }
 }
 
+// FP32 frequency-domain M2L for the scalar kernels on the bf16 matrix cores, with FP32
+// accuracy: every operand x is split exactly into three bf16 parts x = x0 + x1 + x2 (x0 the
+// top 8 significant bits, x1 the next 8, x2 the rest), each product of parts is exact in FP32,
+// and the six products above 2^-24 relative are summed in FP32:
+//   x.t ~ x0 t0 + (x0 t1 + x1 t0) + (x0 t2 + x1 t1 + x2 t0).
+// One v_mfma_f32_16x16x32_bf16 covers two parent offsets (k = 16 a + real index, a the offset
+// of the pair), so an offset pair of a 16-target group costs 6 MFMAs of K = 32 instead of the 8
+// FP32 MFMAs of K = 4 per offset.  A block owns one frequency and a run of consecutive tiles;
+// wave w computes target group w (targets 16w..16w+15 of a tile) over the 13 offset pairs, with
+// the transfer-block parts of its frequency held in registers for the run.  The partners'
+// spectra are staged in LDS as their three bf16 parts (split while staging), double-buffered
+// over tiles as in k_hadamard_stream.  Pairs at which none of the group's targets has a partner
+// are skipped.  Two blocks run per CU.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// the three bf16 parts of an FP32 value, by truncation (each step exact), as the high halves
+// of three words
+__device__ __forceinline__ void split3(float x, uint32_t& h0, uint32_t& h1, uint32_t& h2) {
+  const uint32_t u0 = __float_as_uint(x);
+  h0 = u0 & 0xFFFF0000u;
+  const float r1 = x - __uint_as_float(h0);
+  const uint32_t u1 = __float_as_uint(r1);
+  h1 = u1 & 0xFFFF0000u;
+  const float r2 = r1 - __uint_as_float(h1);
+  h2 = __float_as_uint(r2) & 0xFFFF0000u;  // at most 8 significant bits: exact
+}
+// two bf16 (the high halves of a and b, low halves zero) packed into one word, a in the low half
+__device__ __forceinline__ uint32_t pack_hi(uint32_t a, uint32_t b) { return (a >> 16) | b; }
+
+template <int UMAX>
+__global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tiles, int tiles_per_block, int n_chunks,
+                                                          const int32_t* __restrict__ tile_t0,
+                                                          const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
+                                                          const uint16_t* __restrict__ uidx, const uint32_t* __restrict__ gmask,
+                                                          const uint16_t* __restrict__ Vb, const float* __restrict__ X,
+                                                          float* __restrict__ Y) {
+  constexpr int NOFF = 26, NPAIR = NOFF / 2;
+  constexpr int RW = 16;                  // reals per staged entry and frequency
+  constexpr int EST = 3 * RW / 2 + 4;     // words per staged entry: 3 parts x 16 bf16, padded
+  constexpr int XBUF = (UMAX + 2) * EST;  // entry UMAX stays zero, UMAX + 1 takes unused lanes
+  constexpr int UW = 64 * NOFF / 4;       // partner table as 8-byte words
+  constexpr int UWP = (UW + 255) / 256;
+  constexpr int EPP = 256 / (RW / 4);     // staged entries per pass (a thread stages 4 reals)
+  constexpr int NPASS = (UMAX + EPP - 1) / EPP;
+  static_assert(NPASS < NPAIR, "the staging reads fit in the offset pairs");
+  __shared__ __align__(16) uint32_t Xl[2][XBUF];
+  __shared__ __align__(16) uint16_t ul[2][256 * UWP * 4];
+  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
+  const int f = (local / n_chunks) * 8 + xcd, chunk = local % n_chunks;
+  const int k0 = chunk * tiles_per_block, k1 = min(n_tiles, k0 + tiles_per_block);
+  if (f >= M_dim || k0 >= k1) return;  // whole block
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int row = l & 15, kg = l >> 4;
+  const int pa = kg >> 1;                   // the lane's offset within a pair (k = 16 pa + ...)
+  const int lane_off = 4 * (kg & 1);        // word offset of its 8 reals within a part
+  const int sr = tid & 3, su = tid >> 2;    // staging: reals 4sr..4sr+3 of entries su + EPP * i
+
+  // transfer-block parts of frequency f: bq[j][p] holds, for lane (n = l%16, kg), part p of
+  // B_{2j + kg/2}[8 (kg%2) + i][n], i < 8
+  bf16x8 bq[NPAIR][3];
+#pragma unroll
+  for (int j = 0; j < NPAIR; j++)
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+      bq[j][p] = *reinterpret_cast<const bf16x8*>(Vb + ((((int64_t)j * 3 + p) * M_dim + f) * 64 + l) * 8);
+  for (int e = tid; e < EST; e += 256) {
+    Xl[0][UMAX * EST + e] = 0u;
+    Xl[1][UMAX * EST + e] = 0u;
+  }
+
+  int src[NPASS];
+  floatx4 val[NPASS];
+  uint2 uw[UWP];
+  auto u0_of = [&](int k) __attribute__((always_inline)) { return tile_u0[min(k, k1 - 1)]; };
+  auto nu_of = [&](int k) __attribute__((always_inline)) {
+    const int kk = min(k, k1 - 1);
+    const int n = tile_u0[kk + 1] - tile_u0[kk];
+    return k < k1 ? n : 0;
+  };
+  auto load_src = [&](int i, int u0, int nu) __attribute__((always_inline)) {
+    src[i] = usrc[u0 + min(su + EPP * i, max(nu - 1, 0))];
+  };
+  auto load_val = [&](int i, int nu) __attribute__((always_inline)) {
+    const bool ok = su + EPP * i < nu;
+    const int sl = ok ? src[i] : 0;  // a select on the input keeps it branch-free
+    val[i] = *reinterpret_cast<const floatx4*>(X + ((int64_t)sl * M_dim + f) * RW + 4 * sr);
+  };
+  auto load_uw = [&](int k) __attribute__((always_inline)) {
+    const uint2* ug = reinterpret_cast<const uint2*>(uidx + (int64_t)min(k, k1 - 1) * 64 * NOFF);
+#pragma unroll
+    for (int i = 0; i < UWP; i++) uw[i] = ug[min(tid + 256 * i, UW - 1)];
+  };
+  // entry layout (words): part p at 8p, reals 2q, 2q+1 in word 8p + q
+  auto store_lds = [&](int buf, int nu) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < NPASS; i++) {
+      const int u = su + EPP * i;
+      uint32_t h[3][4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) split3(val[i][q], h[0][q], h[1][q], h[2][q]);
+      uint32_t* d = &Xl[buf][(u < nu ? u : UMAX + 1) * EST + 2 * sr];  // others: trash entry
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+        *reinterpret_cast<uint2*>(d + 8 * p) = uint2{pack_hi(h[p][0], h[p][1]), pack_hi(h[p][2], h[p][3])};
+    }
+#pragma unroll
+    for (int i = 0; i < UWP; i++) reinterpret_cast<uint2*>(ul[buf])[tid + 256 * i] = uw[i];
+  };
+  {
+    const int nu0 = nu_of(k0), u00 = u0_of(k0);
+#pragma unroll
+    for (int i = 0; i < NPASS; i++) load_src(i, u00, nu0);
+#pragma unroll
+    for (int i = 0; i < NPASS; i++) load_val(i, nu0);
+    load_uw(k0);
+    const int nu01 = nu_of(k0 + 1), u01 = u0_of(k0 + 1);
+#pragma unroll
+    for (int i = 0; i < NPASS; i++) load_src(i, u01, nu01);
+    store_lds(0, nu0);
+  }
+#pragma unroll
+  for (int j = 0; j < NPAIR; j++)
+#pragma unroll
+    for (int p = 0; p < 3; p++) asm volatile("" : "+v"(bq[j][p]));
+  __syncthreads();
+
+  for (int k = k0; k < k1; k++) {
+    const int buf = (k - k0) & 1;
+    const int nu1 = nu_of(k + 1), nu2 = nu_of(k + 2), u02 = u0_of(k + 2);
+    // offsets at which a target of the wave's group has a partner
+    const uint32_t wm = __builtin_amdgcn_readfirstlane(gmask[4 * k + w]);
+    const uint32_t* xb = Xl[buf];
+    const uint16_t* ub = ul[buf];
+    // staged position of the partner of target (w, row) at offset 2j + pa
+    auto pidx = [&](int j) __attribute__((always_inline)) { return (int)ub[((2 * j + pa) * 16 + row) * 4 + w]; };
+    auto aop = [&](int pos, int p) __attribute__((always_inline)) {
+      return *reinterpret_cast<const bf16x8*>(&xb[pos * EST + 8 * p + lane_off]);
+    };
+    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    bf16x8 an[3];
+    int pn;
+    {
+      const int p0 = pidx(0);
+#pragma unroll
+      for (int p = 0; p < 3; p++) an[p] = aop(p0, p);
+      pn = pidx(1);
+    }
+#pragma unroll
+    for (int j = 0; j < NPAIR; j++) {
+      bf16x8 ac[3];
+#pragma unroll
+      for (int p = 0; p < 3; p++) ac[p] = an[p];
+      if (j + 1 < NPAIR) {
+#pragma unroll
+        for (int p = 0; p < 3; p++) an[p] = aop(pn, p);
+      }
+      if (j + 2 < NPAIR) pn = pidx(j + 2);
+      if (j < NPASS) {
+        load_val(j, nu1);       // tile k+1, from the slots read during tile k-1
+        load_src(j, u02, nu2);  // tile k+2
+      }
+      if (j == NPASS) load_uw(k + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      if ((wm >> (2 * j)) & 3u) {
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[2], bq[j][0], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[1], bq[j][0], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[1], bq[j][1], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[0], bq[j][1], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[0], bq[j][2], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[0], bq[j][0], acc0, 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    store_lds(buf ^ 1, nu1);  // tile k+1 (the other buffer is free since the last barrier)
+    {
+      const int t0 = tile_t0[k], nt = tile_t0[k + 1] - t0;
+      const floatx4 sum = acc0 + acc1;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int tg = 16 * w + 4 * kg + r;
+        if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * RW + row] = sum[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ----------------------------------------------------------------------------- launchers
 static inline int grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
@@ -1800,14 +1988,23 @@ int DeviceKernels<Real>::hadamard_union_max(int e0) {
 template <typename Real>
 void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
                                          const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx,
-                                         const uint32_t* gmask, const int32_t* offsets, const Real* Vp, const Real* X, Real* Y, int tiles_per_block) {
+                                         const uint32_t* gmask, const int32_t* offsets, const Real* Vp, const Real* X, Real* Y, int tiles_per_block,
+                                         const uint16_t* Vb) {
   if (n_tiles == 0) return;
   if (n_off != 26) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // the 26 V1 parent offsets
   constexpr int FB = sizeof(Real) == 4 ? 4 : 2;
   const int n_fg = (M_dim + FB - 1) / FB;
   const int64_t nb = (int64_t)n_tiles * ((n_fg + 7) / 8) * 8;
   if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-  if (e0 == 1 && e1 == 1 && std::is_same<Real, float>::value) {
+  if (e0 == 1 && e1 == 1 && std::is_same<Real, float>::value && Vb) {
+    // runs of tiles_per_block tiles per block, one frequency per block
+    const int TPB = std::max(1, tiles_per_block);
+    const int n_chunks = (n_tiles + TPB - 1) / TPB;
+    const int64_t nbs = (int64_t)n_chunks * ((M_dim + 7) / 8) * 8;
+    if (nbs >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+    hipLaunchKernelGGL(k_hadamard_split<250>, dim3((unsigned)nbs), dim3(256), 0, s, M_dim, n_tiles, TPB, n_chunks, tile_t0,
+                       tile_u0, usrc, uidx, gmask, Vb, reinterpret_cast<const float*>(X), reinterpret_cast<float*>(Y));
+  } else if (e0 == 1 && e1 == 1 && std::is_same<Real, float>::value) {
     // runs of tiles_per_block tiles per block: the transfer blocks are read once per run, and
     // the grid (n_fg * runs blocks) stays many times the 2 resident blocks per CU
     const int TPB = std::max(1, tiles_per_block);

@@ -70,12 +70,15 @@ struct DeviceKernels {
   // tile target 16 * g + r at [k][o][r][g], the partner's position in that list, or
   // hadamard_union_max(e0) (the zero entry) for none; bit o of gmask[k][g] is set when a
   // target of group g of tile k has a partner at offset o.  tiles_per_block: length of the
-  // run of tiles one block of the FP32 scalar kernel handles.
+  // run of tiles one block of the FP32 scalar kernel handles.  Vb (FP32 scalar kernels only,
+  // else null): the transfer blocks as three bf16 parts, layout [pair j][part p][f][lane 64][8],
+  // lane l holding part p of B_{2j + l/32}[8 ((l/16) % 2) + i][l % 16], i < 8, for the offsets
+  // in the order of uidx; when given, the bf16 matrix-core kernel runs instead of the FP32 one.
   static int hadamard_union_max(int e0);
   static void hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
                              const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx, const uint32_t* gmask,
                              const int32_t* offsets,
-                             const Real* Vp, const Real* X, Real* Y, int tiles_per_block = 16);
+                             const Real* Vp, const Real* X, Real* Y, int tiles_per_block = 16, const uint16_t* Vb = nullptr);
   // inverse DFT, pick the check-surface points, dn[child] += scal * value
   static void ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn,
                         int K1, int e1, int m, const int32_t* lattice, const int32_t* lat_inv, int ns,
