@@ -47,6 +47,60 @@ struct DBuf {
   }
 };
 
+// Order of a V-list tile's staged partners for the bf16 kernel, whose 16 lanes reading one
+// staged entry each (the 16 targets of a group at one offset) are served together: entry
+// position p lands in LDS bank group 7p mod 16, so the positions of the partners a group reads
+// at one offset should differ mod 16.  Greedy colouring by residue mod 16, most-read entries
+// first, residue classes filled to their share of positions; pos[] (union position of each
+// source slot) and the tile's usrc range are permuted in place.
+static void color_union(int k, int n_off, const std::vector<int32_t>& t0, const std::vector<int32_t>& u0,
+                        std::vector<int32_t>& usrc, const std::vector<int32_t>& partner, std::vector<int32_t>& pos) {
+  constexpr int G = 16;
+  const int nu = u0[k + 1] - u0[k];
+  if (nu <= 1) return;
+  std::vector<std::vector<int>> grp;         // local indices read together
+  std::vector<std::vector<int>> memb(nu);    // groups of each local index
+  for (int g = 0; g < 4; g++)
+    for (int o = 0; o < n_off; o++) {
+      std::vector<int> m;
+      for (int r = 0; r < 16; r++) {
+        const int t = t0[k] + 16 * g + r;
+        if (t >= t0[k + 1]) break;
+        const int32_t sl = partner[(size_t)t * n_off + o];
+        if (sl >= 0) m.push_back(pos[sl]);
+      }
+      std::sort(m.begin(), m.end());
+      m.erase(std::unique(m.begin(), m.end()), m.end());
+      if (m.size() < 2) continue;
+      for (int li : m) memb[li].push_back((int)grp.size());
+      grp.push_back(std::move(m));
+    }
+  std::vector<int> order(nu);
+  for (int i = 0; i < nu; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return memb[a].size() > memb[b].size(); });
+  int cap[G], next[G];
+  for (int c = 0; c < G; c++) { cap[c] = (nu - c + G - 1) / G; next[c] = c; }
+  std::vector<int> col(nu, -1);
+  for (int li : order) {
+    int bad[G] = {0};
+    for (int gi : memb[li])
+      for (int lj : grp[gi])
+        if (col[lj] >= 0) bad[col[lj]]++;
+    int best = -1;
+    for (int c = 0; c < G; c++)
+      if (cap[c] > 0 && (best < 0 || bad[c] < bad[best])) best = c;
+    col[li] = best;
+    cap[best]--;
+  }
+  std::vector<int32_t> old(usrc.begin() + u0[k], usrc.begin() + u0[k + 1]);
+  for (int li = 0; li < nu; li++) {
+    const int p = next[col[li]];
+    next[col[li]] += G;
+    usrc[u0[k] + p] = old[li];
+    pos[old[li]] = p;
+  }
+}
+
 template <typename Real>
 class DevicePlan final : public DevicePlanBase {
  public:
@@ -258,9 +312,11 @@ class DevicePlan final : public DevicePlanBase {
       //   B[2s+1][2t+1] = Re M[t][s],
       // stored per MFMA lane: [R][f][kb][nb][lane][q] = B[16kb + 4(lane/16) + q][16nb + lane%16].
       const int E0 = e0_, E1 = e1_, NS = 8 * E0, NR = 8 * E1;
-      std::vector<Real> Vp((size_t)27 * M_dim_ * E0 * E1 * 256, (Real)0);
+      // the FP32 scalar kernels use the bf16 parts below instead
+      const bool bf16_parts = std::is_same<Real, float>::value && E0 == 1 && E1 == 1;
+      std::vector<Real> Vp(bf16_parts ? 0 : (size_t)27 * M_dim_ * E0 * E1 * 256, (Real)0);
       #pragma omp parallel for schedule(static)
-      for (int64_t rf = 0; rf < (int64_t)27 * M_dim_; rf++) {
+      for (int64_t rf = 0; rf < (bf16_parts ? 0 : (int64_t)27 * M_dim_); rf++) {
         const std::complex<double>* Mb = &O_.v1[(size_t)rf * NR * NS];  // [t][s]
         for (int kb = 0; kb < E0; kb++)
           for (int nb = 0; nb < E1; nb++)
@@ -273,7 +329,7 @@ class DevicePlan final : public DevicePlanBase {
               }
       }
       V_.upload(Vp, &total);
-      if (std::is_same<Real, float>::value && E0 == 1 && E1 == 1) {
+      if (bf16_parts) {
         // the same blocks as three bf16 parts for the matrix-core kernel, offsets taken in
         // pairs in the order of v1_offsets: [pair j][part p][f][lane][i] = part p of
         // B_{2j + lane/32}[8 ((lane/16) % 2) + i][lane % 16]; each part rounds the remainder of
@@ -479,17 +535,25 @@ class DevicePlan final : public DevicePlanBase {
       }
       if (n_vtrg_ > t0.back()) close_tile(n_vtrg_);
       n_vtiles_ = (int)t0.size() - 1;
-      // a missing partner points at the staging area's zero entry (index umax)
-      uidx.assign((size_t)n_vtiles_ * 64 * n_off_, (uint16_t)umax);
+      // a missing partner points at the staging area's zero entry (index umax).  The FP32
+      // scalar kernel reads its table as [k][group][row][o % 2][o / 2 (padded to 16)], the
+      // others as [k][o][row][group].
+      const bool pair_layout = std::is_same<Real, float>::value && e0_ == 1 && e1_ == 1;
+      const size_t per_tile = pair_layout ? (size_t)4 * 16 * 2 * 16 : (size_t)64 * n_off_;
+      if (pair_layout && n_off_ > 32) throw Error(kErrInternal, "V-list offsets exceed the pair table");
+      uidx.assign((size_t)n_vtiles_ * per_tile, (uint16_t)umax);
       std::vector<uint32_t> gmask((size_t)n_vtiles_ * 4, 0u);  // offsets present per 16-target group
       for (int k = 0; k < n_vtiles_; k++) {
         for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = u - u0[k];
+        if (pair_layout) color_union(k, n_off_, t0, u0, usrc, v_partner, pos);
         for (int t = t0[k]; t < t0[k + 1]; t++)
           for (int o = 0; o < n_off_; o++) {
             const int32_t sl = v_partner[(size_t)t * n_off_ + o];
             const int tt = t - t0[k];  // target tt = 16 * group + row of the MFMA tiles
             if (sl >= 0) {
-              uidx[(((size_t)k * n_off_ + o) * 16 + (tt & 15)) * 4 + (tt >> 4)] = (uint16_t)pos[sl];
+              const size_t at = pair_layout ? (size_t)k * per_tile + ((size_t)tt * 2 + (o & 1)) * 16 + (o >> 1)
+                                            : (((size_t)k * n_off_ + o) * 16 + (tt & 15)) * 4 + (tt >> 4);
+              uidx[at] = (uint16_t)pos[sl];
               gmask[(size_t)k * 4 + (tt >> 4)] |= 1u << o;
             }
           }

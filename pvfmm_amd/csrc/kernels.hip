@@ -1378,200 +1378,6 @@ __global__ __launch_bounds__(256, (hadamard_blocks_per_cu<Real, E0, FB, UMAX>())
       }
 }
 
-// Streaming FP32 frequency-domain M2L for the scalar kernels.  A block owns FB = 2
-// consecutive frequencies and a run of consecutive tiles.  Each of its 4 waves owns one
-// frequency and two of a tile's four 16-target groups and runs all 26 parent offsets with
-// the 26 transfer blocks of its frequency held in registers for the whole run.  The
-// partners' spectra are staged in two LDS buffers: while a tile's MFMAs run from one buffer,
-// the next tile's partner table and spectra are read from global memory and written to the
-// other, so a block's global latency overlaps its own MFMA work.  The waves of a block
-// meet at one barrier per tile.  Offsets are taken in pairs, whose four accumulators
-// (2 offsets x 2 groups) are independent MFMA chains.  Two blocks run per CU.
-template <int UMAX>
-__global__ __launch_bounds__(256, 2) void k_hadamard_stream(int M_dim, int n_tiles, int tiles_per_block, int n_chunks,
-                                                           const int32_t* __restrict__ tile_t0,
-                                                           const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
-                                                           const uint16_t* __restrict__ uidx, const uint32_t* __restrict__ gmask,
-                                                           const int32_t* __restrict__ offsets, const float* __restrict__ Vp, const float* __restrict__ X,
-                                                           float* __restrict__ Y) {
-  using MF = Mfma<float>;
-  using V4 = MF::V4;
-  constexpr int NOFF = 26, NPAIR = NOFF / 2;
-  constexpr int FB = 2, RW = 16, EST = FB * RW + 4, V4PU = FB * RW / 4;
-  constexpr int XBUF = (UMAX + 2) * EST;  // entry UMAX stays zero, UMAX + 1 takes unused lanes
-  constexpr int UW = 64 * NOFF / 4;       // partner table as 8-byte words
-  constexpr int UWP = (UW + 255) / 256;
-  constexpr int EPP = 256 / V4PU;         // staged entries per pass
-  constexpr int NPASS = (UMAX + EPP - 1) / EPP;
-  static_assert(256 % V4PU == 0, "a thread stages one 4-vector position");
-  static_assert(NPASS < NPAIR, "the staging reads fit in the offset pairs");
-  // the staging writes are branch-free: the table buffer is padded to a whole number of
-  // passes, and lanes past the union write to a trash entry
-  __shared__ __align__(16) float Xl[2][XBUF];
-  __shared__ __align__(16) uint16_t ul[2][256 * UWP * 4];
-  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
-  const int fg = (local / n_chunks) * 8 + xcd, chunk = local % n_chunks;
-  const int f0 = fg * FB;
-  const int k0 = chunk * tiles_per_block, k1 = min(n_tiles, k0 + tiles_per_block);
-  if (f0 >= M_dim || k0 >= k1) return;  // whole block
-  const int fn = min(FB, M_dim - f0);
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  const int fl = w >> 1, gh = w & 1;  // the wave's frequency and group pair {2gh, 2gh+1}
-  const bool live = fl < fn;          // waves past the last frequency compute, not store
-  const int f = f0 + (live ? fl : 0);
-  const int row = l & 15, kg = l >> 4;
-  const int lane_off = fl * RW + 4 * kg;
-  const int sr = tid % V4PU, su = tid / V4PU;  // staging: 4-vector sr of entries su + EPP * i
-  const bool s_live = sr / (RW / 4) < fn;
-
-  V4 bq[NOFF];
-#pragma unroll
-  for (int o = 0; o < NOFF; o++) bq[o] = *reinterpret_cast<const V4*>(Vp + ((int64_t)offsets[o] * M_dim + f) * 256 + l * 4);
-  for (int e = tid; e < EST; e += 256) {
-    Xl[0][UMAX * EST + e] = 0.f;
-    Xl[1][UMAX * EST + e] = 0.f;
-  }
-
-  // staging of a tile into a buffer, software-pipelined over tiles: while tile k computes,
-  // its offset pairs also issue, one pass each, the spectrum reads of tile k+1 and the union
-  // slot reads of tile k+2, so that no wave spends a phase issuing memory instructions;
-  // tile k+1 is written to the other buffer after tile k's MFMAs.  The reads are
-  // branch-free: slot reads past a union are clamped (usrc has one readable slot past its
-  // end) and their values unused, spectrum reads of lanes without an entry go to slot 0
-  // and their LDS writes to a trash entry.  Past the run's end the union sizes are zero.
-  int src[NPASS];
-  V4 val[NPASS];
-  uint2 uw[UWP];
-  // union start and size of tile k (size 0 past the run's end), without branches
-  auto u0_of = [&](int k) __attribute__((always_inline)) { return tile_u0[min(k, k1 - 1)]; };
-  auto nu_of = [&](int k) __attribute__((always_inline)) {
-    const int kk = min(k, k1 - 1);
-    const int n = tile_u0[kk + 1] - tile_u0[kk];
-    return k < k1 ? n : 0;
-  };
-  auto load_src = [&](int i, int u0, int nu) __attribute__((always_inline)) {
-    src[i] = usrc[u0 + min(su + EPP * i, max(nu - 1, 0))];
-  };
-  auto load_val = [&](int i, int nu) __attribute__((always_inline)) {
-    const bool ok = su + EPP * i < nu && s_live;
-    const int sl = ok ? src[i] : 0, fo = ok ? f0 : 0;  // selects on the inputs keep it branch-free
-    val[i] = *reinterpret_cast<const V4*>(X + ((int64_t)sl * M_dim + fo) * RW + 4 * sr);
-  };
-  auto load_uw = [&](int k) __attribute__((always_inline)) {
-    const uint2* ug = reinterpret_cast<const uint2*>(uidx + (int64_t)min(k, k1 - 1) * 64 * NOFF);
-#pragma unroll
-    for (int i = 0; i < UWP; i++) uw[i] = ug[min(tid + 256 * i, UW - 1)];
-  };
-  auto store_lds = [&](int buf, int nu) __attribute__((always_inline)) {
-#pragma unroll
-    for (int i = 0; i < NPASS; i++) {
-      const int u = su + EPP * i;
-      *reinterpret_cast<V4*>(&Xl[buf][(u < nu ? u : UMAX + 1) * EST + 4 * sr]) = val[i];  // others: trash entry
-    }
-#pragma unroll
-    for (int i = 0; i < UWP; i++) reinterpret_cast<uint2*>(ul[buf])[tid + 256 * i] = uw[i];
-  };
-  {
-    const int nu0 = nu_of(k0), u00 = u0_of(k0);
-#pragma unroll
-    for (int i = 0; i < NPASS; i++) load_src(i, u00, nu0);
-#pragma unroll
-    for (int i = 0; i < NPASS; i++) load_val(i, nu0);
-    load_uw(k0);
-    const int nu01 = nu_of(k0 + 1), u01 = u0_of(k0 + 1);
-#pragma unroll
-    for (int i = 0; i < NPASS; i++) load_src(i, u01, nu01);
-    store_lds(0, nu0);
-  }
-  // pin the transfer blocks in registers before the loop (otherwise their loads are sunk
-  // into it)
-#pragma unroll
-  for (int o = 0; o < NOFF; o++) asm volatile("" : "+v"(bq[o]));
-  __syncthreads();
-
-  for (int k = k0; k < k1; k++) {
-    const int buf = (k - k0) & 1;
-    const int nu1 = nu_of(k + 1), nu2 = nu_of(k + 2), u02 = u0_of(k + 2);
-    // offsets at which a target of the wave's two groups has a partner (others are skipped)
-    const uint32_t wm = __builtin_amdgcn_readfirstlane(gmask[4 * k + 2 * gh] | gmask[4 * k + 2 * gh + 1]);
-    const float* xb = Xl[buf];
-    const uint16_t* ub = ul[buf];
-    // partner positions of the wave's 2 groups at offset o: one 4-byte LDS read
-    auto pidx = [&](int o) __attribute__((always_inline)) {
-      return *reinterpret_cast<const uint32_t*>(&ub[(o * 16 + row) * 4 + 2 * gh]);
-    };
-    auto aop = [&](uint32_t pp, int g) __attribute__((always_inline)) {
-      return *reinterpret_cast<const V4*>(&xb[(int)((pp >> (16 * g)) & 0xFFFFu) * EST + lane_off]);
-    };
-    V4 acc[2][2];  // [offset of the pair][group]
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int g = 0; g < 2; g++) acc[a][g] = V4{0.f, 0.f, 0.f, 0.f};
-    // A operands of the next pair, read one pair ahead; partner positions two pairs ahead
-    V4 an[2][2];
-    uint32_t pn[2];
-    {
-      const uint32_t p0 = pidx(0), p1 = pidx(1);
-#pragma unroll
-      for (int g = 0; g < 2; g++) {
-        an[0][g] = aop(p0, g);
-        an[1][g] = aop(p1, g);
-      }
-      pn[0] = pidx(2);
-      pn[1] = pidx(3);
-    }
-#pragma unroll
-    for (int j = 0; j < NPAIR; j++) {
-      V4 ac[2][2];
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int g = 0; g < 2; g++) ac[a][g] = an[a][g];
-      if (j + 1 < NPAIR) {
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-          for (int g = 0; g < 2; g++) an[a][g] = aop(pn[a], g);
-      }
-      if (j + 2 < NPAIR) {
-        pn[0] = pidx(2 * j + 4);
-        pn[1] = pidx(2 * j + 5);
-      }
-      if (j < NPASS) {
-        load_val(j, nu1);    // tile k+1, from the slots read during tile k-1
-        load_src(j, u02, nu2);  // tile k+2
-      }
-      if (j == NPASS) load_uw(k + 1);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int a = 0; a < 2; a++) {
-        if ((wm >> (2 * j + a)) & 1u) {
-#pragma unroll
-          for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int g = 0; g < 2; g++) acc[a][g] = MF::mma(ac[a][g][q], bq[2 * j + a][q], acc[a][g]);
-        }
-      }
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    store_lds(buf ^ 1, nu1);  // tile k+1 (the other buffer is free since the last barrier)
-    if (live) {
-      const int t0 = tile_t0[k], nt = tile_t0[k + 1] - t0;
-#pragma unroll
-      for (int g = 0; g < 2; g++) {
-        const V4 sum = acc[0][g] + acc[1][g];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int tg = (2 * gh + g) * 16 + MF::row(l, r);
-          if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * RW + row] = sum[r];
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
 // FP32 frequency-domain M2L for the scalar kernels on the bf16 matrix cores, with FP32
 // accuracy: every operand x is split exactly into three bf16 parts x = x0 + x1 + x2 (x0 the
 // top 8 significant bits, x1 the next 8, x2 the rest), each product of parts is exact in FP32,
@@ -1582,9 +1388,10 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_stream(int M_dim, int n_til
 // FP32 MFMAs of K = 4 per offset.  A block owns one frequency and a run of consecutive tiles;
 // wave w computes target group w (targets 16w..16w+15 of a tile) over the 13 offset pairs, with
 // the transfer-block parts of its frequency held in registers for the run.  The partners'
-// spectra are staged in LDS as their three bf16 parts (split while staging), double-buffered
-// over tiles as in k_hadamard_stream.  Pairs at which none of the group's targets has a partner
-// are skipped.  Two blocks run per CU.
+// spectra are staged in LDS as their three bf16 parts (split while staging) in two buffers:
+// while a tile's MFMAs run from one buffer, the next tile's partner table and spectra are read
+// from global memory and written to the other, one barrier per tile.  Pairs at which none of
+// the group's targets has a partner are skipped.  Two blocks run per CU.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -1613,13 +1420,17 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
   constexpr int RW = 16;                  // reals per staged entry and frequency
   constexpr int EST = 3 * RW / 2 + 4;     // words per staged entry: 3 parts x 16 bf16, padded
   constexpr int XBUF = (UMAX + 2) * EST;  // entry UMAX stays zero, UMAX + 1 takes unused lanes
-  constexpr int UW = 64 * NOFF / 4;       // partner table as 8-byte words
-  constexpr int UWP = (UW + 255) / 256;
+  constexpr int UT = 4 * 16 * 2 * 16;     // partner table of a tile: [group][row][a][pair (padded to 16)]
+  constexpr int UW = UT / 4;              // as 8-byte words
+  constexpr int UWP = UW / 256;
   constexpr int EPP = 256 / (RW / 4);     // staged entries per pass (a thread stages 4 reals)
   constexpr int NPASS = (UMAX + EPP - 1) / EPP;
-  static_assert(NPASS < NPAIR, "the staging reads fit in the offset pairs");
+  constexpr int TMAX = 64;                // longest run of tiles (the launcher clamps to it)
+  static_assert(NPASS < NPAIR && UW % 256 == 0, "staging shape");
   __shared__ __align__(16) uint32_t Xl[2][XBUF];
-  __shared__ __align__(16) uint16_t ul[2][256 * UWP * 4];
+  __shared__ __align__(16) uint16_t ul[2][UT];
+  __shared__ uint32_t m_g[4 * TMAX];  // the run's skip masks, union starts and target starts
+  __shared__ int32_t m_u[TMAX + 1], m_t[TMAX + 1];
   const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
   const int f = (local / n_chunks) * 8 + xcd, chunk = local % n_chunks;
   const int k0 = chunk * tiles_per_block, k1 = min(n_tiles, k0 + tiles_per_block);
@@ -1642,14 +1453,28 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
     Xl[0][UMAX * EST + e] = 0u;
     Xl[1][UMAX * EST + e] = 0u;
   }
+  const int nk = k1 - k0;
+  for (int i = tid; i < 4 * nk; i += 256) m_g[i] = gmask[4 * k0 + i];
+  for (int i = tid; i <= nk; i += 256) {
+    m_u[i] = tile_u0[k0 + i];
+    m_t[i] = tile_t0[k0 + i];
+  }
 
   int src[NPASS];
   floatx4 val[NPASS];
   uint2 uw[UWP];
-  auto u0_of = [&](int k) __attribute__((always_inline)) { return tile_u0[min(k, k1 - 1)]; };
-  auto nu_of = [&](int k) __attribute__((always_inline)) {
+  // union start and size of tile k (size 0 past the run's end), from global memory (prologue)
+  // or from the run's copy in LDS (loop)
+  auto u0_g = [&](int k) __attribute__((always_inline)) { return tile_u0[min(k, k1 - 1)]; };
+  auto nu_g = [&](int k) __attribute__((always_inline)) {
     const int kk = min(k, k1 - 1);
     const int n = tile_u0[kk + 1] - tile_u0[kk];
+    return k < k1 ? n : 0;
+  };
+  auto u0_l = [&](int k) __attribute__((always_inline)) { return m_u[min(k, k1 - 1) - k0]; };
+  auto nu_l = [&](int k) __attribute__((always_inline)) {
+    const int kk = min(k, k1 - 1) - k0;
+    const int n = m_u[kk + 1] - m_u[kk];
     return k < k1 ? n : 0;
   };
   auto load_src = [&](int i, int u0, int nu) __attribute__((always_inline)) {
@@ -1661,9 +1486,9 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
     val[i] = *reinterpret_cast<const floatx4*>(X + ((int64_t)sl * M_dim + f) * RW + 4 * sr);
   };
   auto load_uw = [&](int k) __attribute__((always_inline)) {
-    const uint2* ug = reinterpret_cast<const uint2*>(uidx + (int64_t)min(k, k1 - 1) * 64 * NOFF);
+    const uint2* ug = reinterpret_cast<const uint2*>(uidx + (int64_t)min(k, k1 - 1) * UT);
 #pragma unroll
-    for (int i = 0; i < UWP; i++) uw[i] = ug[min(tid + 256 * i, UW - 1)];
+    for (int i = 0; i < UWP; i++) uw[i] = ug[tid + 256 * i];
   };
   // entry layout (words): part p at 8p, reals 2q, 2q+1 in word 8p + q
   auto store_lds = [&](int buf, int nu) __attribute__((always_inline)) {
@@ -1682,13 +1507,13 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
     for (int i = 0; i < UWP; i++) reinterpret_cast<uint2*>(ul[buf])[tid + 256 * i] = uw[i];
   };
   {
-    const int nu0 = nu_of(k0), u00 = u0_of(k0);
+    const int nu0 = nu_g(k0), u00 = u0_g(k0);
 #pragma unroll
     for (int i = 0; i < NPASS; i++) load_src(i, u00, nu0);
 #pragma unroll
     for (int i = 0; i < NPASS; i++) load_val(i, nu0);
     load_uw(k0);
-    const int nu01 = nu_of(k0 + 1), u01 = u0_of(k0 + 1);
+    const int nu01 = nu_g(k0 + 1), u01 = u0_g(k0 + 1);
 #pragma unroll
     for (int i = 0; i < NPASS; i++) load_src(i, u01, nu01);
     store_lds(0, nu0);
@@ -1701,24 +1526,25 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
 
   for (int k = k0; k < k1; k++) {
     const int buf = (k - k0) & 1;
-    const int nu1 = nu_of(k + 1), nu2 = nu_of(k + 2), u02 = u0_of(k + 2);
+    const int nu1 = nu_l(k + 1), nu2 = nu_l(k + 2), u02 = u0_l(k + 2);
     // offsets at which a target of the wave's group has a partner
-    const uint32_t wm = __builtin_amdgcn_readfirstlane(gmask[4 * k + w]);
+    const uint32_t wm = __builtin_amdgcn_readfirstlane(m_g[4 * (k - k0) + w]);
     const uint32_t* xb = Xl[buf];
-    const uint16_t* ub = ul[buf];
-    // staged position of the partner of target (w, row) at offset 2j + pa
-    auto pidx = [&](int j) __attribute__((always_inline)) { return (int)ub[((2 * j + pa) * 16 + row) * 4 + w]; };
+    // staged positions of the partners of target (w, row) at offsets 2j + pa, j < 13, read at
+    // once so that the operand reads of a pair wait for no other LDS read
+    u32x4 pw[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) pw[h] = *reinterpret_cast<const u32x4*>(&ul[buf][((w * 16 + row) * 2 + pa) * 16 + 8 * h]);
+    auto pidx = [&](int j) __attribute__((always_inline)) { return (int)((pw[j >> 3][(j >> 1) & 3] >> (16 * (j & 1))) & 0xFFFFu); };
     auto aop = [&](int pos, int p) __attribute__((always_inline)) {
       return *reinterpret_cast<const bf16x8*>(&xb[pos * EST + 8 * p + lane_off]);
     };
     floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
     bf16x8 an[3];
-    int pn;
     {
       const int p0 = pidx(0);
 #pragma unroll
       for (int p = 0; p < 3; p++) an[p] = aop(p0, p);
-      pn = pidx(1);
     }
 #pragma unroll
     for (int j = 0; j < NPAIR; j++) {
@@ -1726,10 +1552,10 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
 #pragma unroll
       for (int p = 0; p < 3; p++) ac[p] = an[p];
       if (j + 1 < NPAIR) {
+        const int pn = pidx(j + 1);
 #pragma unroll
         for (int p = 0; p < 3; p++) an[p] = aop(pn, p);
       }
-      if (j + 2 < NPAIR) pn = pidx(j + 2);
       if (j < NPASS) {
         load_val(j, nu1);       // tile k+1, from the slots read during tile k-1
         load_src(j, u02, nu2);  // tile k+2
@@ -1748,7 +1574,7 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
     }
     store_lds(buf ^ 1, nu1);  // tile k+1 (the other buffer is free since the last barrier)
     {
-      const int t0 = tile_t0[k], nt = tile_t0[k + 1] - t0;
+      const int t0 = m_t[k - k0], nt = m_t[k - k0 + 1] - t0;
       const floatx4 sum = acc0 + acc1;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
@@ -1996,34 +1822,23 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
   const int n_fg = (M_dim + FB - 1) / FB;
   const int64_t nb = (int64_t)n_tiles * ((n_fg + 7) / 8) * 8;
   if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-  if (e0 == 1 && e1 == 1 && std::is_same<Real, float>::value && Vb) {
-    // runs of tiles_per_block tiles per block, one frequency per block
-    const int TPB = std::max(1, tiles_per_block);
-    const int n_chunks = (n_tiles + TPB - 1) / TPB;
-    const int64_t nbs = (int64_t)n_chunks * ((M_dim + 7) / 8) * 8;
-    if (nbs >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-    hipLaunchKernelGGL(k_hadamard_split<250>, dim3((unsigned)nbs), dim3(256), 0, s, M_dim, n_tiles, TPB, n_chunks, tile_t0,
-                       tile_u0, usrc, uidx, gmask, Vb, reinterpret_cast<const float*>(X), reinterpret_cast<float*>(Y));
-  } else if (e0 == 1 && e1 == 1 && std::is_same<Real, float>::value) {
-    // runs of tiles_per_block tiles per block: the transfer blocks are read once per run, and
-    // the grid (n_fg * runs blocks) stays many times the 2 resident blocks per CU
-    const int TPB = std::max(1, tiles_per_block);
-    const int n_chunks = (n_tiles + TPB - 1) / TPB;
-    constexpr int FBS = 2;
-    const int n_fgs = (M_dim + FBS - 1) / FBS;
-    const int64_t nbs = (int64_t)n_chunks * ((n_fgs + 7) / 8) * 8;
-    if (nbs >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-    hipLaunchKernelGGL(k_hadamard_stream<250>, dim3((unsigned)nbs), dim3(256), 0, s, M_dim, n_tiles, TPB, n_chunks, tile_t0,
-                       tile_u0, usrc, uidx, gmask, offsets, reinterpret_cast<const float*>(Vp), reinterpret_cast<const float*>(X),
-                       reinterpret_cast<float*>(Y));
-  } else if (e0 == 1 && e1 == 1) {
-    constexpr int NA = 1;
-    constexpr int FBT = 2, UM = sizeof(Real) == 4 ? 250 : 256;
-    const int n_fgt = (M_dim + FBT - 1) / FBT;
-    const int64_t nbt = (int64_t)n_tiles * ((n_fgt + 7) / 8) * 8;
-    if (nbt >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-    hipLaunchKernelGGL((k_hadamard_tiled<Real, 1, 1, FBT, UM, NA>), dim3((unsigned)nbt), dim3(256), 0, s, M_dim, n_tiles,
-                       tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
+  if (e0 == 1 && e1 == 1) {
+    if constexpr (std::is_same<Real, float>::value) {
+      // runs of tiles_per_block (at most 64) tiles per block, one frequency per block
+      if (!Vb) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+      const int TPB = std::min(64, std::max(1, tiles_per_block));
+      const int n_chunks = (n_tiles + TPB - 1) / TPB;
+      const int64_t nbs = (int64_t)n_chunks * ((M_dim + 7) / 8) * 8;
+      if (nbs >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+      hipLaunchKernelGGL(k_hadamard_split<250>, dim3((unsigned)nbs), dim3(256), 0, s, M_dim, n_tiles, TPB, n_chunks, tile_t0,
+                         tile_u0, usrc, uidx, gmask, Vb, X, Y);
+    } else {
+      const int n_fgt = (M_dim + 1) / 2;
+      const int64_t nbt = (int64_t)n_tiles * ((n_fgt + 7) / 8) * 8;
+      if (nbt >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+      hipLaunchKernelGGL((k_hadamard_tiled<Real, 1, 1, 2, 256, 1>), dim3((unsigned)nbt), dim3(256), 0, s, M_dim, n_tiles,
+                         tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
+    }
   } else if (e0 == 3 && e1 == 3) {
     hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, FB, 80, 1>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
                        tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
