@@ -27,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MI355X_FP32_MFMA_TFLOPS = 157.3
+MI355X_BF16_MFMA_TFLOPS = 2500.0  # dense
+BF16_PRODUCTS_PER_FP32 = 6  # three-way bf16 split of both operands, six partial products kept
 MI355X_FP32_VALU_TFLOPS = 78.6  # non-packed FP32 FMA (packed FP32 is not used)
 MI355X_FP64_TFLOPS = 78.6
 MI355X_HBM_GBS = 8000.0
@@ -98,32 +100,39 @@ def read_traffic(kernel: str):
 def roofline(ph: dict, st: dict, m: int, real_bytes: int):
     """Roofline of the dominant kernel of the mixed evaluator from algorithmic counts.
 
-    V-list Hadamard: 512*k0*k1 flop per (V1 pair, frequency) (the reference's own count,
-    include/fmm_pts.txx:3336) against the FP32 MFMA peak.  The kernel stages each tile's
-    partner spectra in LDS once, so it moves far fewer bytes than the target-stationary model
-    (per pair and frequency 16*k0 reals read, per target parent 16*k1 written, the 26 V1
-    matrices read once), which it would otherwise be bound by; that model's GB/s is reported
-    beside it, and `traffic` is the measured HBM bytes per launch.  U-list P2P: 16 flop per
-    pair for laplace_grad, VALU-bound."""
+    V-list Hadamard (scalar kernels): 512*k0*k1 flop per (V1 pair, frequency) (the reference's
+    own count, include/fmm_pts.txx:3336) in FP32, which the kernel computes on the bf16 matrix
+    cores as six bf16 products per FP32 product (each operand split exactly into three bf16
+    parts); its roofline is 6x that count against the dense bf16 MFMA peak, and the FP32
+    figure is reported beside it.  The kernel stages each tile's partner spectra in LDS once,
+    so it moves far fewer bytes than the target-stationary model (per pair and frequency
+    16*k0 reals read, per target parent 16*k1 written, the 26 V1 matrices read once), which it
+    would otherwise be bound by; that model's GB/s is reported beside it, and `traffic` is the
+    measured HBM bytes per launch.  U-list P2P: 16 flop per pair for laplace_grad, VALU-bound."""
     n1 = 2 * m
     M_dim = n1 * n1 * (n1 // 2 + 1)
     dom = max(ph, key=ph.get)
     t = ph[dom] * 1e-3
     if dom == "V-List:Hadamard":
-        name = "k_hadamard_stream<250>"
+        name = "k_hadamard_split<250>"
         bytes_model = (16.0 * M_dim * real_bytes * st["v_pairs"] + 16.0 * M_dim * real_bytes * st["v_trg_slots"]
                        + 26.0 * M_dim * 128 * real_bytes)
         flops = 512.0 * st["v_pairs"] * M_dim
-        achieved = flops / t / 1e12
-        return {"kernel": name + " (V-list frequency-domain M2L)", "bound": "mfma", "achieved": achieved,
-                "peak": MI355X_FP32_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MI355X_FP32_MFMA_TFLOPS,
+        achieved = BF16_PRODUCTS_PER_FP32 * flops / t / 1e12
+        return {"kernel": name + " (V-list frequency-domain M2L, FP32 via split bf16 MFMA)", "bound": "mfma",
+                "achieved": achieved, "peak": MI355X_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / MI355X_BF16_MFMA_TFLOPS,
                 "traffic": read_traffic(name), "ms_per_launch": ph[dom], "algorithmic_flops": flops,
+                "bf16_flops": BF16_PRODUCTS_PER_FP32 * flops,
+                "fp32_equivalent_tflops": flops / t / 1e12,
+                "fp32_equivalent_frac_of_fp32_mfma_peak": flops / t / 1e12 / MI355X_FP32_MFMA_TFLOPS,
                 "target_stationary_model_bytes": bytes_model,
                 "target_stationary_model_gbs": bytes_model / t / 1e9,
-                "work": f"512 flop x {st['v_pairs']} V1 pairs x {M_dim} frequencies; the kernel runs dense "
-                        f"16-target MFMA groups (offsets without partners skipped) from LDS-staged spectra"}
+                "work": f"512 flop x {st['v_pairs']} V1 pairs x {M_dim} frequencies in FP32, each FP32 product "
+                        f"as {BF16_PRODUCTS_PER_FP32} bf16 products; the kernel runs dense 16-target MFMA groups "
+                        f"(offset pairs without partners skipped) from LDS-staged spectra"}
     if dom == "U-List":
-        name = "k_p2p_leaf<float, 1>"
+        name = "k_p2p_leaf<float, 1, 1>"
         flops = 16.0 * st["u_pairs"]
         achieved = flops / t / 1e12
         return {"kernel": name + " (U-list P2P)", "bound": "valu", "achieved": achieved, "peak": MI355X_FP32_VALU_TFLOPS,

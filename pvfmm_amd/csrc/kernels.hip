@@ -1431,8 +1431,11 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
   __shared__ __align__(16) uint16_t ul[2][UT];
   __shared__ uint32_t m_g[4 * TMAX];  // the run's skip masks, union starts and target starts
   __shared__ int32_t m_u[TMAX + 1], m_t[TMAX + 1];
-  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
-  const int f = (local / n_chunks) * 8 + xcd, chunk = local % n_chunks;
+  // blocks b and b + 8 run on one XCD one after the other and take the two frequencies of one
+  // 128-byte line of X and Y (a frequency's 16 reals are 64 bytes) for the same run of tiles,
+  // so the second block's reads hit that XCD's L2
+  const int b = blockIdx.x, xcd = b & 7, local = b >> 3, lp = local >> 1;
+  const int f = ((lp / n_chunks) * 8 + xcd) * 2 + (local & 1), chunk = lp % n_chunks;
   const int k0 = chunk * tiles_per_block, k1 = min(n_tiles, k0 + tiles_per_block);
   if (f >= M_dim || k0 >= k1) return;  // whole block
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
@@ -1828,7 +1831,7 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
       if (!Vb) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
       const int TPB = std::min(64, std::max(1, tiles_per_block));
       const int n_chunks = (n_tiles + TPB - 1) / TPB;
-      const int64_t nbs = (int64_t)n_chunks * ((M_dim + 7) / 8) * 8;
+      const int64_t nbs = (int64_t)n_chunks * ((M_dim + 15) / 16) * 16;
       if (nbs >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
       hipLaunchKernelGGL(k_hadamard_split<250>, dim3((unsigned)nbs), dim3(256), 0, s, M_dim, n_tiles, TPB, n_chunks, tile_t0,
                          tile_u0, usrc, uidx, gmask, Vb, X, Y);
