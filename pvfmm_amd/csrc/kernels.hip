@@ -1,16 +1,16 @@
 // gfx950 kernels of the particle FMM evaluation.
 //
-//  * P2P (U list, S2M/X onto check surfaces, L2T/W from equivalent surfaces): one wave64 per
-//    target box; source tiles of 64 points staged in LDS as SoA; when a box has fewer than 64
-//    targets the wave splits the source loop over 64/next_pow2(n_targets) lane groups and
-//    combines them with a butterfly of __shfl_xor (wavefront reduction).  Per source box the
-//    partial sum is added to an FP64 accumulator (the FP64 merge of the mixed-precision path).
+//  * P2P (U list, S2M/X onto check surfaces, L2T/W from equivalent surfaces): one 256-thread
+//    block per target leaf or node; sources staged in LDS as SoA chunks; threads are
+//    (target, split) pairs so that small leaves still fill the block; partial sums over at
+//    most 64 sources are added into an FP64 accumulator (the FP64 merge of the
+//    mixed-precision path), the splits of a target in a fixed order.
 //  * Row-gathered GEMM for S2M/L2T (check -> equivalent), M2M and L2L (8 children stored
-//    contiguously so a parent row is one 8*K row of the child block).
+//    contiguously so a parent row is one 8*K row of the child block), on MFMA.
 //  * V list: pruned separable DFT of the m^3 surface lattice embedded in the (2m)^3 grid,
-//    processed one x-frequency slice at a time in LDS; a target-stationary complex Hadamard
-//    sum over the 26 parent offsets with the transfer blocks broadcast from the scalar cache;
-//    the inverse DFT evaluates only the check-surface points.
+//    one x-frequency slice at a time in LDS; a tiled frequency-domain Hadamard sum over the 26
+//    parent offsets on the matrix cores (FP32 scalar kernels on the bf16 cores with exact
+//    three-way operand splits); the inverse DFT evaluates only the check-surface points.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>

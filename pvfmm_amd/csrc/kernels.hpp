@@ -66,9 +66,10 @@ struct DeviceKernels {
   // in real form, pre-permuted per lane (layout [R][f][kb][nb][lane 64][4]).  The target slots
   // form tiles of at most 64 consecutive slots [tile_t0[k], tile_t0[k+1])
   // whose distinct partner source slots usrc[tile_u0[k] .. tile_u0[k+1]) (at most
-  // hadamard_union_max(e0)) are staged in LDS; uidx[k][n_off][16][4] holds, for offset o and
-  // tile target 16 * g + r at [k][o][r][g], the partner's position in that list, or
-  // hadamard_union_max(e0) (the zero entry) for none; bit o of gmask[k][g] is set when a
+  // hadamard_union_max(e0)) are staged in LDS; uidx holds, for offset o and tile target
+  // 16 * g + r, the partner's position in that list, or hadamard_union_max(e0) (the zero
+  // entry) for none, at [k][o][r][g] ([k][n_off][16][4]), or for the FP32 scalar kernel at
+  // [k][g][r][o % 2][o / 2] ([k][4][16][2][16]); bit o of gmask[k][g] is set when a
   // target of group g of tile k has a partner at offset o.  tiles_per_block: length of the
   // run of tiles one block of the FP32 scalar kernel handles.  Vb (FP32 scalar kernels only,
   // else null): the transfer blocks as three bf16 parts, layout [pair j][part p][f][lane 64][8],
