@@ -185,8 +185,8 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
                                                   const Real* __restrict__ sden, int64_t n_src, int64_t den_vs,
                                                   double* __restrict__ out, int64_t out_vs, bool accumulate) {
   constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1, NT = 256, SC = VB * K0 >= 8 ? 256 : 512;
-  __shared__ double lc[3 * SC];  // FP64 source coordinates in the target leaf's frame
-  __shared__ Real ld[VB * K0 * SC];
+  __shared__ __align__(16) double lc[3 * SC];  // FP64 source coordinates in the target leaf's frame
+  __shared__ __align__(16) Real ld[VB * K0 * SC];
   __shared__ double red[K1 * NT];
   const int row = blockIdx.x, tid = threadIdx.x;
   const int T = tleaf[row];
@@ -236,26 +236,84 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
       }
       __syncthreads();
       if (active) {
-        // Real partial sums over at most 64 sources, added into the FP64 accumulator
-        for (int s0 = j; s0 < pos; s0 += 64 * nsplit) {
-          Real part[VB][K1];
-#pragma unroll
-          for (int v = 0; v < VB; v++)
-#pragma unroll
-            for (int k = 0; k < K1; k++) part[v][k] = 0;
-          const int s1 = min(pos, s0 + 64 * nsplit);
-          for (int s = s0; s < s1; s += nsplit) {
-            Real f[VB][K0];
+        if constexpr (sizeof(Real) == 4) {
+          // split j takes the contiguous run [jb, je) of the chunk's sources (a multiple of 4
+          // long, so that 4 sources are read with 16-byte LDS reads at immediate offsets); Real
+          // partial sums over at most 64 sources, added into the FP64 accumulator
+          const int L = ((pos + 4 * nsplit - 1) / (4 * nsplit)) * 4;
+          const int jb = min(pos, j * L), je = min(pos, jb + L);
+          for (int s0 = jb; s0 < je; s0 += 64) {
+            Real part[VB][K1];
 #pragma unroll
             for (int v = 0; v < VB; v++)
 #pragma unroll
-              for (int c = 0; c < K0; c++) f[v][c] = ld[(v * K0 + c) * SC + s];
-            interact_v<KIND, VB, Real>((Real)(p0 - lc[s]), (Real)(p1 - lc[SC + s]), (Real)(p2 - lc[2 * SC + s]), f, part);
+              for (int k = 0; k < K1; k++) part[v][k] = 0;
+            const int s1 = min(je, s0 + 64);
+            int s = s0;
+            for (; s + 4 <= s1; s += 4) {
+              double xs[3][4];
+#pragma unroll
+              for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                  const double2 w2 = *reinterpret_cast<const double2*>(&lc[d * SC + s + 2 * h]);
+                  xs[d][2 * h] = w2.x;
+                  xs[d][2 * h + 1] = w2.y;
+                }
+              Real fs[VB][K0][4];
+#pragma unroll
+              for (int v = 0; v < VB; v++)
+#pragma unroll
+                for (int c = 0; c < K0; c++)
+#pragma unroll
+                  for (int u = 0; u < 4; u++) fs[v][c][u] = ld[(v * K0 + c) * SC + s + u];
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                Real f[VB][K0];
+#pragma unroll
+                for (int v = 0; v < VB; v++)
+#pragma unroll
+                  for (int c = 0; c < K0; c++) f[v][c] = fs[v][c][u];
+                interact_v<KIND, VB, Real>((Real)(p0 - xs[0][u]), (Real)(p1 - xs[1][u]), (Real)(p2 - xs[2][u]), f, part);
+              }
+            }
+            for (; s < s1; s++) {
+              Real f[VB][K0];
+#pragma unroll
+              for (int v = 0; v < VB; v++)
+#pragma unroll
+                for (int c = 0; c < K0; c++) f[v][c] = ld[(v * K0 + c) * SC + s];
+              interact_v<KIND, VB, Real>((Real)(p0 - lc[s]), (Real)(p1 - lc[SC + s]), (Real)(p2 - lc[2 * SC + s]), f, part);
+            }
+#pragma unroll
+            for (int v = 0; v < VB; v++)
+#pragma unroll
+              for (int k = 0; k < K1; k++) acc[v][k] += (double)part[v][k];
           }
+        } else {
+          // FP64: split j takes sources j, j + nsplit, ... (the unrolled form above needs more
+          // registers than the occupancy of this kernel allows in FP64); partial sums over at
+          // most 64 sources, added into the accumulator
+          for (int s0 = j; s0 < pos; s0 += 64 * nsplit) {
+            Real part[VB][K1];
 #pragma unroll
-          for (int v = 0; v < VB; v++)
+            for (int v = 0; v < VB; v++)
 #pragma unroll
-            for (int k = 0; k < K1; k++) acc[v][k] += (double)part[v][k];
+              for (int k = 0; k < K1; k++) part[v][k] = 0;
+            const int s1 = min(pos, s0 + 64 * nsplit);
+            for (int s = s0; s < s1; s += nsplit) {
+              Real f[VB][K0];
+#pragma unroll
+              for (int v = 0; v < VB; v++)
+#pragma unroll
+                for (int c = 0; c < K0; c++) f[v][c] = ld[(v * K0 + c) * SC + s];
+              interact_v<KIND, VB, Real>((Real)(p0 - lc[s]), (Real)(p1 - lc[SC + s]), (Real)(p2 - lc[2 * SC + s]), f, part);
+            }
+#pragma unroll
+            for (int v = 0; v < VB; v++)
+#pragma unroll
+              for (int k = 0; k < K1; k++) acc[v][k] += (double)part[v][k];
+          }
         }
       }
     }
