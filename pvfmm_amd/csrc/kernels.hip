@@ -1436,11 +1436,13 @@ __global__ __launch_bounds__(256, (hadamard_blocks_per_cu<Real, E0, FB, UMAX>())
       }
 }
 
-// FP32 frequency-domain M2L for the scalar kernels on the bf16 matrix cores, with FP32
-// accuracy: every operand x is split exactly into three bf16 parts x = x0 + x1 + x2 (x0 the
-// top 8 significant bits, x1 the next 8, x2 the rest), each product of parts is exact in FP32,
-// and the six products above 2^-24 relative are summed in FP32:
-//   x.t ~ x0 t0 + (x0 t1 + x1 t0) + (x0 t2 + x1 t1 + x2 t0).
+// FP32 frequency-domain M2L for the scalar kernels on the bf16 matrix cores: every operand x
+// is split exactly into three bf16 parts x = x0 + x1 + x2 (x0 the top 8 significant bits, x1
+// the next 8, x2 the rest; |x1| < 2^-7 |x|, |x2| < 2^-14 |x|), each product of parts is exact
+// in FP32, and the six largest products are summed in FP32:
+//   x.t ~ x0 t0 + (x0 t1 + x1 t0) + (x0 t2 + x1 t1 + x2 t0);
+// the dropped x1 t2 + x2 t1 + x2 t2 stay below 2^-21 |x t| (transfer parts are rounded to
+// nearest: |t1| <= 2^-8 |t|, |t2| <= 2^-16 |t|).
 // One v_mfma_f32_16x16x32_bf16 covers two parent offsets (k = 16 a + real index, a the offset
 // of the pair), so an offset pair of a 16-target group costs 6 MFMAs of K = 32 instead of the 8
 // FP32 MFMAs of K = 4 per offset.  A block owns one frequency and a run of consecutive tiles;
