@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
       }
       __syncthreads();
       if (active) {
-        if constexpr (sizeof(Real) == 4) {
+        if constexpr (sizeof(Real) == 4 && VB == 1) {
           // split j takes the contiguous run [jb, je) of the chunk's sources (a multiple of 4
           // long, so that 4 sources are read with 16-byte LDS reads at immediate offsets); Real
           // partial sums over at most 64 sources, added into the FP64 accumulator
@@ -291,9 +291,9 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
               for (int k = 0; k < K1; k++) acc[v][k] += (double)part[v][k];
           }
         } else {
-          // FP64: split j takes sources j, j + nsplit, ... (the unrolled form above needs more
-          // registers than the occupancy of this kernel allows in FP64); partial sums over at
-          // most 64 sources, added into the accumulator
+          // FP64 and several density vectors: split j takes sources j, j + nsplit, ... (the
+          // unrolled form above needs more registers than this kernel's occupancy allows
+          // there); partial sums over at most 64 sources, added into the accumulator
           for (int s0 = j; s0 < pos; s0 += 64 * nsplit) {
             Real part[VB][K1];
 #pragma unroll
