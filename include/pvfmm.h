@@ -159,7 +159,8 @@ int pvfmm_amd_tree_evaluate_device(void* tree, const double* d_den, double* d_ou
 void* pvfmm_amd_tree_stream(void* tree);
 
 /* Launch-shape tuning of a set-up tree; no counterpart in the reference (its CPU kernels have
- * no launch shapes).  params = {hadamard_tiles_per_block, gemm_split_blocks}.
+ * no launch shapes).  params = {hadamard_tiles_per_block, gemm_split_blocks}
+ * (hadamard_tiles_per_block >= 1, used up to 64 by the FP32 kernel; gemm_split_blocks >= 0).
  * autotune times the candidates on this tree with one device-resident density vector d_den
  * (as pvfmm_amd_tree_evaluate_device, n_vec = 1; d_out is overwritten), keeps the fastest and
  * writes the chosen values to params[0..cap).  Returns 0 or an error code. */

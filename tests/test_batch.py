@@ -75,3 +75,9 @@ def test_autotune_keeps_results(prec, tol):
     t.set_tuning(hadamard_tiles_per_block=1, gemm_split_blocks=0)
     r = float(torch.linalg.norm(t.evaluate_device(q) - before) / torch.linalg.norm(before))
     assert r < tol, r
+    # runs longer than the FP32 Hadamard's 64-tile limit are clamped to it; the run length
+    # does not change any sum, so the result is the same bit for bit as with 64
+    t.set_tuning(hadamard_tiles_per_block=64, gemm_split_blocks=0)
+    r64 = t.evaluate_device(q)
+    t.set_tuning(hadamard_tiles_per_block=1000, gemm_split_blocks=0)
+    assert torch.equal(t.evaluate_device(q), r64)
