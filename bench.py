@@ -330,8 +330,8 @@ def main():
                     help="adaptive-grad-2m (headline, default) or a secondary workload")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (the single-GPU MPI_COMM_SELF share); --gpus {args.gpus} is not "
-              "supported in this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (the evaluation of one MPI_COMM_SELF share); --gpus {args.gpus} "
+              "is not supported: the multi-GPU sharding is a design, not built or measured", file=sys.stderr)
         sys.exit(2)
 
     import torch
