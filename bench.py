@@ -416,7 +416,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"laplace_grad FMM evaluation, N={n} adaptive (50% sphere r=0.4, 30% gaussian "
-                               f"sigma=0.02, 20% uniform), m={m}, max_pts={c['max_pts']}, mixed FP32 bulk / FP64 "
+                               f"sigma=0.02, 20% uniform), m={m}, max_pts={c['max_pts']}, mixed FP32 bulk (V-list "
+                               f"Hadamard products as exact 3-part bf16 splits on the matrix cores) / FP64 "
                                f"precompute and merge", "n_targets": n, "m": m, "max_pts": c["max_pts"]},
         "fp64_mtargets_s": n / fp64["step"] / 1e6,
         "fp64_ms_per_step": 1e3 * fp64["step"],
