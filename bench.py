@@ -5,11 +5,13 @@ densities already resident in HBM): Laplace potential + gradient (laplace_grad),
 points on a strongly adaptive octree (50% on the sphere |x-0.5|=0.4, 30% in a Gaussian blob of
 sigma 0.02 at (0.3,0.3,0.7), 20% uniform), densities N(0,1), m = 10, max_pts = 256.
 
-The reported value is the mixed-precision evaluator's Mtargets/s (FP32 P2P, M2L and up/down
-GEMMs; FP64 operator precompute and per-target merge).  The FP64 GPU evaluator is timed the
-same way and the ratio reported; both errors are relative L2 against an FP64 direct sum on 4096
-sampled targets.  The CPU baseline is the FP64 oracle (a restatement of the reference
-algorithm) on a 2^18-point sample of the same distribution, on the host cores.
+The reported value is the mixed-precision evaluator's Mtargets/s (FP32 P2P, DFTs and up/down
+GEMMs, the M2L Hadamard as exact three-part bf16 splits on the matrix cores; FP64 operator
+precompute and per-target merge).  The FP64 GPU evaluator is timed the same way and the ratio
+reported; both errors are relative L2 against the FP64 all-pairs direct sum on the GPU (which
+the tests check against the CPU direct sum) on 4096 sampled targets.  The CPU baseline is the
+FP64 oracle (a restatement of the reference algorithm) on a 2^18-point sample of the same
+distribution, on the host cores; it is the only use of the oracle here.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--n N] [--workload NAME]
 """
@@ -210,11 +212,9 @@ def bench_workload(name: str, steps: int, warmup: int, check: bool):
     def err(kernel, src, den, out, k1):
         if not check:
             return None
-        import oracle
-
         rng = np.random.default_rng(4096)
         idx = np.sort(rng.choice(len(src), min(4096, len(src)), replace=False))
-        ud = oracle.direct(kernel, src, den, src[idx])
+        ud = P.direct(kernel, src, den, src[idx])  # FP64 all-pairs sum on the GPU
         return rel_l2(out.reshape(len(src), k1)[idx].ravel(), ud)
 
     line = {"n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True, "scaling": "weak",
@@ -355,10 +355,9 @@ def main():
     dq = torch.from_numpy(c["den"]).to("cuda:0")
     rng = np.random.default_rng(4096)
     idx = np.sort(rng.choice(n, 4096, replace=False))
-    # FP64 direct sum on the sampled targets (error reference)
-    import oracle
-
-    ud = None if args.no_check else oracle.direct(c["kernel"], c["src"], c["den"], c["src"][idx]).reshape(-1, 3)
+    # FP64 direct sum on the sampled targets (error reference; the GPU all-pairs sum, which the
+    # tests compare with the CPU direct sum)
+    ud = None if args.no_check else P.direct(c["kernel"], c["src"], c["den"], c["src"][idx]).reshape(-1, 3)
 
     res = {}
     for prec in (["fp64", "mixed"] if args.prec == "both" else [args.prec]):
