@@ -1868,9 +1868,9 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
 
 template <typename Real>
 int DeviceKernels<Real>::hadamard_union_max(int e0) {
-  // FP32: 2 frequencies per block and a union of at most 250 keep a block's LDS under 40 KiB
-  // (four blocks per CU); FP64: 2 frequencies, 256; Stokes: 80
-  if (e0 != 1) return 80;
+  // scalar kernels: FP32 250, FP64 256 partners per tile; Stokes (2 frequencies per block, two
+  // blocks per CU in 160 KiB of LDS): FP32 160, FP64 80
+  if (e0 != 1) return sizeof(Real) == 4 ? 160 : 80;
   return sizeof(Real) == 4 ? 250 : 256;
 }
 
@@ -1903,7 +1903,12 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
                          tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
     }
   } else if (e0 == 3 && e1 == 3) {
-    hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, FB, 80, 1>), dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_tiles,
+    // 2 frequencies per block; FP32 tiles take up to 160 partners, FP64 80 (LDS)
+    constexpr int UMS = sizeof(Real) == 4 ? 160 : 80;
+    const int n_fg2 = (M_dim + 1) / 2;
+    const int64_t nb2 = (int64_t)n_tiles * ((n_fg2 + 7) / 8) * 8;
+    if (nb2 >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+    hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, 2, UMS, 1>), dim3((unsigned)nb2), dim3(256), 0, s, M_dim, n_tiles,
                        tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
   } else {
     throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
