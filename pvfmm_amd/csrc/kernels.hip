@@ -964,14 +964,19 @@ template <typename Real, int M>
 __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const int32_t* __restrict__ first_child,
                                                                          const Real* __restrict__ up, int K0, int e0,
                                                                          const int32_t* __restrict__ lat_inv,
-                                                                         const Real* __restrict__ scal, Real* __restrict__ X) {
+                                                                         const Real* __restrict__ scal, Real* __restrict__ X,
+                                                                         int n_slots) {
   using S = DftShape<Real, M>;
   constexpr int N1 = S::N1, NH = S::NH, MD = S::MD, M2 = S::M2, NT = S::NT, LPT = S::LPT, CG = S::CG, KC = S::KC_UP;
   constexpr int CST = KC * M2 + 1;  // child stride (odd, spreads the children over LDS banks)
   constexpr TwTable<N1> TW{};
   __shared__ Real twc[N1], tws[N1];
   __shared__ Real s1r[8 * CST], s1i[8 * CST];
-  const int slot = blockIdx.x / e0, i = blockIdx.x % e0;
+  // the e0 components of a source parent run back to back on one XCD (blocks b, b + 8, ...:
+  // block b runs on XCD b mod 8), so their interleaved stores to X merge in that XCD's L2
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  const int i = local % e0, slot = (local / e0) * 8 + xcd;
+  if (slot >= n_slots) return;  // whole block
   const int fc = first_child[slot];
   const Real sc = scal[slot];
   const int t = threadIdx.x, yz = t % M2, cg = t / M2, y = yz % M, z = yz / M;
@@ -1090,7 +1095,7 @@ template <typename Real, int M>
 __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_ifft_down_m(const int32_t* __restrict__ first_child,
                                                                             const Real* __restrict__ Y, Real* __restrict__ dn,
                                                                             int K1, int e1, const int32_t* __restrict__ lat_inv,
-                                                                            const Real* __restrict__ scal) {
+                                                                            const Real* __restrict__ scal, int n_slots) {
   using S = DftShape<Real, M>;
   constexpr int N1 = S::N1, NH = S::NH, MD = S::MD, M2 = S::M2, NT = S::NT, LPT = S::LPT, CG = S::CG, KC = S::KC_DN;
   constexpr int PL = N1 * M;         // one plane of one child: [ky][z]
@@ -1098,7 +1103,11 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_ifft_down_m(cons
   constexpr TwTable<N1> TW{};
   __shared__ Real twc[N1], tws[N1];
   __shared__ Real s1r[8 * CST], s1i[8 * CST];
-  const int slot = blockIdx.x / e1, j = blockIdx.x % e1;
+  // the e1 components of a target parent run back to back on one XCD, so their interleaved
+  // reads of Y share that XCD's L2 lines
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  const int j = local % e1, slot = (local / e1) * 8 + xcd;
+  if (slot >= n_slots) return;  // whole block
   const int fc = first_child[slot];
   const int t = threadIdx.x, yz = t % M2, cg = t / M2, y = yz % M, z = yz / M;
   if (t < N1) { twc[t] = (Real)TW.c[t]; tws[t] = (Real)TW.s[t]; }
@@ -1841,11 +1850,11 @@ template <typename Real>
 void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0, int m,
                                  const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X) {
   if (n_slots == 0) return;
-  const dim3 g((unsigned)(n_slots * e0));
+  const dim3 g((unsigned)((n_slots + 7) / 8 * 8 * e0));  // whole XCD rounds of slots
 #define PVFMM_FFT_UP_M(MM)                                                                                        \
   case MM:                                                                                                        \
     hipLaunchKernelGGL((k_fft_up_m<Real, MM>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, up, K0, e0, lat_inv, \
-                       scal, X);                                                                                  \
+                       scal, X, n_slots);                                                                         \
     HIP_LAUNCH_CHECK();                                                                                           \
     return;
   switch (m) {
@@ -1920,11 +1929,11 @@ template <typename Real>
 void DeviceKernels<Real>::ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn, int K1, int e1,
                                     int m, const int32_t* lattice, const int32_t* lat_inv, int ns, const Real* scal) {
   if (n_slots == 0) return;
-  const dim3 g((unsigned)(n_slots * e1));
+  const dim3 g((unsigned)((n_slots + 7) / 8 * 8 * e1));  // whole XCD rounds of slots
 #define PVFMM_IFFT_DN_M(MM)                                                                                       \
   case MM:                                                                                                        \
     hipLaunchKernelGGL((k_ifft_down_m<Real, MM>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, Y, dn, K1, e1,     \
-                       lat_inv, scal);                                                                            \
+                       lat_inv, scal, n_slots);                                                                   \
     HIP_LAUNCH_CHECK();                                                                                           \
     return;
   switch (m) {
