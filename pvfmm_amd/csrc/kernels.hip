@@ -1877,9 +1877,10 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
 
 template <typename Real>
 int DeviceKernels<Real>::hadamard_union_max(int e0) {
-  // scalar kernels: FP32 250, FP64 256 partners per tile; Stokes (2 frequencies per block, two
-  // blocks per CU in 160 KiB of LDS): FP32 160, FP64 80
-  if (e0 != 1) return sizeof(Real) == 4 ? 160 : 80;
+  // scalar kernels: FP32 250, FP64 256 partners per tile; Stokes (2 frequencies per block): 160,
+  // which fits two FP32 blocks or one FP64 block in a CU's 160 KiB of LDS (fuller tiles pay for
+  // the per-tile transfer-block loads over more targets)
+  if (e0 != 1) return 160;
   return sizeof(Real) == 4 ? 250 : 256;
 }
 
@@ -1912,8 +1913,8 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
                          tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
     }
   } else if (e0 == 3 && e1 == 3) {
-    // 2 frequencies per block; FP32 tiles take up to 160 partners, FP64 80 (LDS)
-    constexpr int UMS = sizeof(Real) == 4 ? 160 : 80;
+    // 2 frequencies per block, tiles of up to 160 partners
+    constexpr int UMS = 160;
     const int n_fg2 = (M_dim + 1) / 2;
     const int64_t nb2 = (int64_t)n_tiles * ((n_fg2 + 7) / 8) * 8;
     if (nb2 >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
