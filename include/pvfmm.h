@@ -155,6 +155,20 @@ int pvfmm_amd_tree_evaluate(void* tree, const double* den, double* out, int n_ve
 /* Same with device-resident arrays; the work is queued on the tree's stream and this call
  * returns after it completes unless `async` is nonzero. */
 int pvfmm_amd_tree_evaluate_device(void* tree, const double* d_den, double* d_out, int n_vec, int async);
+/* The transpose of the evaluation's linear map (the adjoint FMM, for gradients with respect to
+ * the densities): out = A^T g where pvfmm_amd_tree_evaluate computes out = A den.  g holds
+ * [n_vec][k1*n_trg] values in target input order, out receives [n_vec][k0*n_src] in source
+ * input order.  It runs the evaluation's phases in reverse order with the transposed
+ * translations (S2M^T, U2U^T, the V-list M2L of the opposite offset, D2D^T, L2T^T and the
+ * exchanged near-field interactions), so <g, A q> = <A^T g, q> holds to rounding.  The
+ * reference has no backward pass: its nearest interface is PtFMM_Evaluate
+ * (include/pvfmm.txx:256-328), whose translation operators (include/fmm_pts.txx:582-643,
+ * 719-764, 2275-2599, 5427-5737) these transpose.  Symmetric kernels only (laplace_poten,
+ * stokes_vel; laplace_grad reports PVFMM_ERR_UNSUPPORTED).  The first call builds the
+ * transposed lists on the host and uploads them; it shares the tree's scratch buffers and
+ * stream with the forward evaluation. */
+int pvfmm_amd_tree_evaluate_adjoint(void* tree, const double* g, double* out, int n_vec);
+int pvfmm_amd_tree_evaluate_adjoint_device(void* tree, const double* d_g, double* d_out, int n_vec, int async);
 /* The HIP stream (hipStream_t) the tree's kernels run on. */
 void* pvfmm_amd_tree_stream(void* tree);
 
@@ -172,9 +186,11 @@ int pvfmm_amd_tree_set_tuning(void* tree, const int* params, int n);
  * reference, whose callers keep one context per cloud.  The first evaluation records every
  * tree's evaluation (one density vector each, device-resident) into one HIP graph with one
  * branch per tree; later calls with the same device pointers replay it with a single graph
- * launch on the batch's stream.  Other pointers, or changed launch shapes of a tree, record
- * the graph again.  The trees must outlive the batch and must not be set up again while it
- * exists. */
+ * launch on the batch's stream.  Other pointers, changed launch shapes of a tree, or a tree
+ * set up again (its new plan has new device buffers) record the graph again.  A launch is
+ * ordered after the work already queued on every tree's stream and before work queued there
+ * later, so eager and batched evaluations of one tree may be mixed, also with async = 1.  The
+ * trees must outlive the batch. */
 void* pvfmm_amd_batch_create(void* const* trees, int n_trees);
 int pvfmm_amd_batch_evaluate_device(void* batch, const double* const* d_den, double* const* d_out, int async);
 void* pvfmm_amd_batch_stream(void* batch);
@@ -198,8 +214,9 @@ int pvfmm_amd_tree_phase_times(void* tree, double* ms, int cap);
 const char* pvfmm_amd_phase_name(int phase);
 /* plan statistics: n_nodes, n_leaves, max_depth, s2m_rows, l2t_rows, m2m_rows, l2l_rows, x_rows,
  * x_items, v_src_slots, v_trg_slots, v_pairs, u_rows, u_items, u_pairs, far_items, far_pairs,
- * s2m_pairs, u_src_points_streamed, device_bytes, v_tiles, v_tile_partners */
-int pvfmm_amd_tree_stats(void* tree, int64_t* out22);
+ * s2m_pairs, u_src_points_streamed, device_bytes, v_tiles, v_tile_partners, v_tile_max_partners;
+ * copies the first `cap` values and returns how many exist (negative error code on failure) */
+int pvfmm_amd_tree_stats(void* tree, int64_t* out, int cap);
 
 /* Operator matrices of an fmm handle (FP64), for parity checks against the oracle:
  * which 0 uc2ue0 (V S^+), 1 uc2ue1 (U^T), 2 dc2de0, 3 dc2de1, 4 u2u (8 stacked),

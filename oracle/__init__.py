@@ -38,7 +38,7 @@ RAD0 = 1.05  # include/pvfmm_common.hpp:33
 RAD1 = 2.95  # include/pvfmm_common.hpp:34
 
 # Mat_Type values, include/precomp_mat.hpp:21-39
-UC2UE0, UC2UE1, DC2DE0, DC2DE1, S2U, U2U, D2D, D2T, U0, U1, U2, V, V1, W, X, BC = range(16)
+UC2UE0, UC2UE1, DC2DE0, DC2DE1, S2U, U2U, D2D, D2T, U0, U1, U2, V, W, X, V1, BC = range(16)
 
 KERNELS = {"laplace_poten": 0, "laplace_grad": 1, "stokes_vel": 2}
 KDIM = {0: (1, 1), 1: (1, 3), 2: (3, 3)}
@@ -144,17 +144,21 @@ def expansion_kernel(kid: int) -> int:
 
 
 # ----------------------------------------------------------------------------- operators
-def _c2e(kid, equiv, check):
-    """SVD pseudo-inverse factors with the 4*eps*max_S cutoff (include/fmm_pts.txx:478-581)."""
+def _c2e(kid, equiv, check, eps):
+    """SVD pseudo-inverse factors with the 4*eps*max_S cutoff (include/fmm_pts.txx:478-581),
+    eps = machine epsilon of the evaluation type Real_t (include/fmm_pts.txx:503,555)."""
     M = kernel_matrix(kid, equiv, check)
     U, S, Vh = np.linalg.svd(M)
-    eps = np.finfo(np.float64).eps
     Sinv = np.where(S > eps * S.max() * 4, 1.0 / S, 0.0)
     return (Vh.T * Sinv[None, :]), U.T.copy()
 
 
-def precompute(kernel: str | int, m: int) -> dict:
+def precompute(kernel: str | int, m: int, real: str = "fp64") -> dict:
     """Level-0 operators of FMM_Pts::Precomp (include/fmm_pts.txx:453-764).
+
+    real: the reference's Real_t whose machine epsilon sets the pseudo-inverse cutoff
+    ("fp64", or "fp32" for the reference's single-precision build); the arithmetic is FP64
+    either way.
 
     Returns dict with uc2ue0/1, dc2de0/1, u2u[8], d2d[8] (by child position bits x|y<<1|z<<2)
     and v1[27] (by parent offset (dx+1)+3(dy+1)+9(dz+1), 13 unused), V1 layout
@@ -168,8 +172,9 @@ def precompute(kernel: str | int, m: int) -> dict:
     ue = surf_at(m, z, 0, RAD0)
     dc = surf_at(m, z, 0, RAD0)
     de = surf_at(m, z, 0, RAD1)
-    uc2ue0, uc2ue1 = _c2e(ke, ue, uc)
-    dc2de0, dc2de1 = _c2e(ke, de, dc)
+    eps = float(np.finfo({"fp64": np.float64, "fp32": np.float32}[real]).eps)
+    uc2ue0, uc2ue1 = _c2e(ke, ue, uc, eps)
+    dc2de0, dc2de1 = _c2e(ke, de, dc, eps)
     u2u, d2d = [], []
     for c in range(8):
         bits = np.array([(c >> 0) & 1, (c >> 1) & 1, (c >> 2) & 1], dtype=np.float64)

@@ -128,7 +128,7 @@ struct MortonId {
 // ---------------------------------------------------------------- interaction lists
 // Mat_Type order follows include/precomp_mat.hpp:21-39
 enum MatType { UC2UE0_T = 0, UC2UE1_T, DC2DE0_T, DC2DE1_T, S2U_T, U2U_T, D2D_T, D2T_T,
-               U0_T, U1_T, U2_T, V_T, V1_T, W_T, X_T, BC_T, TYPE_COUNT };
+               U0_T, U1_T, U2_T, V_T, W_T, X_T, V1_T, BC_T, TYPE_COUNT };
 
 struct InteracList {
   std::vector<std::vector<int>> rel;  // rel[t][3*i+d]

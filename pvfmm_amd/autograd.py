@@ -1,21 +1,28 @@
 """Differentiable Laplace FMM (adjoint FMM behind torch.autograd.Function).
 
-For sources = targets = x (n points) and charges q the FMM gives the potential
-phi_i = sum_{j != i} q_j / (4 pi |x_i - x_j|) (laplace_poten, include/kernel.txx:1276-1287) and
-the field u_i = sum_{j != i} q_j (x_i - x_j) / (4 pi |x_i - x_j|^3) = -grad phi(x_i)
-(laplace_grad, include/kernel.txx:1316-1330).  The kernel is symmetric, so the adjoint of
-the potential map is the map itself:
+For sources = targets = x (n points) and charges q the FMM gives the potential phi = A q, an
+approximation of phi_i = sum_{j != i} q_j / (4 pi |x_i - x_j|) (laplace_poten,
+include/kernel.txx:1276-1287), and the field u_i ~ sum_{j != i} q_j (x_i - x_j) /
+(4 pi |x_i - x_j|^3) = -grad phi(x_i) (laplace_grad, include/kernel.txx:1316-1330).  A is the
+FMM's own linear map; its transpose A^T is evaluated by the adjoint FMM
+(PtFMMTree.evaluate_adjoint: the phases in reverse order with the transposed translations), so
+the gradients with respect to the charges are those of the discrete map, exactly:
 
-* potential phi(x, q) with upstream gradient g:
-    dL/dq = K g                        (a potential FMM with density g)
+* potential phi(x, q) = A q with upstream gradient g:
+    dL/dq = A^T g                      (one adjoint FMM)
     dL/dx_i = -g_i u[q]_i - q_i u[g]_i (one gradient FMM with the two densities q and g)
-* energy E(x, q) = 1/2 q . phi:
-    dE/dq = phi                        (kept from the forward pass)
-    dE/dx_i = -q_i u[q]_i              (one gradient FMM)
+* energy E(x, q) = 1/2 q . A q:
+    dE/dq = (A q + A^T q) / 2          (A q kept from the forward pass, one adjoint FMM)
+    dE/dx_i = -q_i u[q]_i              (one gradient FMM, run beside the adjoint)
 
-The tree is built once for the point set given to LaplaceFMM (positions enter through the
-tree); x passed to potential()/energy() must be those positions and only receives the
-gradient.  All FMM work runs in libpvfmm.so on the GPU.
+A approximates the symmetric kernel matrix, so A^T g and A g agree to the FMM's accuracy; the
+adjoint makes the charge gradients consistent with the computed values to rounding.  The
+positions enter through the tree: the coordinate gradients come from the gradient-kernel FMM
+(the derivative of the exact kernel sum), not from differentiating the tree.
+
+The tree is built once for the point set given to LaplaceFMM; x passed to potential()/energy()
+must be those positions and only receives the gradient.  All FMM work runs in libpvfmm.so on
+the GPU.
 """
 from __future__ import annotations
 
@@ -40,10 +47,35 @@ class LaplaceFMM:
     def _phi(self, q):
         return self._tp.evaluate_device(q.contiguous())
 
+    def _phi_adjoint(self, g):
+        return self._tp.evaluate_adjoint_device(g.contiguous())
+
     def _field(self, dens):
         """dens (k, n) -> (k, n, 3) fields -grad phi of each density."""
         out = self._tg.evaluate_device(dens.contiguous())
         return out.reshape(dens.shape[0], self.n, 3)
+
+    def _adjoint_and_field(self, g, dens):
+        """A^T g and the fields of dens, queued on the two trees' own streams so that the two
+        FMMs may overlap on the GPU; returns after both complete."""
+        import torch
+
+        import ctypes
+
+        from . import ffi
+
+        g = g.contiguous()
+        dens = dens.contiguous()
+        adj = torch.empty_like(g)
+        fld = torch.empty((dens.shape[0], 3 * self.n), dtype=torch.float64, device=dens.device)
+        torch.cuda.current_stream(g.device).synchronize()
+        ffi.check(ffi.lib().pvfmm_amd_tree_evaluate_adjoint_device(self._tp._h, ctypes.c_void_p(g.data_ptr()),
+                                                                   ctypes.c_void_p(adj.data_ptr()), 1, 1))
+        ffi.check(ffi.lib().pvfmm_amd_tree_evaluate_device(self._tg._h, ctypes.c_void_p(dens.data_ptr()),
+                                                           ctypes.c_void_p(fld.data_ptr()), dens.shape[0], 1))
+        torch.cuda.ExternalStream(self._tp.stream_handle()).synchronize()
+        torch.cuda.ExternalStream(self._tg.stream_handle()).synchronize()
+        return adj, fld.reshape(dens.shape[0], self.n, 3)
 
     def potential(self, x, q):
         return _Potential.apply(x, q, self)
@@ -77,11 +109,15 @@ def _make_functions():
             (q,) = ctx.saved_tensors
             fmm = ctx.fmm
             g = g.contiguous()
-            dq = fmm._phi(g) if ctx.needs_input_grad[1] else None
-            dx = None
+            dq = dx = None
             if ctx.needs_input_grad[0]:
-                u = fmm._field(torch.stack([q.detach(), g]))
+                if ctx.needs_input_grad[1]:
+                    dq, u = fmm._adjoint_and_field(g, torch.stack([q.detach(), g]))
+                else:
+                    u = fmm._field(torch.stack([q.detach(), g]))
                 dx = -(g[:, None] * u[0] + q.detach()[:, None] * u[1])
+            elif ctx.needs_input_grad[1]:
+                dq = fmm._phi_adjoint(g)
             return dx, dq, None
 
     class Energy(torch.autograd.Function):
@@ -96,11 +132,18 @@ def _make_functions():
         @staticmethod
         def backward(ctx, gE):
             q, phi = ctx.saved_tensors
-            dq = gE * phi if ctx.needs_input_grad[1] else None
-            dx = None
+            fmm = ctx.fmm
+            qd = q.detach()
+            dq = dx = None
             if ctx.needs_input_grad[0]:
-                u = ctx.fmm._field(q.detach()[None, :])[0]
-                dx = -gE * q.detach()[:, None] * u
+                if ctx.needs_input_grad[1]:
+                    aq, u = fmm._adjoint_and_field(qd, qd[None, :])
+                    dq = gE * 0.5 * (phi + aq)
+                else:
+                    u = fmm._field(qd[None, :])
+                dx = -gE * qd[:, None] * u[0]
+            elif ctx.needs_input_grad[1]:
+                dq = gE * 0.5 * (phi + fmm._phi_adjoint(qd))
             return dx, dq, None
 
     return Potential, Energy

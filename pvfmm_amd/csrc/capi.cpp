@@ -3,6 +3,7 @@
 // recorded per thread and never abort the caller.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -13,6 +14,7 @@
 #include "../../include/pvfmm.h"
 #include "common.hpp"
 #include "device_fmm.hpp"
+#include "gpu_octree.hpp"
 #include "kernels.hpp"
 #include "octree.hpp"
 #include "operators.hpp"
@@ -89,7 +91,12 @@ TreeHandle* make_tree(const double* src, int64_t n_src, const double* trg, int64
   auto h = std::make_unique<TreeHandle>();
   h->src.assign(src, src + 3 * n_src);
   h->trg.assign(trg, trg + 3 * n_trg);
-  build_octree(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pt, n_pt, max_pts);
+  // the tree-defining set of PtFMM_CreateTree is the target set itself (include/pvfmm.txx:239)
+  const double* pts = (pt == trg && n_pt == n_trg) ? h->trg.data() : pt;
+  if (device_tree_enabled())
+    build_octree_device(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pts, n_pt, max_pts);
+  else
+    build_octree(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pts, n_pt, max_pts);
   return h.release();
 }
 
@@ -118,15 +125,26 @@ struct BatchHandle {
   std::vector<const double*> den_key;
   std::vector<double*> out_key;
   std::vector<std::pair<int, int>> tune_key;  // launch shapes baked into the graph
+  std::vector<uint64_t> plan_key;             // plans (device addresses) baked into the graph
+  // ordering of a replay against eager work on the trees' own streams (outside the graph)
+  std::vector<hipEvent_t> before;
+  hipEvent_t after = nullptr;
   int records = 0;
   std::vector<std::pair<int, int>> tunings() const {
     std::vector<std::pair<int, int>> v;
     for (TreeHandle* t : trees) v.emplace_back(t->plan->tuning.hadamard_tpb, t->plan->tuning.gemm_split_blocks);
     return v;
   }
+  std::vector<uint64_t> plans() const {
+    std::vector<uint64_t> v;
+    for (TreeHandle* t : trees) v.push_back(t->plan->generation);
+    return v;
+  }
   ~BatchHandle() {
     drop_graph();
     for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+    for (hipEvent_t e : before) (void)hipEventDestroy(e);
+    if (after) (void)hipEventDestroy(after);
     if (fork) (void)hipEventDestroy(fork);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -160,6 +178,7 @@ struct BatchHandle {
     den_key.assign(d_den, d_den + trees.size());
     out_key.assign(d_out, d_out + trees.size());
     tune_key = tunings();
+    plan_key = plans();
     records++;
   }
 };
@@ -175,7 +194,10 @@ BatchHandle* make_batch(void* const* trees, int n) {
   HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   b->joins.resize(n, nullptr);
+  b->before.resize(n, nullptr);
   for (int i = 0; i < n; i++) HIP_CHECK(hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming));
+  for (int i = 0; i < n; i++) HIP_CHECK(hipEventCreateWithFlags(&b->before[i], hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&b->after, hipEventDisableTiming));
   return b.release();
 }
 
@@ -188,11 +210,27 @@ void batch_eval(BatchHandle* b, const double* const* d_den, double* const* d_out
     if (!b->trees[i]->plan) throw Error(kErrState, "a tree of the batch lost its plan");
     b->trees[i]->plan->set_timing(false);  // event records stay out of the graph
   }
+  // a tree set up again has a new plan (new device buffers): the graph is recorded again
   const bool same = b->exec && std::equal(b->den_key.begin(), b->den_key.end(), d_den) &&
-                    std::equal(b->out_key.begin(), b->out_key.end(), d_out) && b->tune_key == b->tunings();
+                    std::equal(b->out_key.begin(), b->out_key.end(), d_out) && b->tune_key == b->tunings() &&
+                    b->plan_key == b->plans();
   if (!same) b->record(d_den, d_out);
+  // the replay uses each plan's scratch buffers, as eager evaluations on the tree's own stream
+  // do: it starts after the work queued there and that work resumes after it
+  for (size_t i = 0; i < n; i++) {
+    HIP_CHECK(hipEventRecord(b->before[i], b->trees[i]->stream));
+    HIP_CHECK(hipStreamWaitEvent(b->stream, b->before[i], 0));
+  }
   HIP_CHECK(hipGraphLaunch(b->exec, b->stream));
+  HIP_CHECK(hipEventRecord(b->after, b->stream));
+  for (size_t i = 0; i < n; i++) HIP_CHECK(hipStreamWaitEvent(b->trees[i]->stream, b->after, 0));
   if (!async) HIP_CHECK(hipStreamSynchronize(b->stream));
+}
+
+void eval_adjoint_device(TreeHandle* t, const double* d_g, double* d_out, int n_vec) {
+  if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
+  if (n_vec > 0 && (!d_g || !d_out)) throw Error(kErrInvalidArgument, "null device array");
+  t->plan->evaluate_adjoint(d_g, d_out, n_vec, 1.0, 1.0);
 }
 
 void eval_host(TreeHandle* t, const double* den, double* out, int n_vec, double den_scale, double out_scale) {
@@ -233,7 +271,7 @@ Context* create_context(Real box_size, int n, int m, int kernel, bool single) {
   c->single = single;
   c->precision = single ? 1 : 0;
   c->fmm = std::make_unique<FmmHandle>();
-  build_operators(c->fmm->O, kernel, m, 0.0, c->precision);
+  build_operators_cached(c->fmm->O, kernel, m, 0.0, c->precision, std::getenv("PVFMM_DIR"));
   c->fmm->precision = c->precision;
   // src/pvfmm-wrapper.cpp:416-432 at np = 1, in the context's precision
   size_t NN = (size_t)std::ceil(std::pow((Real)n, (Real)(1.0 / 3.0)));
@@ -302,7 +340,7 @@ void context_eval(Context* c, const Real* src_pos, const Real* sl_den, const Rea
     c->tree.reset(make_tree(sp.data(), n_src, tp.data(), n_trg, c->grid.data(), (int64_t)(c->grid.size() / 3), c->max_pts));
     if (c->fmm->precision != c->precision) {
       // the pseudo-inverse truncation depends on the evaluation precision
-      build_operators(c->fmm->O, c->kernel, c->m, 0.0, c->precision);
+      build_operators_cached(c->fmm->O, c->kernel, c->m, 0.0, c->precision, std::getenv("PVFMM_DIR"));
       c->fmm->precision = c->precision;
     }
     setup_tree(c->tree.get(), c->fmm.get(), 1);
@@ -417,7 +455,7 @@ void* pvfmm_amd_fmm_create(int kernel, int m, int precision, double pinv_cutoff)
   guarded([&] {
     if (precision != 0 && precision != 1) throw Error(kErrInvalidArgument, "precision must be 0 or 1");
     auto h = std::make_unique<FmmHandle>();
-    build_operators(h->O, kernel, m, pinv_cutoff, precision);
+    build_operators_cached(h->O, kernel, m, pinv_cutoff, precision, std::getenv("PVFMM_DIR"));
     h->precision = precision;
     f = h.release();
   });
@@ -449,6 +487,29 @@ int pvfmm_amd_tree_evaluate_device(void* tree, const double* d_den, double* d_ou
     TreeHandle* t = (TreeHandle*)tree;
     eval_device(t, d_den, d_out, n_vec, 1.0, 1.0);
     if (!async) HIP_CHECK(hipStreamSynchronize(t->stream));
+  });
+}
+int pvfmm_amd_tree_evaluate_adjoint_device(void* tree, const double* d_g, double* d_out, int n_vec, int async) {
+  return guarded([&] {
+    TreeHandle* t = (TreeHandle*)tree;
+    eval_adjoint_device(t, d_g, d_out, n_vec);
+    if (!async) HIP_CHECK(hipStreamSynchronize(t->stream));
+  });
+}
+int pvfmm_amd_tree_evaluate_adjoint(void* tree, const double* g, double* out, int n_vec) {
+  return guarded([&] {
+    TreeHandle* t = (TreeHandle*)tree;
+    if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
+    if (n_vec < 1) throw Error(kErrInvalidArgument, "n_vec must be >= 1");
+    const KernelInfo& ki = t->fmm->O.ker;
+    const size_t ng = (size_t)ki.k1 * t->T.trg.n * n_vec, no = (size_t)ki.k0 * t->T.src.n * n_vec;
+    // the host staging buffers of eval_host, with the roles of input and output exchanged
+    t->d_out.ensure(std::max<size_t>(8, ng * sizeof(double)));
+    t->d_den.ensure(std::max<size_t>(8, no * sizeof(double)));
+    if (ng) HIP_CHECK(hipMemcpyAsync(t->d_out.p, g, ng * sizeof(double), hipMemcpyHostToDevice, t->stream));
+    eval_adjoint_device(t, (const double*)t->d_out.p, (double*)t->d_den.p, n_vec);
+    if (no) HIP_CHECK(hipMemcpyAsync(out, t->d_den.p, no * sizeof(double), hipMemcpyDeviceToHost, t->stream));
+    HIP_CHECK(hipStreamSynchronize(t->stream));
   });
 }
 void* pvfmm_amd_tree_stream(void* tree) { return tree ? (void*)((TreeHandle*)tree)->stream : nullptr; }
@@ -541,16 +602,21 @@ int pvfmm_amd_tree_phase_times(void* tree, double* ms, int cap) {
   return (int)t->plan->phase_ms.size();
 }
 const char* pvfmm_amd_phase_name(int phase) { return phase_name(phase); }
-int pvfmm_amd_tree_stats(void* tree, int64_t* o) {
-  return guarded([&] {
+int pvfmm_amd_tree_stats(void* tree, int64_t* o, int cap) {
+  int count = 0;
+  const int err = guarded([&] {
     TreeHandle* t = (TreeHandle*)tree;
-    if (!t->plan) throw Error(kErrState, "no plan");
+    if (!t || !t->plan) throw Error(kErrState, "no plan");
+    if (cap < 0 || (cap > 0 && !o)) throw Error(kErrInvalidArgument, "bad output array");
     const PlanStats& s = t->plan->stats;
-    int64_t v[22] = {s.n_nodes, s.n_leaves, s.max_depth, s.s2m_rows, s.l2t_rows, s.m2m_rows, s.l2l_rows, s.x_rows, s.x_items,
-                     s.v_src_slots, s.v_trg_slots, s.v_pairs, s.u_rows, s.u_items, s.u_pairs, s.far_items, s.far_pairs,
-                     s.s2m_pairs, s.u_src_points_streamed, s.device_bytes, s.v_tiles, s.v_tile_partners};
-    std::copy(v, v + 22, o);
+    const int64_t v[] = {s.n_nodes, s.n_leaves, s.max_depth, s.s2m_rows, s.l2t_rows, s.m2m_rows, s.l2l_rows, s.x_rows,
+                         s.x_items, s.v_src_slots, s.v_trg_slots, s.v_pairs, s.u_rows, s.u_items, s.u_pairs, s.far_items,
+                         s.far_pairs, s.s2m_pairs, s.u_src_points_streamed, s.device_bytes, s.v_tiles, s.v_tile_partners,
+                         s.v_tile_max_partners};
+    count = (int)(sizeof(v) / sizeof(v[0]));
+    std::copy(v, v + std::min(cap, count), o);
   });
+  return err ? -err : count;
 }
 int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out) {
   FmmHandle* f = (FmmHandle*)fmm;

@@ -4,6 +4,7 @@
 #include "device_fmm.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -16,6 +17,9 @@ void throw_device_error(hipError_t e, const char* file, int line) {
   throw Error(e == hipErrorOutOfMemory ? kErrOutOfMemory : kErrDevice,
               std::string("HIP error '") + hipGetErrorString(e) + "' at " + file + ":" + std::to_string(line));
 }
+
+static std::atomic<uint64_t> g_plan_generation{0};
+DevicePlanBase::DevicePlanBase() : generation(++g_plan_generation) {}
 
 const char* phase_name(int p) {
   static const char* names[] = {"Gather", "S2U-P2P", "S2U-GEMM", "U2U", "X-List", "V-List:FFT", "V-List:Hadamard",
@@ -147,7 +151,7 @@ class DevicePlan final : public DevicePlanBase {
       knob = best_v;
     };
     try {
-      if (sizeof(Real) == 4 && e0_ == 1 && n_vtiles_ > 0) pick(tuning.hadamard_tpb, {4, 8, 16, 32, 64}, {kPhHadamard});
+      if (sizeof(Real) == 4 && e0_ == 1 && vt_.n > 0) pick(tuning.hadamard_tpb, {4, 8, 16, 32, 64}, {kPhHadamard});
       pick(tuning.gemm_split_blocks, {0, 256, 512, 1024, 2048}, {kPhS2UGemm, kPhU2U, kPhD2D, kPhD2TGemm});
     } catch (...) {
       timing_ = was;
@@ -198,11 +202,85 @@ class DevicePlan final : public DevicePlanBase {
   DBuf<int32_t> v_src_fc_, v_trg_fc_, v_partner_, v_offsets_;
   DBuf<Real> v_src_scal_, v_trg_scal_, X_, Y_;
   int n_vsrc_ = 0, n_vtrg_ = 0, n_off_ = 0;
+
   // V-list target tiles for the LDS-staged Hadamard
-  DBuf<int32_t> vt_t0_, vt_u0_, vt_usrc_;
-  DBuf<uint16_t> vt_uidx_;
-  DBuf<uint32_t> vt_gmask_;
-  int n_vtiles_ = 0;
+  struct TileSet {
+    DBuf<int32_t> t0, u0, usrc;
+    DBuf<uint16_t> uidx;
+    DBuf<uint32_t> gmask;
+    int n = 0;
+    int64_t partners = 0, max_partners = 0;
+  };
+  TileSet vt_;
+
+  // Tiles of consecutive target slots: at most 64 targets, at most umax distinct partners
+  // (greedy in slot order, which is Morton order within a level).  partner[t * n_off + o] is
+  // the source slot of target slot t at offset o, or -1.
+  void build_tiles(const std::vector<int32_t>& v_partner, int n_t, int n_s, TileSet& ts, int64_t* total) {
+    const int umax = K::hadamard_union_max(e0_);
+    std::vector<int32_t> t0{0}, u0{0}, usrc;
+    std::vector<uint16_t> uidx;
+    std::vector<int32_t> pos(std::max(1, n_s), -1);
+    std::vector<int32_t> cur;  // distinct partners of the open tile
+    auto close_tile = [&](int t_end) {
+      for (int32_t sl : cur) pos[sl] = -1;
+      usrc.insert(usrc.end(), cur.begin(), cur.end());
+      cur.clear();
+      t0.push_back(t_end);
+      u0.push_back((int32_t)usrc.size());
+    };
+    // first pass: tile boundaries and union lists
+    for (int t = 0; t < n_t; t++) {
+      int add = 0;
+      for (int k = 0; k < n_off_; k++) {
+        const int32_t sl = v_partner[(size_t)t * n_off_ + k];
+        if (sl >= 0 && pos[sl] < 0) { pos[sl] = -2; add++; }  // provisional mark
+      }
+      for (int k = 0; k < n_off_; k++) {
+        const int32_t sl = v_partner[(size_t)t * n_off_ + k];
+        if (sl >= 0 && pos[sl] == -2) pos[sl] = -1;
+      }
+      if (t > t0.back() && (t - t0.back() == 64 || (int)cur.size() + add > umax)) close_tile(t);
+      for (int k = 0; k < n_off_; k++) {
+        const int32_t sl = v_partner[(size_t)t * n_off_ + k];
+        if (sl >= 0 && pos[sl] < 0) { pos[sl] = (int32_t)cur.size(); cur.push_back(sl); }
+      }
+    }
+    if (n_t > t0.back()) close_tile(n_t);
+    const int n_tiles = (int)t0.size() - 1;
+    // a missing partner points at the staging area's zero entry (index umax).  The FP32
+    // scalar kernel reads its table as [k][group][row][o % 2][o / 2 (padded to 16)], the
+    // others as [k][o][row][group].
+    const bool pair_layout = std::is_same<Real, float>::value && e0_ == 1 && e1_ == 1;
+    const size_t per_tile = pair_layout ? (size_t)4 * 16 * 2 * 16 : (size_t)64 * n_off_;
+    if (pair_layout && n_off_ > 32) throw Error(kErrInternal, "V-list offsets exceed the pair table");
+    uidx.assign((size_t)n_tiles * per_tile, (uint16_t)umax);
+    std::vector<uint32_t> gmask((size_t)n_tiles * 4, 0u);  // offsets present per 16-target group
+    for (int k = 0; k < n_tiles; k++) {
+      for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = u - u0[k];
+      if (pair_layout) color_union(k, n_off_, t0, u0, usrc, v_partner, pos);
+      for (int t = t0[k]; t < t0[k + 1]; t++)
+        for (int o = 0; o < n_off_; o++) {
+          const int32_t sl = v_partner[(size_t)t * n_off_ + o];
+          const int tt = t - t0[k];  // target tt = 16 * group + row of the MFMA tiles
+          if (sl >= 0) {
+            const size_t at = pair_layout ? (size_t)k * per_tile + ((size_t)tt * 2 + (o & 1)) * 16 + (o >> 1)
+                                          : (((size_t)k * n_off_ + o) * 16 + (tt & 15)) * 4 + (tt >> 4);
+            uidx[at] = (uint16_t)pos[sl];
+            gmask[(size_t)k * 4 + (tt >> 4)] |= 1u << o;
+          }
+        }
+      for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = -1;
+      if (u0[k + 1] - u0[k] > umax || t0[k + 1] - t0[k] > 64)
+        throw Error(kErrInternal, "V-list tile exceeds its staging capacity");
+      ts.max_partners = std::max<int64_t>(ts.max_partners, u0[k + 1] - u0[k]);
+    }
+    ts.n = n_tiles;
+    ts.partners = (int64_t)usrc.size();
+    usrc.push_back(0);  // one readable slot past the end: the kernels read ahead branch-free
+    ts.t0.upload(t0, total); ts.u0.upload(u0, total); ts.usrc.upload(usrc, total); ts.uidx.upload(uidx, total);
+    ts.gmask.upload(gmask, total);
+  }
   // L2T
   DBuf<int32_t> l2t_ra_, l2t_rc_, l2t_seq_;
   DBuf<Real> l2t_alpha_;
@@ -395,6 +473,7 @@ class DevicePlan final : public DevicePlanBase {
         s2m_pairs += T.pt_cnt0(i) * ns_;
       }
     n_s2m_ = (int)s2m_node.size();
+    h_.s2m_node = s2m_node;
     s2m_node_.upload(s2m_node, &total); s2m_ptr_.upload(s2m_ptr, &total); s2m_items_.upload(s2m_items, &total);
     s2m_orow_.upload(s2m_orow, &total); s2m_alpha_.upload(s2m_alpha, &total);
     chk_vs_ = (int64_t)std::max(1, n_s2m_) * K1_;
@@ -462,6 +541,7 @@ class DevicePlan final : public DevicePlanBase {
     }
     n_x_ = (int)x_node.size();
     x_node_.upload(x_node, &total); x_ptr_.upload(x_ptr, &total); x_items_.upload(x_items, &total); x_orow_.upload(x_orow, &total);
+    h_.x_node = x_node; h_.x_ptr = x_ptr; h_.x_items = x_items;
 
     // ---- V list (V_ListSetup, include/fmm_pts.txx:3346-3595): parents holding sources / targets
     std::vector<int32_t> slot_of(N, -1), v_src_fc, v_trg_fc, v_partner;
@@ -471,6 +551,7 @@ class DevicePlan final : public DevicePlanBase {
         slot_of[i] = (int32_t)v_src_fc.size();
         v_src_fc.push_back(T.first_child[i]);
         v_src_scal.push_back((Real)std::pow(2.0, ki.src_scal * (T.depth[i] + 1)));
+        h_.vsrc_depth.push_back(T.depth[i]);
       }
     std::vector<int32_t> offs(O_.v1_offsets.begin(), O_.v1_offsets.end());
     n_off_ = (int)offs.size();
@@ -494,6 +575,7 @@ class DevicePlan final : public DevicePlanBase {
       v_pairs += cnt;
       v_trg_fc.push_back(T.first_child[i]);
       v_trg_scal.push_back((Real)std::pow(2.0, ki.trg_scal * (T.depth[i] + 1)));
+      h_.vtrg_depth.push_back(T.depth[i]);
       v_partner.insert(v_partner.end(), row.begin(), row.end());
     }
     n_vsrc_ = (int)v_src_fc.size();
@@ -501,72 +583,11 @@ class DevicePlan final : public DevicePlanBase {
     v_src_fc_.upload(v_src_fc, &total); v_trg_fc_.upload(v_trg_fc, &total);
     v_src_scal_.upload(v_src_scal, &total); v_trg_scal_.upload(v_trg_scal, &total);
     v_partner_.upload(v_partner, &total); v_offsets_.upload(offs, &total);
-    {
-      // Tiles of consecutive target parents: at most 64 targets, at most umax distinct partners
-      // (greedy in slot order, which is Morton order within a level).
-      const int umax = K::hadamard_union_max(e0_);
-      std::vector<int32_t> t0{0}, u0{0}, usrc;
-      std::vector<uint16_t> uidx;
-      std::vector<int32_t> pos(n_vsrc_, -1);
-      std::vector<int32_t> cur;  // distinct partners of the open tile
-      auto close_tile = [&](int t_end) {
-        for (int32_t sl : cur) pos[sl] = -1;
-        usrc.insert(usrc.end(), cur.begin(), cur.end());
-        cur.clear();
-        t0.push_back(t_end);
-        u0.push_back((int32_t)usrc.size());
-      };
-      // first pass: tile boundaries and union lists
-      for (int t = 0; t < n_vtrg_; t++) {
-        int add = 0;
-        for (int k = 0; k < n_off_; k++) {
-          const int32_t sl = v_partner[(size_t)t * n_off_ + k];
-          if (sl >= 0 && pos[sl] < 0) { pos[sl] = -2; add++; }  // provisional mark
-        }
-        for (int k = 0; k < n_off_; k++) {
-          const int32_t sl = v_partner[(size_t)t * n_off_ + k];
-          if (sl >= 0 && pos[sl] == -2) pos[sl] = -1;
-        }
-        if (t > t0.back() && (t - t0.back() == 64 || (int)cur.size() + add > umax)) close_tile(t);
-        for (int k = 0; k < n_off_; k++) {
-          const int32_t sl = v_partner[(size_t)t * n_off_ + k];
-          if (sl >= 0 && pos[sl] < 0) { pos[sl] = (int32_t)cur.size(); cur.push_back(sl); }
-        }
-      }
-      if (n_vtrg_ > t0.back()) close_tile(n_vtrg_);
-      n_vtiles_ = (int)t0.size() - 1;
-      // a missing partner points at the staging area's zero entry (index umax).  The FP32
-      // scalar kernel reads its table as [k][group][row][o % 2][o / 2 (padded to 16)], the
-      // others as [k][o][row][group].
-      const bool pair_layout = std::is_same<Real, float>::value && e0_ == 1 && e1_ == 1;
-      const size_t per_tile = pair_layout ? (size_t)4 * 16 * 2 * 16 : (size_t)64 * n_off_;
-      if (pair_layout && n_off_ > 32) throw Error(kErrInternal, "V-list offsets exceed the pair table");
-      uidx.assign((size_t)n_vtiles_ * per_tile, (uint16_t)umax);
-      std::vector<uint32_t> gmask((size_t)n_vtiles_ * 4, 0u);  // offsets present per 16-target group
-      for (int k = 0; k < n_vtiles_; k++) {
-        for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = u - u0[k];
-        if (pair_layout) color_union(k, n_off_, t0, u0, usrc, v_partner, pos);
-        for (int t = t0[k]; t < t0[k + 1]; t++)
-          for (int o = 0; o < n_off_; o++) {
-            const int32_t sl = v_partner[(size_t)t * n_off_ + o];
-            const int tt = t - t0[k];  // target tt = 16 * group + row of the MFMA tiles
-            if (sl >= 0) {
-              const size_t at = pair_layout ? (size_t)k * per_tile + ((size_t)tt * 2 + (o & 1)) * 16 + (o >> 1)
-                                            : (((size_t)k * n_off_ + o) * 16 + (tt & 15)) * 4 + (tt >> 4);
-              uidx[at] = (uint16_t)pos[sl];
-              gmask[(size_t)k * 4 + (tt >> 4)] |= 1u << o;
-            }
-          }
-        for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = -1;
-        if (u0[k + 1] - u0[k] > umax || t0[k + 1] - t0[k] > 64)
-          throw Error(kErrInternal, "V-list tile exceeds its staging capacity");
-      }
-      stats.v_tiles = n_vtiles_;
-      stats.v_tile_partners = (int64_t)usrc.size();
-      usrc.push_back(0);  // one readable slot past the end: the kernels read ahead branch-free
-      vt_t0_.upload(t0, &total); vt_u0_.upload(u0, &total); vt_usrc_.upload(usrc, &total); vt_uidx_.upload(uidx, &total);
-      vt_gmask_.upload(gmask, &total);
-    }
+    h_.v_partner = v_partner; h_.v_src_fc = v_src_fc; h_.v_trg_fc = v_trg_fc; h_.offs = offs;
+    build_tiles(v_partner, n_vtrg_, n_vsrc_, vt_, &total);
+    stats.v_tiles = vt_.n;
+    stats.v_tile_partners = vt_.partners;
+    stats.v_tile_max_partners = vt_.max_partners;
     X_.alloc((int64_t)n_vsrc_ * M_dim_ * 16 * e0_, &total);
     Y_.alloc((int64_t)n_vtrg_ * M_dim_ * 16 * e1_, &total);
 
@@ -608,6 +629,8 @@ class DevicePlan final : public DevicePlanBase {
     far_ptr_.upload(far_ptr, &total); far_node_.upload(far_node, &total); far_den_.upload(far_den, &total);
     far_rad_.upload(far_rad, &total);
     u_ptr_.upload(u_ptr, &total); u_items_.upload(u_items, &total);
+    h_.tleaf = tleaf; h_.u_ptr = u_ptr; h_.u_items = u_items;
+    h_.far_ptr = far_ptr; h_.far_node = far_node; h_.far_rad = far_rad;
 
     ev_.resize(kPhaseCount + 1);
     for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
@@ -623,8 +646,255 @@ class DevicePlan final : public DevicePlanBase {
     stats.u_src_points_streamed = u_stream;
     stats.far_items = (int64_t)far_node.size(); stats.far_pairs = far_pairs; stats.s2m_pairs = s2m_pairs;
     stats.device_bytes = total;
+    device_bytes_ = total;
   }
 
+  // ---- adjoint evaluation: the transpose of the evaluation's linear map, phase by phase in
+  // reverse order with the transposed translations.  For the symmetric kernels (K(x, y) =
+  // K(y, x)^T: laplace_poten, stokes_vel) every transposed point interaction is the same kernel
+  // with sources and targets exchanged, and the V-list M2L of offset R transposes to the one of
+  // offset -R (the check and equivalent lattices of a node coincide at radius RAD0).
+  struct HostLists {  // forward lists kept on the host to build the transposed ones
+    std::vector<int32_t> tleaf, u_ptr, u_items, far_ptr, far_node, x_node, x_ptr, x_items, s2m_node;
+    std::vector<double> far_rad;
+    std::vector<int32_t> v_partner, v_src_fc, v_trg_fc, offs, vsrc_depth, vtrg_depth;
+  } h_;
+  int64_t device_bytes_ = 0;
+
+  struct AdjointPlan {
+    DBuf<Real> uc2ue0T, uc2ue1T, dc2de0T, dc2de1T, u2uT, d2dT;
+    DBuf<Real> tden;     // adjoint input gathered into target order (k1 x n_trg per vector)
+    DBuf<double> sval;   // adjoint output accumulated in source order (k0 x n_src per vector)
+    DBuf<int32_t> iota;  // 0, 1, 2, ...: one item per row
+    DBuf<int32_t> u_leaf, u_ptr, u_items;  // U list: source leaf <- target leaves
+    int n_u = 0;
+    DBuf<int32_t> w_node, w_ptr, w_items;  // W list: RAD0 surface of a source node <- target leaves
+    int n_w = 0;
+    DBuf<int32_t> x_leaf, x_ptr, x_node;   // X list: source leaf <- RAD0 check surfaces
+    DBuf<int64_t> x_den;
+    DBuf<double> x_rad;
+    int n_x = 0;
+    DBuf<int64_t> s2u_den;                 // S2U: source leaf <- its RAD1 check surface
+    DBuf<double> s2u_rad;
+    TileSet vt;                            // V list: source parents <- target parents
+    DBuf<int32_t> v_fc;                    // first child of each transposed target slot
+    DBuf<Real> fft_scal, ifft_scal;
+    int n_vt = 0;
+  };
+  std::unique_ptr<AdjointPlan> adj_;
+
+  static std::vector<Real> transposed(const std::vector<Real>& a, int rows, int cols) {
+    std::vector<Real> t((size_t)rows * cols);
+    for (int i = 0; i < rows; i++)
+      for (int j = 0; j < cols; j++) t[(size_t)j * rows + i] = a[(size_t)i * cols + j];
+    return t;
+  }
+
+  // CSR of the transposed relation: for rows r with items it, the rows of item -> list of r
+  static void transpose_csr(const std::vector<int32_t>& row_key, const std::vector<int32_t>& ptr,
+                            const std::vector<int32_t>& items, int n_keys, std::vector<int32_t>& out_row,
+                            std::vector<int32_t>& out_ptr, std::vector<int32_t>& out_items) {
+    std::vector<int32_t> cnt(n_keys + 1, 0);
+    for (int32_t s : items) cnt[s + 1]++;
+    std::vector<int32_t> start(n_keys + 1, 0);
+    for (int k = 0; k < n_keys; k++) start[k + 1] = start[k] + cnt[k + 1];
+    std::vector<int32_t> flat(items.size());
+    std::vector<int32_t> fill = start;
+    for (size_t r = 0; r + 1 < ptr.size(); r++)
+      for (int32_t i = ptr[r]; i < ptr[r + 1]; i++) flat[fill[items[i]]++] = row_key[r];
+    out_row.clear(); out_ptr.assign(1, 0); out_items.clear();
+    for (int k = 0; k < n_keys; k++) {
+      if (start[k + 1] == start[k]) continue;
+      out_row.push_back(k);
+      out_items.insert(out_items.end(), flat.begin() + start[k], flat.begin() + start[k + 1]);
+      out_ptr.push_back((int32_t)out_items.size());
+    }
+  }
+
+  void build_adjoint() {
+    auto a = std::make_unique<AdjointPlan>();
+    int64_t total = 0;
+    const int N = n_nodes_;
+    // transposed operators, in the padded layouts of the forward ones
+    auto conv = [](const std::vector<double>& v) { return std::vector<Real>(v.begin(), v.end()); };
+    auto pad_cols = [&](const std::vector<double>& m, int rows, int r, int rp) {
+      std::vector<Real> out((size_t)rows * rp, Real(0));
+      for (int i = 0; i < rows; i++)
+        for (int j = 0; j < r; j++) out[(size_t)i * rp + j] = (Real)m[(size_t)i * r + j];
+      return out;
+    };
+    auto pad_rows = [&](const std::vector<double>& m, int r, int cols, int rp) {
+      std::vector<Real> out((size_t)rp * cols, Real(0));
+      for (int i = 0; i < r; i++)
+        for (int j = 0; j < cols; j++) out[(size_t)i * cols + j] = (Real)m[(size_t)i * cols + j];
+      return out;
+    };
+    a->uc2ue0T.upload(transposed(pad_cols(O_.uc2ue0, K1_, O_.rank_u, ru_), K1_, ru_), &total);  // ru x K1
+    a->uc2ue1T.upload(transposed(pad_rows(O_.uc2ue1, O_.rank_u, K0_, ru_), ru_, K0_), &total);  // K0 x ru
+    a->dc2de0T.upload(transposed(pad_cols(O_.dc2de0, K1_, O_.rank_d, rd_), K1_, rd_), &total);  // rd x K1
+    a->dc2de1T.upload(transposed(pad_rows(O_.dc2de1, O_.rank_d, K0_, rd_), rd_, K0_), &total);  // K0 x rd
+    a->u2uT.upload(transposed(conv(O_.u2u), 8 * K0_, K0_), &total);                            // K0 x 8K0
+    a->d2dT.upload(transposed(conv(O_.d2d), K1_, 8 * K1_), &total);                            // 8K1 x K1
+    a->tden.alloc((int64_t)k1_ * n_trg_ * nvb_, &total);
+    a->sval.alloc((int64_t)k0_ * n_src_ * nvb_, &total);
+    {
+      std::vector<int32_t> iota(std::max<int>(n_tleaf_, n_s2m_) + 1);
+      for (size_t i = 0; i < iota.size(); i++) iota[i] = (int32_t)i;
+      a->iota.upload(iota, &total);
+    }
+    // U list
+    {
+      std::vector<int32_t> row, ptr, items;
+      transpose_csr(h_.tleaf, h_.u_ptr, h_.u_items, N, row, ptr, items);
+      a->n_u = (int)row.size();
+      a->u_leaf.upload(row, &total); a->u_ptr.upload(ptr, &total); a->u_items.upload(items, &total);
+    }
+    // W list: the far items after each target leaf's own L2T surface
+    {
+      std::vector<int32_t> key, ptr{0}, items;
+      for (size_t r = 0; r < h_.tleaf.size(); r++) {
+        for (int32_t i = h_.far_ptr[r]; i < h_.far_ptr[r + 1]; i++)
+          if (h_.far_rad[i] == kRad0) items.push_back(h_.far_node[i]);
+        key.push_back(h_.tleaf[r]);
+        ptr.push_back((int32_t)items.size());
+      }
+      std::vector<int32_t> row, tptr, titems;
+      transpose_csr(key, ptr, items, N, row, tptr, titems);
+      a->n_w = (int)row.size();
+      a->w_node.upload(row, &total); a->w_ptr.upload(tptr, &total); a->w_items.upload(titems, &total);
+    }
+    // X list
+    {
+      std::vector<int32_t> row, ptr, items;
+      transpose_csr(h_.x_node, h_.x_ptr, h_.x_items, N, row, ptr, items);
+      a->n_x = (int)row.size();
+      std::vector<int64_t> den(items.size());
+      for (size_t i = 0; i < items.size(); i++) den[i] = (int64_t)items[i] * K1_;
+      a->x_leaf.upload(row, &total); a->x_ptr.upload(ptr, &total); a->x_node.upload(items, &total);
+      a->x_den.upload(den, &total);
+      a->x_rad.upload(std::vector<double>(items.size(), kRad0), &total);
+    }
+    // S2U: row r reads its own check potential, chk row r
+    {
+      std::vector<int64_t> den(n_s2m_);
+      for (int r = 0; r < n_s2m_; r++) den[r] = (int64_t)r * K1_;
+      a->s2u_den.upload(den, &total);
+      a->s2u_rad.upload(std::vector<double>(n_s2m_, kRad1), &total);
+    }
+    // V list: new targets are the source parents, new sources the target parents; the pair
+    // (t, R, s) of the evaluation becomes (s, -R, t), whose offset index is that of 26 - R
+    {
+      const int n_off = n_off_;
+      std::vector<int> neg(n_off, -1);
+      for (int o = 0; o < n_off; o++)
+        for (int o2 = 0; o2 < n_off; o2++)
+          if (h_.offs[o2] == 26 - h_.offs[o]) neg[o] = o2;
+      for (int o = 0; o < n_off; o++)
+        if (neg[o] < 0) throw Error(kErrInternal, "V-list offsets are not closed under negation");
+      std::vector<int32_t> full((size_t)n_vsrc_ * n_off, -1);
+      std::vector<char> has(n_vsrc_, 0);
+      for (int t = 0; t < n_vtrg_; t++)
+        for (int o = 0; o < n_off; o++) {
+          const int32_t sl = h_.v_partner[(size_t)t * n_off + o];
+          if (sl < 0) continue;
+          full[(size_t)sl * n_off + neg[o]] = t;
+          has[sl] = 1;
+        }
+      std::vector<int32_t> partner, fc;
+      std::vector<Real> fscal(n_vtrg_), iscal;
+      const KernelInfo& ki = O_.ker;
+      for (int sl = 0; sl < n_vsrc_; sl++) {
+        if (!has[sl]) continue;
+        partner.insert(partner.end(), full.begin() + (size_t)sl * n_off, full.begin() + (size_t)(sl + 1) * n_off);
+        fc.push_back(h_.v_src_fc[sl]);
+        iscal.push_back((Real)std::pow(2.0, ki.trg_scal * (h_.vsrc_depth[sl] + 1)));
+      }
+      for (int t = 0; t < n_vtrg_; t++) fscal[t] = (Real)std::pow(2.0, ki.src_scal * (h_.vtrg_depth[t] + 1));
+      a->n_vt = (int)fc.size();
+      build_tiles(partner, a->n_vt, n_vtrg_, a->vt, &total);
+      a->v_fc.upload(fc, &total);
+      a->fft_scal.upload(fscal, &total);
+      a->ifft_scal.upload(iscal, &total);
+    }
+    device_bytes_ += total;
+    stats.device_bytes = device_bytes_;
+    adj_ = std::move(a);
+  }
+
+  void adjoint_one(const double* d_g, double* d_out, double den_scale, double out_scale, int nv) {
+    AdjointPlan& a = *adj_;
+    const int ek = O_.ker.exp_kind;  // == the point kernel for the symmetric kernels
+    const int64_t g_vs = (int64_t)k1_ * n_trg_, o_vs = (int64_t)k0_ * n_src_;
+    const int sp = tuning.gemm_split_blocks;
+    // scatter^T: the adjoint input in sorted target order (times the output scale)
+    K::gather_density(st_, n_trg_, k1_, d_g, trg_perm_.p, out_scale, a.tden.p, nv, g_vs);
+    K::fill(st_, eq_.p, eq_vs_ * nv);
+    K::fill(st_, dn_.p, dn_vs_ * nv);
+    K::fill64(st_, a.sval.p, o_vs * nv);
+    // U list^T: target points -> source points of the leaves listing them
+    K::p2p_points(st_, ek, a.n_u, a.u_leaf.p, a.u_ptr.p, a.u_items.p, src_begin_.p, src_end_.p, trg_begin_.p, trg_end_.p,
+                  center_.p, sx64_.p, n_src_, tx64_.p, a.tden.p, n_trg_, a.sval.p, true, nv, g_vs, o_vs);
+    // (L2T P2P)^T: target points -> their leaf's downward equivalent surface (RAD1)
+    K::p2p_surface(st_, ek, n_tleaf_, t_leaf_.p, a.iota.p, t_leaf_.p, trg_begin_.p, trg_end_.p, center_.p, node_size_.p, kRad1,
+                   unit_surf_.p, ns_, tx_.p, a.tden.p, n_trg_, eq_.p, l2t_rc_.p, K0_, true, nv, g_vs, eq_vs_);
+    // (W list M2T)^T: target points -> upward equivalent surfaces (RAD0) of their W sources
+    K::p2p_surface(st_, ek, a.n_w, a.w_node.p, a.w_ptr.p, a.w_items.p, trg_begin_.p, trg_end_.p, center_.p, node_size_.p, kRad0,
+                   unit_surf_.p, ns_, tx_.p, a.tden.p, n_trg_, eq_.p, a.w_node.p, K0_, true, nv, g_vs, eq_vs_);
+    // D2T GEMMs^T: equivalent -> check potential of each target leaf
+    K::gemm_rows(st_, n_l2t_, rd_, K0_, eq_.p, l2t_rc_.p, K0_, a.dc2de1T.p, ctmp_.p, l2t_seq_.p, rd_, nullptr, false,
+                 gemm_work_.p, gemm_work_elems_, sp, nv, eq_vs_, ctmp_vs_);
+    K::gemm_rows(st_, n_l2t_, K1_, rd_, ctmp_.p, l2t_seq_.p, rd_, a.dc2de0T.p, dn_.p, l2t_ra_.p, K1_, l2t_alpha_.p, true,
+                 gemm_work_.p, gemm_work_elems_, sp, nv, ctmp_vs_, dn_vs_);
+    // D2D^T, bottom-up: parent += [8 children] x [D2D_0 .. D2D_7]^T
+    for (int l = max_depth_; l >= 1; l--) {
+      const LevelRows& L = *l2l_[l - 1];
+      K::gemm_rows(st_, L.n, K1_, 8 * K1_, dn_.p, L.rc.p, K1_, a.d2dT.p, dn_.p, L.ra.p, K1_, nullptr, true, gemm_work_.p,
+                   gemm_work_elems_, sp, nv, dn_vs_, dn_vs_);
+    }
+    // V list^T: the M2L of offset -R from the target parents' children to the source parents'
+    // (the spectra buffers swap roles: Y_ holds the new sources, X_ the new targets)
+    for (int v = 0; v < nv; v++) {
+      K::fft_up(st_, n_vtrg_, v_trg_fc_.p, dn_.p + v * dn_vs_, K1_, e1_, m_, lattice_.p, lattice_inv_.p, a.fft_scal.p, Y_.p);
+      K::hadamard_tiled(st_, a.vt.n, M_dim_, e0_, e1_, n_off_, a.vt.t0.p, a.vt.u0.p, a.vt.usrc.p, a.vt.uidx.p, a.vt.gmask.p,
+                        v_offsets_.p, V_.p, Y_.p, X_.p, tuning.hadamard_tpb, Vb_.p);
+      K::ifft_down(st_, a.n_vt, a.v_fc.p, X_.p, eq_.p + v * eq_vs_, K0_, e0_, m_, lattice_.p, lattice_inv_.p, ns_,
+                   a.ifft_scal.p);
+    }
+    // (X list S2L)^T: downward check surfaces (RAD0) -> source points
+    K::far_points(st_, ek, a.n_x, a.x_leaf.p, a.x_ptr.p, a.x_node.p, a.x_den.p, a.x_rad.p, src_begin_.p, src_end_.p,
+                  center_.p, node_size_.p, unit_surf_.p, ns_, sx_.p, n_src_, dn_.p, a.sval.p, true, nv, dn_vs_, o_vs);
+    // U2U^T, top-down: [8 children] += parent x [U2U_0 .. U2U_7]^T
+    for (int l = 0; l <= max_depth_ - 1; l++) {
+      const LevelRows& L = *m2m_[l];
+      K::gemm_rows(st_, L.n, 8 * K0_, K0_, eq_.p, L.rc.p, K0_, a.u2uT.p, eq_.p, L.ra.p, K0_, nullptr, true, gemm_work_.p,
+                   gemm_work_elems_, sp, nv, eq_vs_, eq_vs_);
+    }
+    // S2U GEMMs^T: upward equivalent -> check potential of each source leaf
+    K::gemm_rows(st_, n_s2m_, ru_, K0_, eq_.p, s2m_node_.p, K0_, a.uc2ue1T.p, ctmp_.p, s2m_orow_.p, ru_, nullptr, false,
+                 gemm_work_.p, gemm_work_elems_, sp, nv, eq_vs_, ctmp_vs_);
+    K::gemm_rows(st_, n_s2m_, K1_, ru_, ctmp_.p, s2m_orow_.p, ru_, a.uc2ue0T.p, chk_.p, s2m_orow_.p, K1_, s2m_alpha_.p, false,
+                 gemm_work_.p, gemm_work_elems_, sp, nv, ctmp_vs_, chk_vs_);
+    // (S2U P2P)^T: upward check surface (RAD1) -> the leaf's source points
+    K::far_points(st_, ek, n_s2m_, s2m_node_.p, a.iota.p, s2m_node_.p, a.s2u_den.p, a.s2u_rad.p, src_begin_.p, src_end_.p,
+                  center_.p, node_size_.p, unit_surf_.p, ns_, sx_.p, n_src_, chk_.p, a.sval.p, true, nv, chk_vs_, o_vs);
+    // gather^T: back to the input order of the sources (times the density scale)
+    K::scatter_output(st_, n_src_, k0_, a.sval.p, src_perm_.p, den_scale, d_out, nv, o_vs);
+  }
+
+ public:
+  bool adjoint_supported() const override { return O_.ker.pt_kind == O_.ker.exp_kind; }
+  void evaluate_adjoint(const double* d_g, double* d_out, int n_vec, double den_scale, double out_scale) override {
+    if (!adjoint_supported())
+      throw Error(kErrUnsupported, std::string("the adjoint evaluation needs a symmetric kernel (laplace_poten, stokes_vel); ") +
+                                       O_.ker.name + " is not");
+    if (n_vec < 1) throw Error(kErrInvalidArgument, "n_vec must be >= 1");
+    if (!adj_) build_adjoint();
+    for (int v = 0; v < n_vec; v += nvb_)
+      adjoint_one(d_g + (int64_t)v * k1_ * n_trg_, d_out + (int64_t)v * k0_ * n_src_, den_scale, out_scale,
+                  std::min(nvb_, n_vec - v));
+  }
+
+ private:
   void evaluate_one(const double* d_den, double* d_out, double den_scale, double out_scale, int nv) {
     const int pk = O_.ker.pt_kind, ek = O_.ker.exp_kind;
     const int64_t den_vs = (int64_t)k0_ * n_src_, val_vs = (int64_t)k1_ * n_trg_;
@@ -663,7 +933,7 @@ class DevicePlan final : public DevicePlanBase {
       K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p + v * eq_vs_, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p, X_.p);
       if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 1], st_));
       if (nv == 1) mark(6);
-      K::hadamard_tiled(st_, n_vtiles_, M_dim_, e0_, e1_, n_off_, vt_t0_.p, vt_u0_.p, vt_usrc_.p, vt_uidx_.p, vt_gmask_.p,
+      K::hadamard_tiled(st_, vt_.n, M_dim_, e0_, e1_, n_off_, vt_.t0.p, vt_.u0.p, vt_.usrc.p, vt_.uidx.p, vt_.gmask.p,
                         v_offsets_.p, V_.p, X_.p, Y_.p, tuning.hadamard_tpb, Vb_.p);
       if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 2], st_));
       if (nv == 1) mark(7);

@@ -35,7 +35,7 @@ struct PlanStats {
   int64_t u_rows = 0, u_items = 0, u_pairs = 0, far_items = 0, far_pairs = 0, s2m_pairs = 0;
   int64_t u_src_points_streamed = 0;
   int64_t device_bytes = 0;
-  int64_t v_tiles = 0, v_tile_partners = 0;
+  int64_t v_tiles = 0, v_tile_partners = 0, v_tile_max_partners = 0;
 };
 
 // Launch shapes an evaluation uses; the defaults suit large trees, and autotune() picks them
@@ -47,7 +47,11 @@ struct Tuning {
 
 class DevicePlanBase {
  public:
+  DevicePlanBase();
   virtual ~DevicePlanBase() = default;
+  // Process-wide unique id of this plan: recorded graphs that bake in a plan's device
+  // addresses are keyed on it, so a tree set up again (a new plan) is never replayed stale.
+  const uint64_t generation;
   // Times each candidate launch shape on this tree (3 evaluations each, the fastest counts)
   // and keeps the best; d_den / d_out as for evaluate (one vector), d_out is overwritten.
   virtual Tuning autotune(const double* d_den, double* d_out) = 0;
@@ -55,6 +59,11 @@ class DevicePlanBase {
   // d_den: user layout (k0 per source, input order) for n_vec vectors with stride k0*n_src;
   // d_out: user layout (k1 per target, input order), overwritten.
   virtual void evaluate(const double* d_den, double* d_out, int n_vec, double den_scale, double out_scale) = 0;
+  // The transpose of evaluate's linear map (same scales): d_g in the user layout of the
+  // targets (k1 per target), d_out in that of the sources (k0 per source), overwritten.  Built
+  // on first use; symmetric kernels only (adjoint_supported()).
+  virtual void evaluate_adjoint(const double* d_g, double* d_out, int n_vec, double den_scale, double out_scale) = 0;
+  virtual bool adjoint_supported() const = 0;
   virtual hipStream_t stream() const = 0;
   virtual bool is_fp32() const = 0;
   // per-phase times of the last evaluation (ms); enabled by set_timing(true)

@@ -11,7 +11,7 @@ namespace pvfmm_amd {
 
 // Interaction list types, numbered as Mat_Type in include/precomp_mat.hpp:21-39.
 enum ListType : int {
-  kS2U = 4, kU2U = 5, kD2D = 6, kD2T = 7, kU0 = 8, kU1 = 9, kU2 = 10, kV = 11, kV1 = 12, kW = 13, kX = 14,
+  kS2U = 4, kU2U = 5, kD2D = 6, kD2T = 7, kU0 = 8, kU1 = 9, kU2 = 10, kV = 11, kW = 12, kX = 13, kV1 = 14,
   kTypeCount = 16
 };
 

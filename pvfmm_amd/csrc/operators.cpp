@@ -7,9 +7,20 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
+#include <ctime>
 #include <limits>
+#include <memory>
+#include <string>
 
 #include <omp.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "device_fmm.hpp"
+#include "kernels.hpp"
+#include "precompute.hpp"
 
 namespace pvfmm_amd {
 
@@ -95,6 +106,15 @@ void pinv_jacobi(const std::vector<double>& A, int r, int c, double cutoff_rel, 
     }
     if (rotations == 0) break;
   }
+  pinv_factors(G, W, r, c, cutoff_rel, F0, F1, rank_out);
+}
+
+// The pseudo-inverse factors from converged Jacobi rotations: G (c rows of length r) holds
+// U S as rows, W (c rows of length c) holds V^T.  Singular values below cutoff_rel * max are
+// dropped (include/fmm_pts.txx:498-503), the kept ones ordered largest first.
+void pinv_factors(const std::vector<double>& G, const std::vector<double>& W, int r, int c, double cutoff_rel,
+                  std::vector<double>& F0, std::vector<double>& F1, int& rank_out) {
+  const double eps = std::numeric_limits<double>::epsilon();
   std::vector<double> s2(c);
   double smax = 0;
   for (int j = 0; j < c; j++) {
@@ -177,8 +197,84 @@ static void dft3_r2c(int n, const std::vector<double>& in, std::vector<std::comp
       }
 }
 
-// ----------------------------------------------------------------------------- build
-void build_operators(Operators& O, int kernel_id, int m, double cutoff_rel, int precision) {
+// ----------------------------------------------------------------------------- device precompute
+namespace {
+
+struct DevArray {
+  void* p = nullptr;
+  DevArray() = default;
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  ~DevArray() { if (p) (void)hipFree(p); }
+  template <typename T>
+  T* alloc(size_t n) {
+    HIP_CHECK(hipMalloc(&p, sizeof(T) * std::max<size_t>(1, n)));
+    return (T*)p;
+  }
+  template <typename T>
+  T* upload(const std::vector<T>& v) {
+    T* d = alloc<T>(v.size());
+    if (!v.empty()) HIP_CHECK(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return d;
+  }
+};
+
+struct DevStream {
+  hipStream_t s = nullptr;
+  DevStream() { HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~DevStream() { if (s) (void)hipStreamDestroy(s); }
+};
+
+// pinv of the r x c device matrix A by Jacobi sweeps on the device; factors as pinv_jacobi
+void pinv_device(hipStream_t s, const double* A, int r, int c, double cutoff_rel, std::vector<double>& F0,
+                 std::vector<double>& F1, int& rank) {
+  const int cp = c + (c & 1);
+  DevArray g, w, f;
+  double* G = g.alloc<double>((size_t)cp * r);
+  double* W = w.alloc<double>((size_t)cp * c);
+  int* flag = f.alloc<int>(1);
+  jacobi_init_device(s, A, r, c, cp, G, W);
+  jacobi_sweeps_device(s, G, W, r, c, cp, flag, 60);
+  std::vector<double> hG((size_t)cp * r), hW((size_t)cp * c);
+  HIP_CHECK(hipMemcpyAsync(hG.data(), G, sizeof(double) * hG.size(), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(hW.data(), W, sizeof(double) * hW.size(), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  pinv_factors(hG, hW, r, c, cutoff_rel, F0, F1, rank);
+}
+
+// C = A (M x K) * B (K x N) on the device (FP64 matrix cores), C rows with stride ldc
+void gemm_device(hipStream_t s, int M, int N, int K, const double* A, const double* B, double* C, int64_t ldc,
+                 const int32_t* iota) {
+  DeviceKernels<double>::gemm_rows(s, M, N, K, A, iota, K, B, C, iota, ldc, nullptr, false);
+}
+
+}  // namespace
+
+bool device_precompute_enabled() {
+  const char* e = std::getenv("PVFMM_PRECOMP");
+  if (e && std::string(e) == "host") return false;
+  int n = 0;
+  return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+// ----------------------------------------------------------------------------- stages
+// Parent surfaces of a unit box centred at 0.5 and the child surfaces of its 8 children.
+static std::vector<double> unit_box_surface(const Operators& O, int which) {
+  const double c0[3] = {0.5, 0.5, 0.5};
+  switch (which) {
+    case 0: return surf_points(O, c0, 1.0, kRad0);  // upward equivalent
+    case 1: return surf_points(O, c0, 1.0, kRad1);  // upward check
+    case 2: return surf_points(O, c0, 1.0, kRad1);  // downward equivalent
+    case 3: return surf_points(O, c0, 1.0, kRad0);  // downward check
+    default: break;
+  }
+  const int c = (which - 4) % 8;  // 4 + c: child upward equivalent, 12 + c: child downward check
+  const double cc[3] = {0.25 + 0.5 * (c & 1), 0.25 + 0.5 * ((c >> 1) & 1), 0.25 + 0.5 * ((c >> 2) & 1)};
+  return surf_points(O, cc, 0.5, kRad0);
+}
+
+// Surface lattice, unit surface offsets and spectrum sizes (no arithmetic of the operators).
+static void operators_geometry(Operators& O, int kernel_id, int m, double cutoff_rel, int precision) {
   if (m < 2 || (m & 1)) throw Error(kErrInvalidArgument, "multipole order must be even and >= 2");
   O = Operators();
   O.m = m;
@@ -192,8 +288,6 @@ void build_operators(Operators& O, int kernel_id, int m, double cutoff_rel, int 
   O.n1 = 2 * m;
   O.nh = m + 1;
   O.M_dim = O.n1 * O.n1 * O.nh;
-  const int ns = O.ns, e0 = O.ker.e0, e1 = O.ker.e1, kind = O.ker.exp_kind;
-
   // Surface lattice in the reference's point order (surface(), include/fmm_pts.txx:54-97):
   // corner point, three face blocks, then the point reflection of the first half.
   O.lattice.clear();
@@ -204,91 +298,289 @@ void build_operators(Operators& O, int kernel_id, int m, double cutoff_rel, int 
   for (int i = 0; i < m - 1; i++) for (int j = 0; j < m - 1; j++) push(i + 1, j, 0);
   const int half = (int)O.lattice.size() / 3;
   for (int i = 0; i < half; i++) push(m - 1 - O.lattice[3 * i], m - 1 - O.lattice[3 * i + 1], m - 1 - O.lattice[3 * i + 2]);
-  O.unit_surf.resize((size_t)ns * 3);
-  for (int i = 0; i < 3 * ns; i++) O.unit_surf[i] = 0.5 * (2.0 * O.lattice[i] - (m - 1)) / (m - 1);
+  O.unit_surf.resize((size_t)O.ns * 3);
+  for (int i = 0; i < 3 * O.ns; i++) O.unit_surf[i] = 0.5 * (2.0 * O.lattice[i] - (m - 1)) / (m - 1);
+  O.v1_offsets.clear();
+  for (int R = 0; R < 27; R++)
+    if (R != 13) O.v1_offsets.push_back(R);
+}
 
-  const double c0[3] = {0.5, 0.5, 0.5};
-  // check -> equivalent pseudo-inverses at box size 1 (include/fmm_pts.txx:478-581)
-  {
-    std::vector<double> ue = surf_points(O, c0, 1.0, kRad0), uc = surf_points(O, c0, 1.0, kRad1);
-    std::vector<double> M((size_t)ns * e0 * ns * e1);
-    kernel_matrix(kind, ue.data(), ns, uc.data(), ns, M.data());
-    pinv_jacobi(M, ns * e0, ns * e1, cutoff_rel, O.uc2ue0, O.uc2ue1, O.rank_u);
-    std::vector<double> de = surf_points(O, c0, 1.0, kRad1), dc = surf_points(O, c0, 1.0, kRad0);
-    kernel_matrix(kind, de.data(), ns, dc.data(), ns, M.data());
-    pinv_jacobi(M, ns * e0, ns * e1, cutoff_rel, O.dc2de0, O.dc2de1, O.rank_d);
+// Check -> equivalent pseudo-inverse factors at box size 1 (include/fmm_pts.txx:478-581).
+static void compute_factors(Operators& O) {
+  const int ns = O.ns, e0 = O.ker.e0, e1 = O.ker.e1, kind = O.ker.exp_kind, K0 = ns * e0, K1 = ns * e1;
+  const std::vector<double> ue = unit_box_surface(O, 0), uc = unit_box_surface(O, 1);
+  const std::vector<double> de = unit_box_surface(O, 2), dc = unit_box_surface(O, 3);
+  if (O.on_device) {
+    DevStream st;
+    DevArray a, b, dm;
+    double* M = dm.alloc<double>((size_t)K0 * K1);
+    build_matrix_device(st.s, kind, a.upload(ue), ns, b.upload(uc), ns, M);
+    pinv_device(st.s, M, K0, K1, O.pinv_rel_cutoff, O.uc2ue0, O.uc2ue1, O.rank_u);
+    DevArray c, d;
+    build_matrix_device(st.s, kind, c.upload(de), ns, d.upload(dc), ns, M);
+    pinv_device(st.s, M, K0, K1, O.pinv_rel_cutoff, O.dc2de0, O.dc2de1, O.rank_d);
+    return;
   }
-  // U2U = (K_ce->pc V S^+) U^T and D2D = V S^+ (U^T K_pe->cc), the association order of
-  // include/fmm_pts.txx:582-643; parent of size 1, child of size 1/2
-  {
-    const int K0 = ns * e0, K1 = ns * e1, ru = O.rank_u, rd = O.rank_d;
-    O.u2u.assign((size_t)8 * K0 * K0, 0.0);
-    O.d2d.assign((size_t)K1 * 8 * K1, 0.0);
-    std::vector<double> uc = surf_points(O, c0, 1.0, kRad1);
-    std::vector<double> de = surf_points(O, c0, 1.0, kRad1);
-    std::vector<double> M((size_t)K0 * K1), T((size_t)K0 * K1), Tu((size_t)K0 * std::max(ru, rd));
+  std::vector<double> M((size_t)K0 * K1);
+  kernel_matrix(kind, ue.data(), ns, uc.data(), ns, M.data());
+  pinv_jacobi(M, K0, K1, O.pinv_rel_cutoff, O.uc2ue0, O.uc2ue1, O.rank_u);
+  kernel_matrix(kind, de.data(), ns, dc.data(), ns, M.data());
+  pinv_jacobi(M, K0, K1, O.pinv_rel_cutoff, O.dc2de0, O.dc2de1, O.rank_d);
+}
+
+// U2U = (K_ce->pc V S^+) U^T and D2D = V S^+ (U^T K_pe->cc), the association order of
+// include/fmm_pts.txx:582-643; parent of size 1, child of size 1/2.
+static void compute_translations(Operators& O) {
+  const int ns = O.ns, e0 = O.ker.e0, e1 = O.ker.e1, kind = O.ker.exp_kind, K0 = ns * e0, K1 = ns * e1;
+  const int ru = O.rank_u, rd = O.rank_d;
+  O.u2u.assign((size_t)8 * K0 * K0, 0.0);
+  O.d2d.assign((size_t)K1 * 8 * K1, 0.0);
+  const std::vector<double> uc = unit_box_surface(O, 1), de = unit_box_surface(O, 2);
+  if (O.on_device) {
+    DevStream st;
+    const hipStream_t s = st.s;
+    std::vector<int32_t> iota(std::max(K0, K1));
+    for (size_t i = 0; i < iota.size(); i++) iota[i] = (int32_t)i;
+    DevArray di, dm, dt, du0, du1, dd0, dd1, du2u, dd2d, duc, dde;
+    const int32_t* I = di.upload(iota);
+    const double* UC = duc.upload(uc);
+    const double* DE = dde.upload(de);
+    double* M = dm.alloc<double>((size_t)K0 * K1);
+    const double* U0 = du0.upload(O.uc2ue0);  // K1 x ru
+    const double* U1 = du1.upload(O.uc2ue1);  // ru x K0
+    const double* D0 = dd0.upload(O.dc2de0);  // K1 x rd
+    const double* D1 = dd1.upload(O.dc2de1);  // rd x K0
+    double* T = dt.alloc<double>((size_t)std::max(K0, K1) * std::max(std::max(ru, rd), K1));
+    double* u2u = du2u.alloc<double>((size_t)8 * K0 * K0);
+    double* d2d = dd2d.alloc<double>((size_t)K1 * 8 * K1);
     for (int c = 0; c < 8; c++) {
-      double cc[3] = {0.25 + 0.5 * (c & 1), 0.25 + 0.5 * ((c >> 1) & 1), 0.25 + 0.5 * ((c >> 2) & 1)};
-      std::vector<double> ce = surf_points(O, cc, 0.5, kRad0);  // child upward equivalent
-      kernel_matrix(kind, ce.data(), ns, uc.data(), ns, M.data());
-      matmul(M.data(), O.uc2ue0.data(), Tu.data(), K0, K1, ru);
-      matmul(Tu.data(), O.uc2ue1.data(), &O.u2u[(size_t)c * K0 * K0], K0, ru, K0);
-      std::vector<double> cd = surf_points(O, cc, 0.5, kRad0);  // child downward check
-      kernel_matrix(kind, de.data(), ns, cd.data(), ns, M.data());
-      matmul(O.dc2de1.data(), M.data(), Tu.data(), rd, K0, K1);
-      matmul(O.dc2de0.data(), Tu.data(), T.data(), K1, rd, K1);
-      for (int i = 0; i < K1; i++)
-        std::memcpy(&O.d2d[(size_t)i * 8 * K1 + (size_t)c * K1], &T[(size_t)i * K1], sizeof(double) * K1);
+      DevArray dce, dcd;
+      build_matrix_device(s, kind, dce.upload(unit_box_surface(O, 4 + c)), ns, UC, ns, M);
+      gemm_device(s, K0, ru, K1, M, U0, T, ru, I);
+      gemm_device(s, K0, K0, ru, T, U1, u2u + (size_t)c * K0 * K0, K0, I);
+      build_matrix_device(s, kind, DE, ns, dcd.upload(unit_box_surface(O, 12 + c)), ns, M);
+      gemm_device(s, rd, K1, K0, D1, M, T, K1, I);
+      gemm_device(s, K1, K1, rd, D0, T, d2d + (size_t)c * K1, 8 * K1, I);
+      HIP_CHECK(hipStreamSynchronize(s));  // the child surfaces are freed at the end of the iteration
     }
+    HIP_CHECK(hipMemcpyAsync(O.u2u.data(), u2u, sizeof(double) * O.u2u.size(), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(O.d2d.data(), d2d, sizeof(double) * O.d2d.size(), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return;
   }
-  // M2L transfer spectra.  Child boxes of size 1: equivalent and check surfaces share the
-  // lattice spacing h = RAD0/(m-1).  For source-minus-target child offset r, the kernel of
-  // lattice displacement w = v - u is K(w h - r), stored at w mod n1 (circular convolution).
-  {
-    const int n1 = O.n1, M_dim = O.M_dim, ne = e0 * e1;
-    const double h = kRad0 / (m - 1);
-    std::vector<std::vector<std::complex<double>>> spec(343);  // offsets r in [-3,3]^3
-    #pragma omp parallel for schedule(dynamic, 1)
-    for (int ri = 0; ri < 343; ri++) {
-      int rx = ri % 7 - 3, ry = (ri / 7) % 7 - 3, rz = ri / 49 - 3;
-      if (std::max(std::abs(rx), std::max(std::abs(ry), std::abs(rz))) < 2) continue;
-      std::vector<std::vector<double>> grid(ne, std::vector<double>((size_t)n1 * n1 * n1, 0.0));
-      double blk[9];
-      for (int wz = -(m - 1); wz <= m - 1; wz++)
-        for (int wy = -(m - 1); wy <= m - 1; wy++)
-          for (int wx = -(m - 1); wx <= m - 1; wx++) {
-            double r[3] = {wx * h - rx, wy * h - ry, wz * h - rz};
-            eval_pair(kind, r, blk);
-            size_t g = ((size_t)((wz + n1) % n1) * n1 + (wy + n1) % n1) * n1 + (wx + n1) % n1;
-            for (int q = 0; q < ne; q++) grid[q][g] = blk[q];
-          }
-      std::vector<std::complex<double>> s, all((size_t)ne * M_dim);
-      for (int q = 0; q < ne; q++) {
-        dft3_r2c(n1, grid[q], s);
-        std::copy(s.begin(), s.end(), all.begin() + (size_t)q * M_dim);
-      }
-      spec[ri] = std::move(all);
-    }
-    const double inv_n3 = 1.0 / ((double)n1 * n1 * n1);
-    const int bw = 8 * e1, bh = 8 * e0;  // block: rows t*e1+j, cols s*e0+i
-    O.v1.assign((size_t)27 * M_dim * bw * bh, 0.0);
-    O.v1_offsets.clear();
-    for (int R = 0; R < 27; R++) {
-      if (R == 13) continue;
-      O.v1_offsets.push_back(R);
-      int Rx = R % 3 - 1, Ry = (R / 3) % 3 - 1, Rz = R / 9 - 1;
-      for (int t = 0; t < 8; t++)
-        for (int s = 0; s < 8; s++) {
-          int rx = 2 * Rx + (s & 1) - (t & 1), ry = 2 * Ry + ((s >> 1) & 1) - ((t >> 1) & 1), rz = 2 * Rz + ((s >> 2) & 1) - ((t >> 2) & 1);
-          if (std::max(std::abs(rx), std::max(std::abs(ry), std::abs(rz))) < 2) continue;
-          const auto& sp = spec[(rz + 3) * 49 + (ry + 3) * 7 + (rx + 3)];
-          for (int i = 0; i < e0; i++)
-            for (int j = 0; j < e1; j++)
-              for (int f = 0; f < M_dim; f++)
-                O.v1[(((size_t)R * M_dim + f) * bw + (t * e1 + j)) * bh + (s * e0 + i)] = sp[(size_t)(i * e1 + j) * M_dim + f] * inv_n3;
+  std::vector<double> M((size_t)K0 * K1), T((size_t)K0 * K1), Tu((size_t)K0 * std::max(ru, rd));
+  for (int c = 0; c < 8; c++) {
+    const std::vector<double> ce = unit_box_surface(O, 4 + c);  // child upward equivalent
+    kernel_matrix(kind, ce.data(), ns, uc.data(), ns, M.data());
+    matmul(M.data(), O.uc2ue0.data(), Tu.data(), K0, K1, ru);
+    matmul(Tu.data(), O.uc2ue1.data(), &O.u2u[(size_t)c * K0 * K0], K0, ru, K0);
+    const std::vector<double> cd = unit_box_surface(O, 12 + c);  // child downward check
+    kernel_matrix(kind, de.data(), ns, cd.data(), ns, M.data());
+    matmul(O.dc2de1.data(), M.data(), Tu.data(), rd, K0, K1);
+    matmul(O.dc2de0.data(), Tu.data(), T.data(), K1, rd, K1);
+    for (int i = 0; i < K1; i++)
+      std::memcpy(&O.d2d[(size_t)i * 8 * K1 + (size_t)c * K1], &T[(size_t)i * K1], sizeof(double) * K1);
+  }
+}
+
+// M2L transfer spectra (include/fmm_pts.txx:671-764).  Child boxes of size 1: equivalent and
+// check surfaces share the lattice spacing h = RAD0/(m-1).  For source-minus-target child offset
+// r, the kernel of lattice displacement w = v - u is K(w h - r), stored at w mod n1 (circular
+// convolution).
+static void compute_v1(Operators& O) {
+  const int m = O.m, e0 = O.ker.e0, e1 = O.ker.e1, kind = O.ker.exp_kind;
+  const int n1 = O.n1, M_dim = O.M_dim, ne = e0 * e1;
+  const double h = kRad0 / (m - 1);
+  std::vector<std::vector<std::complex<double>>> spec(343);  // offsets r in [-3,3]^3
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int ri = 0; ri < 343; ri++) {
+    int rx = ri % 7 - 3, ry = (ri / 7) % 7 - 3, rz = ri / 49 - 3;
+    if (std::max(std::abs(rx), std::max(std::abs(ry), std::abs(rz))) < 2) continue;
+    std::vector<std::vector<double>> grid(ne, std::vector<double>((size_t)n1 * n1 * n1, 0.0));
+    double blk[9];
+    for (int wz = -(m - 1); wz <= m - 1; wz++)
+      for (int wy = -(m - 1); wy <= m - 1; wy++)
+        for (int wx = -(m - 1); wx <= m - 1; wx++) {
+          double r[3] = {wx * h - rx, wy * h - ry, wz * h - rz};
+          eval_pair(kind, r, blk);
+          size_t g = ((size_t)((wz + n1) % n1) * n1 + (wy + n1) % n1) * n1 + (wx + n1) % n1;
+          for (int q = 0; q < ne; q++) grid[q][g] = blk[q];
         }
+    std::vector<std::complex<double>> sp, all((size_t)ne * M_dim);
+    for (int q = 0; q < ne; q++) {
+      dft3_r2c(n1, grid[q], sp);
+      std::copy(sp.begin(), sp.end(), all.begin() + (size_t)q * M_dim);
+    }
+    spec[ri] = std::move(all);
+  }
+  const double inv_n3 = 1.0 / ((double)n1 * n1 * n1);
+  const int bw = 8 * e1, bh = 8 * e0;  // block: rows t*e1+j, cols s*e0+i
+  O.v1.assign((size_t)27 * M_dim * bw * bh, 0.0);
+  for (int R = 0; R < 27; R++) {
+    if (R == 13) continue;
+    int Rx = R % 3 - 1, Ry = (R / 3) % 3 - 1, Rz = R / 9 - 1;
+    for (int t = 0; t < 8; t++)
+      for (int s = 0; s < 8; s++) {
+        int rx = 2 * Rx + (s & 1) - (t & 1), ry = 2 * Ry + ((s >> 1) & 1) - ((t >> 1) & 1), rz = 2 * Rz + ((s >> 2) & 1) - ((t >> 2) & 1);
+        if (std::max(std::abs(rx), std::max(std::abs(ry), std::abs(rz))) < 2) continue;
+        const auto& sp = spec[(rz + 3) * 49 + (ry + 3) * 7 + (rx + 3)];
+        for (int i = 0; i < e0; i++)
+          for (int j = 0; j < e1; j++)
+            for (int f = 0; f < M_dim; f++)
+              O.v1[(((size_t)R * M_dim + f) * bw + (t * e1 + j)) * bh + (s * e0 + i)] = sp[(size_t)(i * e1 + j) * M_dim + f] * inv_n3;
+      }
+  }
+}
+
+// ----------------------------------------------------------------------------- build
+void build_operators(Operators& O, int kernel_id, int m, double cutoff_rel, int precision) {
+  operators_geometry(O, kernel_id, m, cutoff_rel, precision);
+  O.on_device = device_precompute_enabled();
+  compute_factors(O);
+  compute_translations(O);
+  compute_v1(O);
+}
+
+// ----------------------------------------------------------------------------- cache file
+// The container of PrecompMat::Save2File / LoadFile (include/precomp_mat.txx:212-347): int
+// sizeof(T), int scale_invar, then for each of the 16 Mat_Type values (precomp_mat.hpp:21-39)
+// int n and n matrices, each int rows, int cols and rows*cols values of T.
+namespace {
+enum : int { kUC2UE0 = 0, kUC2UE1 = 1, kDC2DE0 = 2, kDC2DE1 = 3, kU2UT = 5, kD2DT = 6, kV1T = 14, kMatTypes = 16 };
+struct Mat {
+  int rows = 0, cols = 0;
+  std::vector<double> v;
+};
+}  // namespace
+
+static bool read_container(const std::string& path, std::vector<std::vector<Mat>>& mats) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  std::unique_ptr<FILE, int (*)(FILE*)> guard(f, &std::fclose);
+  int32_t sz = 0, scale_invar = 0;
+  if (std::fread(&sz, 4, 1, f) != 1 || std::fread(&scale_invar, 4, 1, f) != 1) return false;
+  if (sz != (int32_t)sizeof(double) || scale_invar != 1) return false;  // FP64 data of a scale-invariant kernel
+  mats.assign(kMatTypes, {});
+  for (int t = 0; t < kMatTypes; t++) {
+    int32_t n = 0;
+    if (std::fread(&n, 4, 1, f) != 1 || n < 0 || n > 4096) return false;
+    mats[t].resize(n);
+    for (int j = 0; j < n; j++) {
+      int32_t r = 0, c = 0;
+      if (std::fread(&r, 4, 1, f) != 1 || std::fread(&c, 4, 1, f) != 1 || r < 0 || c < 0) return false;
+      Mat& M = mats[t][j];
+      M.rows = r;
+      M.cols = c;
+      M.v.resize((size_t)r * c);
+      if ((size_t)r * c && std::fread(M.v.data(), sizeof(double), M.v.size(), f) != M.v.size()) return false;
     }
   }
+  return true;
+}
+
+static void write_container(const std::string& path, const std::vector<std::vector<Mat>>& mats) {
+  const std::string tmp = path + ".tmp" + std::to_string((long long)std::time(nullptr));
+  FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return;  // an unwritable directory only costs the next run its precompute
+  bool ok = true;
+  auto put = [&](const void* p, size_t sz, size_t n) { ok = ok && std::fwrite(p, sz, n, f) == n; };
+  const int32_t sz = (int32_t)sizeof(double), scale_invar = 1;
+  put(&sz, 4, 1);
+  put(&scale_invar, 4, 1);
+  for (int t = 0; t < kMatTypes; t++) {
+    const int32_t n = t < (int)mats.size() ? (int32_t)mats[t].size() : 0;
+    put(&n, 4, 1);
+    for (int j = 0; j < n; j++) {
+      const Mat& M = mats[t][j];
+      const int32_t r = M.rows, c = M.cols;
+      put(&r, 4, 1);
+      put(&c, 4, 1);
+      if (!M.v.empty()) put(M.v.data(), sizeof(double), M.v.size());
+    }
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (ok) ok = std::rename(tmp.c_str(), path.c_str()) == 0;
+  if (!ok) std::remove(tmp.c_str());
+}
+
+std::string precomp_file_name(const Operators& O, int precision) {
+  // Precomp_<ker_name>_m<m>.data as the reference names its FP64 files (include/fmm_pts.txx:
+  // 244-262); the mixed evaluator's operators (FP64 values, FP32 cutoff) get their own suffix
+  return std::string("Precomp_") + O.ker.name + "_m" + std::to_string(O.m) + (precision ? "_mixed" : "") + ".data";
+}
+
+void build_operators_cached(Operators& O, int kernel_id, int m, double cutoff_rel, int precision, const char* dir) {
+  if (!dir || !*dir || cutoff_rel > 0) {  // custom cutoffs are not cached
+    build_operators(O, kernel_id, m, cutoff_rel, precision);
+    return;
+  }
+  operators_geometry(O, kernel_id, m, cutoff_rel, precision);
+  O.on_device = device_precompute_enabled();
+  const std::string path = std::string(dir) + "/" + precomp_file_name(O, precision);
+  const int K0 = O.K0(), K1 = O.K1(), M_dim = O.M_dim, bw = 8 * O.ker.e1, bh = 8 * O.ker.e0;
+  std::vector<std::vector<Mat>> mats;
+  const bool exists = read_container(path, mats);
+  // factors: F0 (K1 x r), F1 (r x K0) with the same r; translations: 8 + 8 full blocks;
+  // spectra: 27 entries, 26 of M_dim x (2 bw bh)
+  bool have_f = exists && mats[kUC2UE0].size() == 1 && mats[kUC2UE1].size() == 1 && mats[kDC2DE0].size() == 1 &&
+                mats[kDC2DE1].size() == 1;
+  if (have_f) {
+    const Mat &a = mats[kUC2UE0][0], &b = mats[kUC2UE1][0], &c = mats[kDC2DE0][0], &d = mats[kDC2DE1][0];
+    have_f = a.rows == K1 && b.cols == K0 && a.cols == b.rows && a.cols > 0 && c.rows == K1 && d.cols == K0 &&
+             c.cols == d.rows && c.cols > 0;
+  }
+  bool have_t = have_f && mats[kU2UT].size() == 8 && mats[kD2DT].size() == 8;
+  for (int c = 0; have_t && c < 8; c++)
+    have_t = mats[kU2UT][c].rows == K0 && mats[kU2UT][c].cols == K0 && mats[kD2DT][c].rows == K1 && mats[kD2DT][c].cols == K1;
+  bool have_v = exists && mats[kV1T].size() == 27;
+  for (int R = 0; have_v && R < 27; R++)
+    have_v = R == 13 ? mats[kV1T][R].v.empty() : (mats[kV1T][R].rows == M_dim && mats[kV1T][R].cols == 2 * bw * bh);
+  if (have_f) {
+    O.uc2ue0 = mats[kUC2UE0][0].v; O.uc2ue1 = mats[kUC2UE1][0].v; O.rank_u = mats[kUC2UE0][0].cols;
+    O.dc2de0 = mats[kDC2DE0][0].v; O.dc2de1 = mats[kDC2DE1][0].v; O.rank_d = mats[kDC2DE0][0].cols;
+  } else {
+    compute_factors(O);
+  }
+  if (have_t) {
+    O.u2u.resize((size_t)8 * K0 * K0);
+    O.d2d.resize((size_t)K1 * 8 * K1);
+    for (int c = 0; c < 8; c++) {
+      std::copy(mats[kU2UT][c].v.begin(), mats[kU2UT][c].v.end(), O.u2u.begin() + (size_t)c * K0 * K0);
+      for (int i = 0; i < K1; i++)
+        std::copy(mats[kD2DT][c].v.begin() + (size_t)i * K1, mats[kD2DT][c].v.begin() + (size_t)(i + 1) * K1,
+                  O.d2d.begin() + (size_t)i * 8 * K1 + (size_t)c * K1);
+    }
+  } else {
+    compute_translations(O);
+  }
+  if (have_v) {
+    O.v1.assign((size_t)27 * M_dim * bw * bh, 0.0);
+    for (int R = 0; R < 27; R++)
+      if (R != 13) std::memcpy(&O.v1[(size_t)R * M_dim * bw * bh], mats[kV1T][R].v.data(), sizeof(double) * mats[kV1T][R].v.size());
+  } else {
+    compute_v1(O);
+  }
+  O.from_cache = have_f && have_t && have_v;
+  if (exists) return;  // an existing file is never replaced (PrecompMat::Save2File with replace = false)
+  std::vector<std::vector<Mat>> out(kMatTypes);
+  out[kUC2UE0] = {Mat{K1, O.rank_u, O.uc2ue0}};
+  out[kUC2UE1] = {Mat{O.rank_u, K0, O.uc2ue1}};
+  out[kDC2DE0] = {Mat{K1, O.rank_d, O.dc2de0}};
+  out[kDC2DE1] = {Mat{O.rank_d, K0, O.dc2de1}};
+  for (int c = 0; c < 8; c++) {
+    out[kU2UT].push_back(Mat{K0, K0, std::vector<double>(O.u2u.begin() + (size_t)c * K0 * K0, O.u2u.begin() + (size_t)(c + 1) * K0 * K0)});
+    Mat D{K1, K1, std::vector<double>((size_t)K1 * K1)};
+    for (int i = 0; i < K1; i++)
+      std::copy(O.d2d.begin() + (size_t)i * 8 * K1 + (size_t)c * K1, O.d2d.begin() + (size_t)i * 8 * K1 + (size_t)(c + 1) * K1,
+                D.v.begin() + (size_t)i * K1);
+    out[kD2DT].push_back(std::move(D));
+  }
+  for (int R = 0; R < 27; R++) {
+    if (R == 13) { out[kV1T].push_back(Mat{}); continue; }
+    const double* b = reinterpret_cast<const double*>(&O.v1[(size_t)R * M_dim * bw * bh]);
+    out[kV1T].push_back(Mat{M_dim, 2 * bw * bh, std::vector<double>(b, b + (size_t)2 * M_dim * bw * bh)});
+  }
+  write_container(path, out);
 }
 
 }  // namespace pvfmm_amd

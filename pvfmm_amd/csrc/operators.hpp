@@ -3,6 +3,7 @@
 #pragma once
 #include <complex>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -35,6 +36,9 @@ struct Operators {
   std::vector<std::complex<double>> v1;
   std::vector<int32_t> v1_offsets;  // the 26 offsets R in use
 
+  bool on_device = false;   // the factors and U2U/D2D came from the device precompute
+  bool from_cache = false;  // every operator was read from a precompute file
+
   int K0() const { return ns * ker.e0; }
   int K1() const { return ns * ker.e1; }
 };
@@ -42,6 +46,14 @@ struct Operators {
 // cutoff_rel <= 0 selects the reference's rule 4*eps*max_S with eps of `precision`
 // (0: FP64, 1: FP32 as the reference's float build).
 void build_operators(Operators& O, int kernel_id, int m, double cutoff_rel = 0.0, int precision = 0);
+
+// The same through a precompute file in directory `dir` (no file I/O when dir is null or empty,
+// or for a custom cutoff): Precomp_<ker_name>_m<m>[_mixed].data in the container format of the
+// reference's PrecompMat::Save2File / LoadFile (include/precomp_mat.txx:212-347).  Operators
+// found in the file with the expected shapes are read, the others computed; a missing file is
+// written (atomically, through a temporary name), an existing one is never replaced.
+void build_operators_cached(Operators& O, int kernel_id, int m, double cutoff_rel, int precision, const char* dir);
+std::string precomp_file_name(const Operators& O, int precision);
 
 // Expansion-kernel matrix K[src*e0+i][trg*e1+j] (Kernel::BuildMatrix layout,
 // include/kernel.txx:1059-1135), FP64.
@@ -52,5 +64,12 @@ void kernel_matrix(int kind, const double* src, int n_src, const double* trg, in
 // (rank x r) with pinv = F0 F1.
 void pinv_jacobi(const std::vector<double>& A, int r, int c, double cutoff_rel, std::vector<double>& F0,
                  std::vector<double>& F1, int& rank);
+// The factors from converged rotations (G: c rows U_j s_j of length r, W: c rows of V^T).
+void pinv_factors(const std::vector<double>& G, const std::vector<double>& W, int r, int c, double cutoff_rel,
+                  std::vector<double>& F0, std::vector<double>& F1, int& rank);
+
+// The SVD, BuildMatrix and U2U/D2D products run on the GPU when one is visible (unless the
+// environment sets PVFMM_PRECOMP=host); the host Jacobi path serves machines without one.
+bool device_precompute_enabled();
 
 }  // namespace pvfmm_amd

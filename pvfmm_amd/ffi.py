@@ -68,6 +68,8 @@ def _declare(L):
         "pvfmm_amd_tree_setup": (i, [vp, vp, i]),
         "pvfmm_amd_tree_evaluate": (i, [vp, c_double_p, c_double_p, i]),
         "pvfmm_amd_tree_evaluate_device": (i, [vp, vp, vp, i, i]),
+        "pvfmm_amd_tree_evaluate_adjoint": (i, [vp, c_double_p, c_double_p, i]),
+        "pvfmm_amd_tree_evaluate_adjoint_device": (i, [vp, vp, vp, i, i]),
         "pvfmm_amd_tree_stream": (vp, [vp]),
         "pvfmm_amd_tree_autotune": (i, [vp, vp, vp, c_int32_p, i]),
         "pvfmm_amd_tree_get_tuning": (i, [vp, c_int32_p, i]),
@@ -85,7 +87,7 @@ def _declare(L):
         "pvfmm_amd_tree_set_timing": (i, [vp, i]),
         "pvfmm_amd_tree_phase_times": (i, [vp, c_double_p, i]),
         "pvfmm_amd_phase_name": (ctypes.c_char_p, [i]),
-        "pvfmm_amd_tree_stats": (i, [vp, c_int64_p]),
+        "pvfmm_amd_tree_stats": (i, [vp, c_int64_p, i]),
         "pvfmm_amd_fmm_operator": (i64, [vp, i, c_double_p]),
         "pvfmm_amd_device_count": (i, []),
         "pvfmm_amd_direct": (i, [i, c_double_p, i64, c_double_p, c_double_p, i64, c_double_p]),

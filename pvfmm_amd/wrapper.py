@@ -272,6 +272,37 @@ class PtFMMTree:
                                                            ctypes.c_void_p(out.data_ptr()), n_vec, 0 if sync else 1))
         return out
 
+    def evaluate_adjoint(self, g) -> np.ndarray:
+        """The transpose of evaluate: g (n_vec, k1*n_trg) or (k1*n_trg,) in target order ->
+        A^T g of shape (n_vec, k0*n_src) or (k0*n_src,) in source order (symmetric kernels)."""
+        k0, k1 = self.dims()
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        n_vec = 1 if g.ndim == 1 else g.shape[0]
+        if g.size != n_vec * k1 * self.n_trg:
+            raise ValueError("adjoint input has the wrong size")
+        out = np.zeros((n_vec, k0 * self.n_src)) if g.ndim > 1 else np.zeros(k0 * self.n_src)
+        ffi.check(ffi.lib().pvfmm_amd_tree_evaluate_adjoint(self._h, ffi.dptr(g), ffi.dptr(out), n_vec))
+        return out
+
+    def evaluate_adjoint_device(self, g, out=None, sync: bool = True):
+        """evaluate_adjoint on contiguous float64 GPU tensors; returns the output tensor."""
+        import torch
+
+        k0, k1 = self.dims()
+        if not (g.is_cuda and g.dtype == torch.float64 and g.is_contiguous()):
+            raise ValueError("g must be a contiguous float64 GPU tensor")
+        n_vec = 1 if g.dim() == 1 else g.shape[0]
+        if g.numel() != n_vec * k1 * self.n_trg:
+            raise ValueError("adjoint input tensor has the wrong size")
+        if out is None:
+            shape = (k0 * self.n_src,) if g.dim() == 1 else (n_vec, k0 * self.n_src)
+            out = torch.empty(shape, dtype=torch.float64, device=g.device)
+        torch.cuda.current_stream(g.device).synchronize()
+        ffi.check(ffi.lib().pvfmm_amd_tree_evaluate_adjoint_device(self._h, ctypes.c_void_p(g.data_ptr()),
+                                                                   ctypes.c_void_p(out.data_ptr()), n_vec,
+                                                                   0 if sync else 1))
+        return out
+
     def stream_handle(self) -> int:
         return ffi.lib().pvfmm_amd_tree_stream(self._h) or 0
 
@@ -320,18 +351,24 @@ class PtFMMTree:
         return {ffi.lib().pvfmm_amd_phase_name(i).decode(): float(buf[i]) for i in range(n)}
 
     def stats(self) -> dict:
-        out = np.zeros(22, dtype=np.int64)
-        ffi.check(ffi.lib().pvfmm_amd_tree_stats(self._h, out.ctypes.data_as(ffi.c_int64_p)))
         names = ["n_nodes", "n_leaves", "max_depth", "s2m_rows", "l2t_rows", "m2m_rows", "l2l_rows", "x_rows",
                  "x_items", "v_src_slots", "v_trg_slots", "v_pairs", "u_rows", "u_items", "u_pairs", "far_items",
-                 "far_pairs", "s2m_pairs", "u_src_points_streamed", "device_bytes", "v_tiles", "v_tile_partners"]
-        return dict(zip(names, out.tolist()))
+                 "far_pairs", "s2m_pairs", "u_src_points_streamed", "device_bytes", "v_tiles", "v_tile_partners",
+                 "v_tile_max_partners"]
+        out = np.zeros(len(names), dtype=np.int64)
+        n = ffi.lib().pvfmm_amd_tree_stats(self._h, out.ctypes.data_as(ffi.c_int64_p), len(names))
+        if n < 0:
+            ffi.check(-n)
+        return dict(zip(names, out[:n].tolist()))
 
 
 class PtFMMBatch:
     """Set-up trees of independent point clouds evaluated together: the first evaluation
     records every tree's evaluation into one HIP graph (one branch per tree), later calls with
-    the same tensors replay it with one graph launch.  The trees must stay alive and set up."""
+    the same tensors replay it with one graph launch.  A tree set up again (with another PtFMM
+    or n_vec_max) or retuned makes the next call record the graph again.  Replays are ordered
+    against eager evaluations of the same trees (they share the trees' scratch buffers), also
+    with sync=False.  The trees must stay alive and set up."""
 
     def __init__(self, trees):
         self.trees = list(trees)

@@ -81,3 +81,45 @@ def test_autotune_keeps_results(prec, tol):
     r64 = t.evaluate_device(q)
     t.set_tuning(hadamard_tiles_per_block=1000, gemm_split_blocks=0)
     assert torch.equal(t.evaluate_device(q), r64)
+
+
+def test_batch_records_again_after_setup_and_orders_eager_calls():
+    """A tree of the batch set up again (another PtFMM: a new plan with new device buffers)
+    makes the next batch call record the graph again; replays and eager evaluations of the same
+    trees, queued without synchronizing (sync=False), are ordered (they share the plans'
+    scratch buffers) and each equals its own eager reference."""
+    import torch
+
+    cs = clouds([4000, 12000], 9)
+    fmm6 = P.PtFMM("laplace_poten", 6, "mixed")
+    trees = [P.PtFMMTree(x, x, 64).setup(fmm6) for x, _ in cs]
+    dens = [torch.tensor(q, device="cuda:0") for _, q in cs]
+    outs = [torch.empty_like(d) for d in dens]
+    batch = P.PtFMMBatch(trees)
+    batch.evaluate_device(dens, outs)
+    assert batch.records() == 1
+    trees[1].setup(P.PtFMM("laplace_poten", 8, "mixed"))
+    batch.evaluate_device(dens, outs)
+    assert batch.records() == 2
+    ref = [t.evaluate_device(d.clone()) for t, d in zip(trees, dens)]
+    for o, r in zip(outs, ref):
+        assert torch.equal(o, r)
+    # async: replay, eager evaluation of other densities on tree 0, replay again, all queued
+    other = dens[0] * -2.0 + 1.0
+    ref_other = trees[0].evaluate_device(other.clone())
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for o in outs:
+            o.zero_()
+        o_other = torch.zeros_like(other)
+        torch.cuda.synchronize()
+        batch.evaluate_device(dens, outs, sync=False)
+        trees[0].evaluate_device(other, out=o_other, sync=False)
+        batch.evaluate_device(dens, outs, sync=False)
+        torch.cuda.ExternalStream(batch.stream_handle()).synchronize()
+        for t in trees:
+            torch.cuda.ExternalStream(t.stream_handle()).synchronize()
+        assert torch.equal(o_other, ref_other), rep
+        for o, r in zip(outs, ref):
+            assert torch.equal(o, r), rep
+    assert batch.records() == 2

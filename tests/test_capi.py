@@ -84,8 +84,9 @@ def test_fortran_context_create_and_errors():
 
 def test_list_counts_match_reference_tables():
     L = ffi.lib()
-    # include/interac_list.txx:39-56: U0 56, U1 27, U2 56, V 316, V1 26, W 152, X 152
-    assert [L.pvfmm_amd_list_count(t) for t in (8, 9, 10, 11, 12, 13, 14)] == [56, 27, 56, 316, 26, 152, 152]
+    # include/interac_list.txx:39-56 with the Mat_Type numbering of include/precomp_mat.hpp:21-39
+    # (U0 8, U1 9, U2 10, V 11, W 12, X 13, V1 14): U0 56, U1 27, U2 56, V 316, W 152, X 152, V1 26
+    assert [L.pvfmm_amd_list_count(t) for t in (8, 9, 10, 11, 12, 13, 14)] == [56, 27, 56, 316, 152, 152, 26]
 
 
 def test_python_context_validation():

@@ -107,7 +107,7 @@ def test_gpu_direct_sum_matches_kat(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("mixed", 1e-5)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("mixed", 1e-5)])
 def test_gpu_fmm_matches_fixture(gpu, prec, tol):
     f = load("fmm_small.npz")
     t = P.PtFMMTree(f["x"], f["x"], int(f["max_pts"]))

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+import treecheck
 
 RNG = np.random.default_rng(1234)
 
@@ -147,6 +148,33 @@ def _check_tree_invariants(tr, src, trg, max_pts):
     for i in RNG.choice(len(leaves), min(60, len(leaves)), replace=False):
         touch = np.all((lo <= lo[i] + size[i] + 1e-15) & (lo + size[:, None] >= lo[i] - 1e-15), axis=1)
         assert np.all(np.abs(leaves[touch, 3] - leaves[i, 3]) <= 1)
+    # minimality: no leaf's parent could be coarsened without breaking the split rule or 2:1
+    n_cand, n_balance = treecheck.assert_minimal(nd, max_pts)
+    assert n_cand > 0
+    return n_balance
+
+
+def test_minimality_check_catches_over_refinement():
+    """The minimality check must reject a tree with one needless split: a leaf of a light
+    region subdivided into 8 leaves, far from any finer leaf."""
+    x = RNG.random((400, 3)) * 0.5  # points only in the low octant; the other 7 stay leaves
+    tr = O.OracleTree(x, x, 64)
+    nd = tr.nodes()
+    treecheck.assert_minimal(nd, 64)
+    # split the empty leaf at the far corner of the cube (its neighbours are at most one level
+    # finer by the 2:1 balance, so nothing justifies the split)
+    leaves = np.where(nd[:, 4] == 1)[0]
+    far = max(leaves, key=lambda i: tuple(nd[i, [2, 1, 0]]))
+    assert nd[far, 9] == 0
+    d = int(nd[far, 3]) + 1
+    s = 1 << (30 - d)
+    kids = [[nd[far, 0] + s * (c & 1), nd[far, 1] + s * ((c >> 1) & 1), nd[far, 2] + s * ((c >> 2) & 1),
+             d, 1, far, 0, 0, 0, 0] for c in range(8)]
+    bad = nd.copy()
+    bad[far, 4] = 0
+    bad = np.concatenate([bad, np.array(kids, dtype=np.int64)])
+    with pytest.raises(AssertionError, match="could be coarsened"):
+        treecheck.assert_minimal(bad, 64)
 
 
 @pytest.mark.parametrize("kind,max_pts", [("uniform", 16), ("sphere", 32), ("blob", 8)])
@@ -160,7 +188,8 @@ def test_tree_invariants(kind, max_pts):
     else:
         x = np.clip(RNG.normal(0.3, 0.03, (n, 3)), 0, 1 - 1e-9)
     tr = O.OracleTree(x, x, max_pts)
-    _check_tree_invariants(tr, x, x, max_pts)
+    n_balance = _check_tree_invariants(tr, x, x, max_pts)
+    print(f"{kind}: {n_balance} light internal nodes kept by the 2:1 balance")
 
 
 def _brute_lists(nd):

@@ -7,6 +7,8 @@ import pytest
 
 import oracle as O
 import pvfmm_amd as P
+import treecheck
+from pvfmm_amd import workloads as W
 
 RNG = np.random.default_rng(99)
 LIST_TYPES = [O.S2U, O.U2U, O.D2D, O.D2T, O.U0, O.U1, O.U2, O.W, O.X, O.V1]
@@ -78,7 +80,20 @@ def test_tree_and_lists_bit_exact(seed):
     x = _points(kinds[seed], n, rng)
     max_pts = int(rng.choice([16, 40, 128] if kinds[seed] == "dup" else [4, 16, 40, 128]))
     trg = x if seed % 2 == 0 else _points("uniform", int(rng.integers(50, 3000)), rng)
-    assert_same_tree(x, trg, max_pts)
+    nd = assert_same_tree(x, trg, max_pts)
+    treecheck.assert_minimal(nd, max_pts)
+    treecheck.assert_minimal(P.PtFMMTree(x, trg, max_pts).nodes(), max_pts)
+
+
+@pytest.mark.parametrize("name,n", [("adaptive-grad-2m", 1 << 21), ("ellipsoid-stokes-1m", 1 << 20),
+                                    ("uniform-poten-64k", 1 << 16)])
+def test_tree_and_lists_bit_exact_at_workload_parameters(name, n):
+    """The benchmark point sets at full size (2^21 points of the adaptive sphere/blob/uniform
+    mix, 2^20 on the ellipsoid surface, 65,536 uniform) at their own max_pts: tree, lists and
+    leaf point sets bit-exact against the oracle, and the tree minimal."""
+    c = W.WORKLOADS[name](n) if name != "uniform-poten-64k" else W.uniform_poten_64k()
+    nd = assert_same_tree(c["src"], c["trg"], c["max_pts"])
+    treecheck.assert_minimal(nd, c["max_pts"])
 
 
 def test_tree_with_separate_defining_points():
@@ -205,3 +220,56 @@ def test_product_m2l_spectra_equal_dense(kernel, kid):
                 eq = O.surf_at(m, bs * 1.0, 0, O.RAD0)
                 dense[t] += (X[s].ravel() @ O.kernel_matrix(kid, eq, chk)).reshape(ns, e)
         assert np.linalg.norm(y - dense) / np.linalg.norm(dense) < 1e-12
+
+
+# ----------------------------------------------------------------------------- GPU tree build
+def _tree_arrays(t):
+    return t.nodes(), t.point_order(0), t.point_order(1), {ty: t.lists(ty) for ty in LIST_TYPES}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", list(range(6)) + ["adaptive-grad-2m", "ellipsoid-stokes-1m", "grid-pts"])
+def test_device_tree_matches_host_and_oracle(gpu, monkeypatch, case):
+    """The GPU tree builder (the default on a GPU) against the host builder (PVFMM_TREE=host):
+    nodes, point orders and every list identical; and the device tree against the oracle as in
+    the CPU tests (bit-exact lists and leaf point sets, minimal).  The cases are the seeded
+    distributions of test_tree_and_lists_bit_exact (duplicated points included), the benchmark
+    point sets at full size and a separate tree-defining point set."""
+    import time
+
+    pt = None
+    if isinstance(case, int):
+        rng = np.random.default_rng(1000 + case)
+        kinds = ["uniform", "sphere", "blob", "cubed", "uniform", "dup"]
+        n = int(rng.integers(100, 5000))
+        x = _points(kinds[case], n, rng)
+        max_pts = int(rng.choice([16, 40, 128] if kinds[case] == "dup" else [4, 16, 40, 128]))
+        trg = x if case % 2 == 0 else _points("uniform", int(rng.integers(50, 3000)), rng)
+    elif case == "grid-pts":
+        x = RNG.random((800, 3)) * 0.3
+        g = (np.arange(6) + 0.5) / 6
+        pt = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)[:, ::-1].copy()
+        trg, max_pts = x, 128
+    else:
+        c = W.WORKLOADS[case]()
+        x, trg, max_pts = c["src"], c["trg"], c["max_pts"]
+    monkeypatch.delenv("PVFMM_TREE", raising=False)
+    t0 = time.perf_counter()
+    td = P.PtFMMTree(x, trg, max_pts, pt=pt)
+    t_dev = time.perf_counter() - t0
+    monkeypatch.setenv("PVFMM_TREE", "host")
+    t0 = time.perf_counter()
+    th = P.PtFMMTree(x, trg, max_pts, pt=pt)
+    t_host = time.perf_counter() - t0
+    monkeypatch.delenv("PVFMM_TREE")
+    print(f"{case}: {len(x)} points, {td.num_nodes()} nodes; tree build device {t_dev * 1e3:.1f} ms, host {t_host * 1e3:.1f} ms")
+    nd, od0, od1, ld = _tree_arrays(td)
+    nh, oh0, oh1, lh = _tree_arrays(th)
+    assert np.array_equal(nd, nh)
+    assert np.array_equal(od0, oh0) and np.array_equal(od1, oh1)
+    for ty in LIST_TYPES:
+        assert np.array_equal(ld[ty], lh[ty]), ty
+    if len(x) <= (1 << 21):
+        on = assert_same_tree(x, trg, max_pts, pt=pt)  # the device tree (default) against the oracle
+        if pt is None:
+            treecheck.assert_minimal(on, max_pts)
