@@ -173,12 +173,14 @@ int pvfmm_amd_tree_evaluate_adjoint_device(void* tree, const double* d_g, double
 void* pvfmm_amd_tree_stream(void* tree);
 
 /* Launch-shape tuning of a set-up tree; no counterpart in the reference (its CPU kernels have
- * no launch shapes).  params = {hadamard_tiles_per_block, gemm_split_blocks}
- * (hadamard_tiles_per_block >= 1, used up to 64 by the FP32 kernel; gemm_split_blocks >= 0).
- * autotune times the candidates on this tree with one device-resident density vector d_den
- * (as pvfmm_amd_tree_evaluate_device, n_vec = 1; d_out is overwritten), keeps the fastest and
- * writes the chosen values to params[0..cap).  Returns 0 or an error code. */
-int pvfmm_amd_tree_autotune(void* tree, const double* d_den, double* d_out, int* params, int cap);
+ * no launch shapes).  params = {hadamard_tiles_per_block, gemm_split_blocks,
+ * hadamard_vec_targets_per_wave} (tiles per block >= 1, used up to 64 by the FP32 kernel;
+ * split blocks >= 0; targets per wave >= 1, used by the 16-vector FP32 kernel of plans set up
+ * for 8 or more vectors).  set_tuning takes 2 or 3 values.  autotune times the candidates on
+ * this tree with n_vec device-resident density vectors d_den (1 <= n_vec <= n_vec_max, as
+ * pvfmm_amd_tree_evaluate_device; d_out is overwritten), keeps the fastest and writes the
+ * chosen values to params[0..cap).  Returns 0 or an error code. */
+int pvfmm_amd_tree_autotune(void* tree, const double* d_den, double* d_out, int n_vec, int* params, int cap);
 int pvfmm_amd_tree_get_tuning(void* tree, int* params, int cap);
 int pvfmm_amd_tree_set_tuning(void* tree, const int* params, int n);
 

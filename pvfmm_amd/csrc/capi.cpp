@@ -132,7 +132,8 @@ struct BatchHandle {
   int records = 0;
   std::vector<std::pair<int, int>> tunings() const {
     std::vector<std::pair<int, int>> v;
-    for (TreeHandle* t : trees) v.emplace_back(t->plan->tuning.hadamard_tpb, t->plan->tuning.gemm_split_blocks);
+    for (TreeHandle* t : trees)
+      v.emplace_back(t->plan->tuning.hadamard_tpb, t->plan->tuning.gemm_split_blocks * 64 + t->plan->tuning.hadamard_vec_tpw);
     return v;
   }
   std::vector<uint64_t> plans() const {
@@ -514,33 +515,35 @@ int pvfmm_amd_tree_evaluate_adjoint(void* tree, const double* g, double* out, in
 }
 void* pvfmm_amd_tree_stream(void* tree) { return tree ? (void*)((TreeHandle*)tree)->stream : nullptr; }
 
-int pvfmm_amd_tree_autotune(void* tree, const double* d_den, double* d_out, int* params, int cap) {
+int pvfmm_amd_tree_autotune(void* tree, const double* d_den, double* d_out, int n_vec, int* params, int cap) {
   return guarded([&] {
     TreeHandle* t = (TreeHandle*)tree;
     if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
     if (!d_den || !d_out) throw Error(kErrInvalidArgument, "null device array");
-    const Tuning tu = t->plan->autotune(d_den, d_out);
+    const Tuning tu = t->plan->autotune(d_den, d_out, n_vec);
     HIP_CHECK(hipStreamSynchronize(t->stream));
-    const int v[2] = {tu.hadamard_tpb, tu.gemm_split_blocks};
-    for (int i = 0; i < std::min(cap, 2); i++) params[i] = v[i];
+    const int v[3] = {tu.hadamard_tpb, tu.gemm_split_blocks, tu.hadamard_vec_tpw};
+    for (int i = 0; i < std::min(cap, 3); i++) params[i] = v[i];
   });
 }
 int pvfmm_amd_tree_get_tuning(void* tree, int* params, int cap) {
   return guarded([&] {
     TreeHandle* t = (TreeHandle*)tree;
     if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
-    const int v[2] = {t->plan->tuning.hadamard_tpb, t->plan->tuning.gemm_split_blocks};
-    for (int i = 0; i < std::min(cap, 2); i++) params[i] = v[i];
+    const int v[3] = {t->plan->tuning.hadamard_tpb, t->plan->tuning.gemm_split_blocks, t->plan->tuning.hadamard_vec_tpw};
+    for (int i = 0; i < std::min(cap, 3); i++) params[i] = v[i];
   });
 }
 int pvfmm_amd_tree_set_tuning(void* tree, const int* params, int n) {
   return guarded([&] {
     TreeHandle* t = (TreeHandle*)tree;
     if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
-    if (n < 2 || !params) throw Error(kErrInvalidArgument, "two tuning values expected");
-    if (params[0] < 1 || params[1] < 0) throw Error(kErrInvalidArgument, "tiles per block >= 1, split blocks >= 0");
+    if (n < 2 || !params) throw Error(kErrInvalidArgument, "at least two tuning values expected");
+    if (params[0] < 1 || params[1] < 0 || (n > 2 && params[2] < 1))
+      throw Error(kErrInvalidArgument, "tiles per block >= 1, split blocks >= 0, targets per wave >= 1");
     t->plan->tuning.hadamard_tpb = params[0];
     t->plan->tuning.gemm_split_blocks = params[1];
+    if (n > 2) t->plan->tuning.hadamard_vec_tpw = params[2];
   });
 }
 

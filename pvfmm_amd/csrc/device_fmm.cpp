@@ -126,14 +126,15 @@ class DevicePlan final : public DevicePlanBase {
                    std::min(nvb_, n_vec - v));
   }
 
-  Tuning autotune(const double* d_den, double* d_out) override {
+  Tuning autotune(const double* d_den, double* d_out, int n_vec) override {
+    if (n_vec < 1 || n_vec > nvb_) throw Error(kErrInvalidArgument, "autotune takes 1 to n_vec_max density vectors");
     const bool was = timing_;
     timing_ = true;
     // fastest per-phase times over a few evaluations with the current tuning
     auto measure = [&]() {
       std::vector<double> best(kPhaseCount, 1e300);
       for (int r = 0; r < 3; r++) {
-        evaluate_one(d_den, d_out, 1.0, 1.0, 1);
+        evaluate_one(d_den, d_out, 1.0, 1.0, n_vec);
         for (int p = 0; p < kPhaseCount; p++) best[p] = std::min(best[p], phase_ms[p]);
       }
       return best;
@@ -151,7 +152,10 @@ class DevicePlan final : public DevicePlanBase {
       knob = best_v;
     };
     try {
-      if (sizeof(Real) == 4 && e0_ == 1 && vt_.n > 0) pick(tuning.hadamard_tpb, {4, 8, 16, 32, 64}, {kPhHadamard});
+      if (sizeof(Real) == 4 && e0_ == 1 && vt_.n > 0) {
+        if (use_vec16(n_vec)) pick(tuning.hadamard_vec_tpw, {4, 8, 16, 32}, {kPhHadamard});
+        else pick(tuning.hadamard_tpb, {4, 8, 16, 32, 64}, {kPhHadamard});
+      }
       pick(tuning.gemm_split_blocks, {0, 256, 512, 1024, 2048}, {kPhS2UGemm, kPhU2U, kPhD2D, kPhD2TGemm});
     } catch (...) {
       timing_ = was;
@@ -201,6 +205,11 @@ class DevicePlan final : public DevicePlanBase {
   // V list
   DBuf<int32_t> v_src_fc_, v_trg_fc_, v_partner_, v_offsets_;
   DBuf<Real> v_src_scal_, v_trg_scal_, X_, Y_;
+  // spectra of a group of 16 density vectors for the 16-vector Hadamard (FP32 scalar kernels,
+  // plans set up for 8 or more vectors): [slot][f][vector 16][16 reals]
+  DBuf<Real> Xv_, Yv_;
+  bool vec16_ = false;
+  bool use_vec16(int nv) const { return vec16_ && nv >= 8; }
   int n_vsrc_ = 0, n_vtrg_ = 0, n_off_ = 0;
 
   // V-list target tiles for the LDS-staged Hadamard
@@ -590,6 +599,16 @@ class DevicePlan final : public DevicePlanBase {
     stats.v_tile_max_partners = vt_.max_partners;
     X_.alloc((int64_t)n_vsrc_ * M_dim_ * 16 * e0_, &total);
     Y_.alloc((int64_t)n_vtrg_ * M_dim_ * 16 * e1_, &total);
+    {
+      // 16-vector spectra when the plan batches 8 or more vectors and they fit in 64 GiB
+      const int64_t bytes = (int64_t)(n_vsrc_ + n_vtrg_) * M_dim_ * 256 * (int64_t)sizeof(Real);
+      vec16_ = std::is_same<Real, float>::value && e0_ == 1 && e1_ == 1 && nvb_ >= 8 && n_vtrg_ > 0 &&
+               bytes <= (int64_t(64) << 30);
+      if (vec16_) {
+        Xv_.alloc((int64_t)n_vsrc_ * M_dim_ * 256, &total);
+        Yv_.alloc((int64_t)n_vtrg_ * M_dim_ * 256, &total);
+      }
+    }
 
     // ---- far field (L2T own surface + W list M2T) and U list per target leaf
     std::vector<int32_t> far_ptr{0}, far_node, u_ptr{0}, u_items;
@@ -924,24 +943,34 @@ class DevicePlan final : public DevicePlanBase {
     K::p2p_surface(st_, ek, n_x_, x_node_.p, x_ptr_.p, x_items_.p, src_begin_.p, src_end_.p, center_.p, node_size_.p, kRad0,
                    unit_surf_.p, ns_, sx_.p, sden_.p, n_src_, dn_.p, x_orow_.p, K1_, true, nv, den_vs, dn_vs_);
     mark(5);
-    // V list, one vector at a time through the spectra buffers (with several vectors the three
-    // sub-phases are timed per vector and summed)
-    if (timing_ && nv > 1) ensure_vlist_events(nv);
-    for (int v = 0; v < nv; v++) {
-      const bool sub = timing_ && nv > 1;
-      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v], st_));
-      K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p + v * eq_vs_, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p, X_.p);
-      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 1], st_));
-      if (nv == 1) mark(6);
-      K::hadamard_tiled(st_, vt_.n, M_dim_, e0_, e1_, n_off_, vt_.t0.p, vt_.u0.p, vt_.usrc.p, vt_.uidx.p, vt_.gmask.p,
-                        v_offsets_.p, V_.p, X_.p, Y_.p, tuning.hadamard_tpb, Vb_.p);
-      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 2], st_));
-      if (nv == 1) mark(7);
-      K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, Y_.p, dn_.p + v * dn_vs_, K1_, e1_, m_, lattice_.p, lattice_inv_.p, ns_,
-                   v_trg_scal_.p);
-      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * v + 3], st_));
+    // V list: groups of 16 vectors through the 16-vector Hadamard, or one vector at a time
+    // through the spectra buffers (with several passes the three sub-phases are timed per
+    // pass and summed)
+    const bool grouped = use_vec16(nv);
+    const int n_pass = grouped ? (nv + 15) / 16 : nv;
+    if (timing_ && n_pass > 1) ensure_vlist_events(n_pass);
+    for (int g = 0; g < n_pass; g++) {
+      const bool sub = timing_ && n_pass > 1;
+      const int v0 = grouped ? 16 * g : g, gn = grouped ? std::min(16, nv - v0) : 1;
+      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * g], st_));
+      for (int v = v0; v < v0 + gn; v++)
+        K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p + v * eq_vs_, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p,
+                  grouped ? Xv_.p : X_.p, grouped ? 16 : 1, grouped ? v - v0 : 0);
+      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * g + 1], st_));
+      if (n_pass == 1) mark(6);
+      if (grouped)
+        K::hadamard_vec16(st_, n_vtrg_, M_dim_, v_partner_.p, Vb_.p, Xv_.p, Yv_.p, tuning.hadamard_vec_tpw);
+      else
+        K::hadamard_tiled(st_, vt_.n, M_dim_, e0_, e1_, n_off_, vt_.t0.p, vt_.u0.p, vt_.usrc.p, vt_.uidx.p, vt_.gmask.p,
+                          v_offsets_.p, V_.p, X_.p, Y_.p, tuning.hadamard_tpb, Vb_.p);
+      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * g + 2], st_));
+      if (n_pass == 1) mark(7);
+      for (int v = v0; v < v0 + gn; v++)
+        K::ifft_down(st_, n_vtrg_, v_trg_fc_.p, grouped ? Yv_.p : Y_.p, dn_.p + v * dn_vs_, K1_, e1_, m_, lattice_.p,
+                     lattice_inv_.p, ns_, v_trg_scal_.p, grouped ? 16 : 1, grouped ? v - v0 : 0);
+      if (sub) HIP_CHECK(hipEventRecord(vev_[4 * g + 3], st_));
     }
-    if (nv > 1) {
+    if (n_pass > 1) {
       mark(6);
       mark(7);
     }
@@ -977,9 +1006,9 @@ class DevicePlan final : public DevicePlanBase {
         HIP_CHECK(hipEventElapsedTime(&ms, ev_[p], ev_[p + 1]));
         phase_ms[p] = ms;
       }
-      if (nv > 1) {
+      if (n_pass > 1) {
         phase_ms[kPhFFT] = phase_ms[kPhHadamard] = phase_ms[kPhIFFT] = 0;
-        for (int v = 0; v < nv; v++)
+        for (int v = 0; v < n_pass; v++)
           for (int q = 0; q < 3; q++) {
             float ms = 0;
             HIP_CHECK(hipEventElapsedTime(&ms, vev_[4 * v + q], vev_[4 * v + q + 1]));

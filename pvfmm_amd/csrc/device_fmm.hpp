@@ -43,6 +43,7 @@ struct PlanStats {
 struct Tuning {
   int hadamard_tpb = 16;        // tiles per block run of the FP32 scalar Hadamard
   int gemm_split_blocks = 1024;  // GEMMs with fewer tile blocks than this split K
+  int hadamard_vec_tpw = 16;    // targets per wave of the 16-vector FP32 scalar Hadamard
 };
 
 class DevicePlanBase {
@@ -53,8 +54,9 @@ class DevicePlanBase {
   // addresses are keyed on it, so a tree set up again (a new plan) is never replayed stale.
   const uint64_t generation;
   // Times each candidate launch shape on this tree (3 evaluations each, the fastest counts)
-  // and keeps the best; d_den / d_out as for evaluate (one vector), d_out is overwritten.
-  virtual Tuning autotune(const double* d_den, double* d_out) = 0;
+  // and keeps the best; d_den / d_out as for evaluate with n_vec vectors (one pass: at most
+  // n_vec_max), d_out is overwritten.
+  virtual Tuning autotune(const double* d_den, double* d_out, int n_vec) = 0;
   Tuning tuning;
   // d_den: user layout (k0 per source, input order) for n_vec vectors with stride k0*n_src;
   // d_out: user layout (k1 per target, input order), overwritten.

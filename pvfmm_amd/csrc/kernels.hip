@@ -699,7 +699,7 @@ __device__ void init_twiddles(Real* twr, Real* twi, int n1, double sign) {
 template <typename Real>
 __global__ __launch_bounds__(512) void k_fft_up(const int32_t* __restrict__ first_child, const Real* __restrict__ up, int K0, int e0,
                                                 int m, int CB, const int32_t* __restrict__ lattice, const Real* __restrict__ scal,
-                                                Real* __restrict__ X) {
+                                                Real* __restrict__ X, int vg, int vi) {
   extern __shared__ unsigned char smem_raw[];
   Real* sm = reinterpret_cast<Real*>(smem_raw);
   const int n1 = 2 * m, nh = m + 1, M_dim = n1 * n1 * nh, ns = 6 * (m - 1) * (m - 1) + 2;
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(512) void k_fft_up(const int32_t* __restrict__ firs
     cube[c * m3 + (z * m + y) * m + x] = up[(int64_t)(fc + c) * K0 + p * e0 + i] * sc;
   }
   __syncthreads();
-  Real* Xs = X + ((int64_t)slot * M_dim * 8 * e0 + c0 * e0 + i) * 2;
+  Real* Xs = X + (((int64_t)slot * M_dim * vg + vi) * 8 * e0 + c0 * e0 + i) * 2;
   for (int kx = 0; kx < nh; kx++) {
     for (int o = tid; o < CB * m2; o += nt) {  // x: real -> complex
       const Real* line = &cube[(o / m2) * m3 + (o % m2) * m];
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(512) void k_fft_up(const int32_t* __restrict__ firs
         ai += vr[z * n1] * twi[e] + vi[z * n1] * twr[e];
       }
       const int64_t f = kx + nh * (ky + n1 * kz);
-      Real* d = Xs + (f * 8 * e0 + c * e0) * 2;
+      Real* d = Xs + (f * vg * 8 * e0 + c * e0) * 2;
       d[0] = ar;
       d[1] = ai;
     }
@@ -776,7 +776,8 @@ __global__ __launch_bounds__(512) void k_fft_up(const int32_t* __restrict__ firs
 template <typename Real>
 __global__ __launch_bounds__(512) void k_ifft_down(const int32_t* __restrict__ first_child, const Real* __restrict__ Y,
                                                    Real* __restrict__ dn, int K1, int e1, int m, int CB,
-                                                   const int32_t* __restrict__ lattice, int ns, const Real* __restrict__ scal) {
+                                                   const int32_t* __restrict__ lattice, int ns, const Real* __restrict__ scal,
+                                                   int vg, int vi) {
   extern __shared__ unsigned char smem_raw[];
   Real* sm = reinterpret_cast<Real*>(smem_raw);
   const int n1 = 2 * m, nh = m + 1, M_dim = n1 * n1 * nh;
@@ -798,13 +799,13 @@ __global__ __launch_bounds__(512) void k_ifft_down(const int32_t* __restrict__ f
   Real acc[MAXQ];
 #pragma unroll
   for (int q = 0; q < MAXQ; q++) acc[q] = 0;
-  const Real* Ys = Y + ((int64_t)slot * M_dim * 8 * e1 + c0 * e1 + j) * 2;
+  const Real* Ys = Y + (((int64_t)slot * M_dim * vg + vi) * 8 * e1 + c0 * e1 + j) * 2;
   __syncthreads();
   for (int kx = 0; kx < nh; kx++) {
     for (int o = tid; o < CB * nn; o += nt) {  // children fastest: contiguous rows of Y
       const int c = o % CB, q = o / CB;         // q = ky + n1*kz
       const int64_t f = kx + (int64_t)nh * q;
-      const Real* src = Ys + (f * 8 * e1 + c * e1) * 2;
+      const Real* src = Ys + (f * vg * 8 * e1 + c * e1) * 2;
       yr[c * nn + q] = src[0];
       yi[c * nn + q] = src[1];
     }
@@ -965,7 +966,7 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
                                                                          const Real* __restrict__ up, int K0, int e0,
                                                                          const int32_t* __restrict__ lat_inv,
                                                                          const Real* __restrict__ scal, Real* __restrict__ X,
-                                                                         int n_slots) {
+                                                                         int n_slots, int vg, int vi) {
   using S = DftShape<Real, M>;
   constexpr int N1 = S::N1, NH = S::NH, MD = S::MD, M2 = S::M2, NT = S::NT, LPT = S::LPT, CG = S::CG, KC = S::KC_UP;
   constexpr int CST = KC * M2 + 1;  // child stride (odd, spreads the children over LDS banks)
@@ -989,7 +990,7 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
     for (int q = 0; q < LPT; q++) v[q][x] = p >= 0 ? up[(int64_t)(fc + cg + CG * q) * K0 + p * e0 + i] * sc : Real(0);
   }
   __syncthreads();
-  Real* Xs = X + ((int64_t)slot * MD * 8 * e0 + i) * 2;
+  Real* Xs = X + (((int64_t)slot * MD * vg + vi) * 8 * e0 + i) * 2;
   for (int k0 = 0; k0 < NH; k0 += KC) {
     const int kn = (NH - k0 < KC) ? NH - k0 : KC;
 #pragma unroll
@@ -1033,8 +1034,8 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
           // out of the unrolled loop (they would not fit in registers for FP64)
           Real* xb = Xs;
           asm volatile("" : "+s"(xb));
-          Real* d0 = xb + (f0 * 8 * e0 + c * e0) * 2;
-          Real* d1 = xb + (f1 * 8 * e0 + c * e0) * 2;
+          Real* d0 = xb + (f0 * vg * 8 * e0 + c * e0) * 2;
+          Real* d1 = xb + (f1 * vg * 8 * e0 + c * e0) * 2;
           d0[0] = er + orr; d0[1] = ei + oi;
           d1[0] = er - orr; d1[1] = ei - oi;
         });
@@ -1095,7 +1096,8 @@ template <typename Real, int M>
 __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_ifft_down_m(const int32_t* __restrict__ first_child,
                                                                             const Real* __restrict__ Y, Real* __restrict__ dn,
                                                                             int K1, int e1, const int32_t* __restrict__ lat_inv,
-                                                                            const Real* __restrict__ scal, int n_slots) {
+                                                                            const Real* __restrict__ scal, int n_slots, int vg,
+                                                                            int vi) {
   using S = DftShape<Real, M>;
   constexpr int N1 = S::N1, NH = S::NH, MD = S::MD, M2 = S::M2, NT = S::NT, LPT = S::LPT, CG = S::CG, KC = S::KC_DN;
   constexpr int PL = N1 * M;         // one plane of one child: [ky][z]
@@ -1118,7 +1120,7 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_ifft_down_m(cons
   for (int q = 0; q < LPT; q++)
 #pragma unroll
     for (int x = 0; x < M; x++) acc[q][x] = 0;
-  const Real* Ys = Y + ((int64_t)slot * MD * 8 * e1 + j) * 2;
+  const Real* Ys = Y + (((int64_t)slot * MD * vg + vi) * 8 * e1 + j) * 2;
   for (int k0 = 0; k0 < NH; k0 += KC) {
     const int kn = (NH - k0 < KC) ? NH - k0 : KC;
     const int ntask = 8 * N1 * kn;
@@ -1127,8 +1129,8 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_ifft_down_m(cons
       Real pr_[M], pi_[M], qr_[M], qi_[M];
 #pragma unroll
       for (int a = 0; a < M; a++) {
-        const Real* s0 = Ys + ((kx + (int64_t)NH * (ky + N1 * a)) * 8 * e1 + c * e1) * 2;
-        const Real* s1 = Ys + ((kx + (int64_t)NH * (ky + N1 * (a + M))) * 8 * e1 + c * e1) * 2;
+        const Real* s0 = Ys + ((kx + (int64_t)NH * (ky + N1 * a)) * vg * 8 * e1 + c * e1) * 2;
+        const Real* s1 = Ys + ((kx + (int64_t)NH * (ky + N1 * (a + M))) * vg * 8 * e1 + c * e1) * 2;
         const Real ur = s0[0], ui = s0[1], vr = s1[0], vi = s1[1];
         pr_[a] = ur + vr; pi_[a] = ui + vi;
         qr_[a] = ur - vr; qi_[a] = ui - vi;
@@ -1658,6 +1660,68 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
   }
 }
 
+// The same products for 16 density vectors of one tree at once: the MFMA rows are the 16
+// vectors of one target (instead of 16 targets of one vector), so the 16 rows of an operand
+// read one partner's spectra, 16 consecutive 64-byte rows of X in the vector-group layout
+// [slot][f][vector 16][16 reals]; no partner table, no padding of absent partners and no LDS
+// staging are needed, the loads are whole 1 KiB blocks straight from L2.  A block owns one
+// frequency (its transfer-block parts in registers) and 4 waves; wave w computes targets
+// [t0 + w tpw, t0 + (w + 1) tpw) in turn, over the 13 offset pairs each, absent partners
+// read as zero.
+template <int NPAIR>
+__global__ __launch_bounds__(256, 2) void k_hadamard_vec16(int M_dim, int n_trg, int tpw, int n_chunks,
+                                                          const int32_t* __restrict__ partner, const uint16_t* __restrict__ Vb,
+                                                          const float* __restrict__ X, float* __restrict__ Y) {
+  constexpr int NOFF = 2 * NPAIR;
+  const int b = blockIdx.x, chunk = b % n_chunks, f = b / n_chunks;
+  if (f >= M_dim) return;  // whole block
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int row = l & 15, kg = l >> 4, pa = kg >> 1, half = kg & 1;
+  bf16x8 bq[NPAIR][3];
+#pragma unroll
+  for (int j = 0; j < NPAIR; j++)
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+      bq[j][p] = *reinterpret_cast<const bf16x8*>(Vb + ((((int64_t)j * 3 + p) * M_dim + f) * 64 + l) * 8);
+  const int t_begin = (chunk * 4 + w) * tpw, t_end = min(n_trg, t_begin + tpw);
+  for (int t = t_begin; t < t_end; t++) {
+    const int32_t* pt = partner + (int64_t)t * NOFF;
+    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < NPAIR; j++) {
+      const int sl = pt[2 * j + pa];
+      floatx4 v0 = {0.f, 0.f, 0.f, 0.f}, v1 = {0.f, 0.f, 0.f, 0.f};
+      if (sl >= 0) {
+        const float* src = X + (((int64_t)sl * M_dim + f) * 16 + row) * 16 + 8 * half;
+        v0 = *reinterpret_cast<const floatx4*>(src);
+        v1 = *reinterpret_cast<const floatx4*>(src + 4);
+      }
+      uint32_t h[3][8];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        split3(v0[q], h[0][q], h[1][q], h[2][q]);
+        split3(v1[q], h[0][q + 4], h[1][q + 4], h[2][q + 4]);
+      }
+      bf16x8 a[3];
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        const u32x4 wd = {pack_hi(h[p][0], h[p][1]), pack_hi(h[p][2], h[p][3]), pack_hi(h[p][4], h[p][5]),
+                          pack_hi(h[p][6], h[p][7])};
+        a[p] = __builtin_bit_cast(bf16x8, wd);
+      }
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], bq[j][0], acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bq[j][0], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bq[j][1], acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bq[j][1], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bq[j][2], acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bq[j][0], acc0, 0, 0, 0);
+    }
+    const floatx4 sum = acc0 + acc1;
+#pragma unroll
+    for (int r = 0; r < 4; r++) Y[(((int64_t)t * M_dim + f) * 16 + 4 * kg + r) * 16 + row] = sum[r];
+  }
+}
+
 // ----------------------------------------------------------------------------- launchers
 static inline int grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
@@ -1848,13 +1912,13 @@ static int fft_children_per_block(size_t per_child, size_t fixed) {
 
 template <typename Real>
 void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0, int m,
-                                 const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X) {
+                                 const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X, int vg, int vi) {
   if (n_slots == 0) return;
   const dim3 g((unsigned)((n_slots + 7) / 8 * 8 * e0));  // whole XCD rounds of slots
 #define PVFMM_FFT_UP_M(MM)                                                                                        \
   case MM:                                                                                                        \
     hipLaunchKernelGGL((k_fft_up_m<Real, MM>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, up, K0, e0, lat_inv, \
-                       scal, X, n_slots);                                                                         \
+                       scal, X, n_slots, vg, vi);                                                                 \
     HIP_LAUNCH_CHECK();                                                                                           \
     return;
   switch (m) {
@@ -1871,7 +1935,7 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
   const int cb = fft_children_per_block<Real>(per_child, fixed);
   const size_t lds = fixed + cb * per_child;
   hipLaunchKernelGGL(k_fft_up<Real>, dim3(n_slots * e0 * (8 / cb)), dim3(512), lds, s, first_child, up, K0, e0, m, cb, lattice,
-                     scal, X);
+                     scal, X, vg, vi);
   HIP_LAUNCH_CHECK();
 }
 
@@ -1927,14 +1991,33 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
 }
 
 template <typename Real>
+void DeviceKernels<Real>::hadamard_vec16(hipStream_t s, int n_trg, int M_dim, const int32_t* partner, const uint16_t* Vb,
+                                         const Real* X, Real* Y, int targets_per_wave) {
+  if (n_trg == 0) return;
+  if constexpr (std::is_same<Real, float>::value) {
+    if (!Vb) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+    const int tpw = std::max(1, targets_per_wave);
+    const int n_chunks = (n_trg + 4 * tpw - 1) / (4 * tpw);
+    const int64_t nb = (int64_t)n_chunks * M_dim;
+    if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+    hipLaunchKernelGGL(k_hadamard_vec16<13>, dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_trg, tpw, n_chunks, partner, Vb,
+                       X, Y);
+    HIP_LAUNCH_CHECK();
+  } else {
+    throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // FP32 scalar kernels only
+  }
+}
+
+template <typename Real>
 void DeviceKernels<Real>::ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn, int K1, int e1,
-                                    int m, const int32_t* lattice, const int32_t* lat_inv, int ns, const Real* scal) {
+                                    int m, const int32_t* lattice, const int32_t* lat_inv, int ns, const Real* scal, int vg,
+                                    int vi) {
   if (n_slots == 0) return;
   const dim3 g((unsigned)((n_slots + 7) / 8 * 8 * e1));  // whole XCD rounds of slots
 #define PVFMM_IFFT_DN_M(MM)                                                                                       \
   case MM:                                                                                                        \
     hipLaunchKernelGGL((k_ifft_down_m<Real, MM>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, Y, dn, K1, e1,     \
-                       lat_inv, scal, n_slots);                                                                   \
+                       lat_inv, scal, n_slots, vg, vi);                                                           \
     HIP_LAUNCH_CHECK();                                                                                           \
     return;
   switch (m) {
@@ -1952,7 +2035,7 @@ void DeviceKernels<Real>::ifft_down(hipStream_t s, int n_slots, const int32_t* f
   if (cb * ns > 24 * 512) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // register accumulators
   const size_t lds = fixed + cb * per_child;
   hipLaunchKernelGGL(k_ifft_down<Real>, dim3(n_slots * e1 * (8 / cb)), dim3(512), lds, s, first_child, Y, dn, K1, e1, m, cb, lattice,
-                     ns, scal);
+                     ns, scal, vg, vi);
   HIP_LAUNCH_CHECK();
 }
 

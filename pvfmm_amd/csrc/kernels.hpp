@@ -60,8 +60,11 @@ struct DeviceKernels {
   // V-list: forward DFT of the 8 children's equivalent densities of each source parent.
   // X layout [slot][f][8*e0] complex.  lattice[3p..3p+2] is the m^3 lattice cell of surface
   // point p; lat_inv[(z*m + y)*m + x] is its inverse (-1 inside the cube).
+  // vg, vi: X holds vector groups of vg density vectors, [slot][f][vector vg][8*e0 complex];
+  // this call writes vector vi of the group (vg = 1, vi = 0: one vector).
   static void fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0,
-                     int m, const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X);
+                     int m, const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X, int vg = 1,
+                     int vi = 0);
   // Y[slot][f][8*e1] = sum_R V[R][f] X[partner[slot][R]][f] on MFMA; Vp is the transfer block
   // in real form, pre-permuted per lane (layout [R][f][kb][nb][lane 64][4]).  The target slots
   // form tiles of at most 64 consecutive slots [tile_t0[k], tile_t0[k+1])
@@ -83,7 +86,13 @@ struct DeviceKernels {
   // inverse DFT, pick the check-surface points, dn[child] += scal * value
   static void ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn,
                         int K1, int e1, int m, const int32_t* lattice, const int32_t* lat_inv, int ns,
-                        const Real* scal);
+                        const Real* scal, int vg = 1, int vi = 0);
+  // The FP32 scalar-kernel Hadamard for a group of 16 density vectors: the MFMA rows are the
+  // 16 vectors of one target, X and Y in the vector-group layout of fft_up / ifft_down with
+  // vg = 16, partner[t * 26 + o] the source slot of target slot t at offset v1_offsets[o] (or
+  // -1), Vb as for hadamard_tiled.  targets_per_wave: targets one wave computes in turn.
+  static void hadamard_vec16(hipStream_t s, int n_trg, int M_dim, const int32_t* partner, const uint16_t* Vb,
+                             const Real* X, Real* Y, int targets_per_wave = 16);
 };
 
 // FP64 all-pairs sum on the device (AoS src[3n], den[k0 n], trg[3n], out[k1 n] device arrays);

@@ -71,7 +71,7 @@ def _declare(L):
         "pvfmm_amd_tree_evaluate_adjoint": (i, [vp, c_double_p, c_double_p, i]),
         "pvfmm_amd_tree_evaluate_adjoint_device": (i, [vp, vp, vp, i, i]),
         "pvfmm_amd_tree_stream": (vp, [vp]),
-        "pvfmm_amd_tree_autotune": (i, [vp, vp, vp, c_int32_p, i]),
+        "pvfmm_amd_tree_autotune": (i, [vp, vp, vp, i, c_int32_p, i]),
         "pvfmm_amd_tree_get_tuning": (i, [vp, c_int32_p, i]),
         "pvfmm_amd_tree_set_tuning": (i, [vp, c_int32_p, i]),
         "pvfmm_amd_batch_create": (vp, [ctypes.POINTER(vp), i]),

@@ -307,29 +307,31 @@ class PtFMMTree:
         return ffi.lib().pvfmm_amd_tree_stream(self._h) or 0
 
     # -- launch-shape tuning
-    TUNING_KEYS = ("hadamard_tiles_per_block", "gemm_split_blocks")
+    TUNING_KEYS = ("hadamard_tiles_per_block", "gemm_split_blocks", "hadamard_vec_targets_per_wave")
 
     def autotune(self, den=None) -> dict:
-        """Times the launch-shape candidates on this tree with one density vector on the GPU
-        (seeded N(0,1) when none is given), keeps the fastest and returns it."""
+        """Times the launch-shape candidates on this tree with density vectors on the GPU (one
+        seeded N(0,1) vector when none is given; a (n_vec, k0*n_src) tensor tunes the batched
+        pass of n_vec <= n_vec_max vectors), keeps the fastest and returns it."""
         import torch
 
         k0, k1 = self.dims()
         if den is None:
             g = torch.Generator(device="cuda:0").manual_seed(0)
             den = torch.randn(k0 * self.n_src, dtype=torch.float64, device="cuda:0", generator=g)
-        if not (den.is_cuda and den.dtype == torch.float64 and den.is_contiguous() and den.numel() == k0 * self.n_src):
-            raise ValueError("den must be one contiguous float64 GPU density vector")
-        out = torch.empty(k1 * self.n_trg, dtype=torch.float64, device=den.device)
+        n_vec = 1 if den.dim() == 1 else den.shape[0]
+        if not (den.is_cuda and den.dtype == torch.float64 and den.is_contiguous() and den.numel() == n_vec * k0 * self.n_src):
+            raise ValueError("den must be contiguous float64 GPU density vectors of k0*n_src values")
+        out = torch.empty(n_vec * k1 * self.n_trg, dtype=torch.float64, device=den.device)
         torch.cuda.current_stream(den.device).synchronize()
-        p = (ctypes.c_int32 * 2)()
+        p = (ctypes.c_int32 * 3)()
         ffi.check(ffi.lib().pvfmm_amd_tree_autotune(self._h, ctypes.c_void_p(den.data_ptr()),
-                                                    ctypes.c_void_p(out.data_ptr()), p, 2))
+                                                    ctypes.c_void_p(out.data_ptr()), n_vec, p, 3))
         return dict(zip(self.TUNING_KEYS, list(p)))
 
     def tuning(self) -> dict:
-        p = (ctypes.c_int32 * 2)()
-        ffi.check(ffi.lib().pvfmm_amd_tree_get_tuning(self._h, p, 2))
+        p = (ctypes.c_int32 * 3)()
+        ffi.check(ffi.lib().pvfmm_amd_tree_get_tuning(self._h, p, 3))
         return dict(zip(self.TUNING_KEYS, list(p)))
 
     def set_tuning(self, **kw):
@@ -338,8 +340,8 @@ class PtFMMTree:
             if k not in cur:
                 raise ValueError(f"unknown tuning key {k}")
             cur[k] = int(v)
-        p = (ctypes.c_int32 * 2)(*[cur[k] for k in self.TUNING_KEYS])
-        ffi.check(ffi.lib().pvfmm_amd_tree_set_tuning(self._h, p, 2))
+        p = (ctypes.c_int32 * 3)(*[cur[k] for k in self.TUNING_KEYS])
+        ffi.check(ffi.lib().pvfmm_amd_tree_set_tuning(self._h, p, 3))
         return self
 
     def set_timing(self, on: bool = True):

@@ -61,7 +61,7 @@ def test_autotune_keeps_results(prec, tol):
     x = rng.random((40000, 3))
     q = torch.tensor(rng.standard_normal(len(x)), device="cuda:0")
     t = P.PtFMMTree(x, x, 64).setup(P.PtFMM("laplace_grad", 6, prec))
-    assert t.tuning() == {"hadamard_tiles_per_block": 16, "gemm_split_blocks": 1024}
+    assert t.tuning() == {"hadamard_tiles_per_block": 16, "gemm_split_blocks": 1024, "hadamard_vec_targets_per_wave": 16}
     before = t.evaluate_device(q)
     chosen = t.autotune(q)
     print(f"{prec}: autotuned {chosen}")
@@ -123,3 +123,33 @@ def test_batch_records_again_after_setup_and_orders_eager_calls():
         for o, r in zip(outs, ref):
             assert torch.equal(o, r), rep
     assert batch.records() == 2
+
+
+@pytest.mark.parametrize("kernel", ["laplace_poten", "laplace_grad"])
+def test_sixteen_vector_pass_and_its_tuning(kernel):
+    """A plan set up for 8 or more vectors runs the V list of a pass through the 16-vector
+    Hadamard (the vectors of one target in the MFMA rows).  Its products and their order are
+    those of the one-vector kernel (absent partners add zero products), so every vector equals
+    its own one-vector evaluation to rounding of the other phases; the nv > 1 autotuner picks a
+    targets-per-wave shape and keeps the results."""
+    import torch
+
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.random((20000, 3)), 0.3 + 0.05 * rng.random((10000, 3))])
+    n = len(x)
+    q = torch.tensor(rng.standard_normal((20, n)), device="cuda:0")
+    fmm = P.PtFMM(kernel, 6, "mixed")
+    tb = P.PtFMMTree(x, x, 64).setup(fmm, n_vec_max=20)
+    t1 = P.PtFMMTree(x, x, 64).setup(fmm, n_vec_max=1)
+    ub = tb.evaluate_device(q)  # one pass: a group of 16 and a group of 4
+    worst = 0.0
+    for v in range(20):
+        u1 = t1.evaluate_device(q[v].contiguous())
+        worst = max(worst, float(torch.linalg.norm(ub[v] - u1) / torch.linalg.norm(u1)))
+    print(f"{kernel}: 20 vectors in one pass vs one at a time, worst rel {worst:.2e}")
+    assert worst < 1e-6
+    chosen = tb.autotune(q[:16].contiguous())
+    print(f"autotuned for 16 vectors: {chosen}")
+    assert chosen["hadamard_vec_targets_per_wave"] in (4, 8, 16, 32)
+    ub2 = tb.evaluate_device(q)
+    assert float(torch.linalg.norm(ub2 - ub) / torch.linalg.norm(ub)) < 1e-6
