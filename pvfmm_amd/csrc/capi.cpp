@@ -655,7 +655,9 @@ int pvfmm_amd_direct(int kernel, const double* src, int64_t n_src, const double*
     struct Buf {
       double* p = nullptr;
       ~Buf() { if (p) (void)hipFree(p); }
-    } ds, dd, dt, dout;
+    } ds, dd, dt, dout, part;
+    const DirectSplit sp = direct_split(ki.pt_kind, n_src, n_trg);
+    if (sp.scratch_bytes) HIP_CHECK(hipMalloc(&part.p, sp.scratch_bytes));
     HIP_CHECK(hipMalloc(&ds.p, sizeof(double) * 3 * n_src));
     HIP_CHECK(hipMalloc(&dd.p, sizeof(double) * ki.k0 * n_src));
     HIP_CHECK(hipMalloc(&dt.p, sizeof(double) * 3 * n_trg));
@@ -663,7 +665,7 @@ int pvfmm_amd_direct(int kernel, const double* src, int64_t n_src, const double*
     HIP_CHECK(hipMemcpy(ds.p, src, sizeof(double) * 3 * n_src, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(dd.p, den, sizeof(double) * ki.k0 * n_src, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(dt.p, trg, sizeof(double) * 3 * n_trg, hipMemcpyHostToDevice));
-    direct_sum(nullptr, ki.pt_kind, ds.p, n_src, dd.p, dt.p, n_trg, dout.p);
+    direct_sum(nullptr, ki.pt_kind, ds.p, n_src, dd.p, dt.p, n_trg, dout.p, part.p);
     HIP_CHECK(hipMemcpy(out, dout.p, sizeof(double) * ki.k1 * n_trg, hipMemcpyDeviceToHost));
   });
 }

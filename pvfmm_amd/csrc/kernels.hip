@@ -13,6 +13,7 @@
 //    three-way operand splits); the inverse DFT evaluates only the check-surface points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -124,11 +125,15 @@ __global__ void k_fill(T* p, int64_t n) {
 }
 
 // ----------------------------------------------------------------------------- direct sum
-// FP64 all-pairs evaluation (the reference's Kernel<T>::ker_poten, include/kernel.hpp:84):
-// one thread per target, sources staged through LDS 256 at a time.  AoS inputs and outputs.
+// FP64 all-pairs evaluation (the reference's Kernel<T>::ker_poten, include/kernel.hpp:84).
+// Block (b, c) takes targets [256 b, 256 b + 256), one per thread, and the sources of chunk c,
+// [c chunk, (c + 1) chunk), staged through LDS 256 at a time.  With one chunk the block writes
+// the scaled result; with several it writes its unscaled partial sums to
+// part[c][k1 n_trg] and k_direct_reduce adds the chunks in chunk order.  AoS inputs and outputs.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_direct(const double* __restrict__ src, int64_t n_src, const double* __restrict__ den,
-                                                const double* __restrict__ trg, int64_t n_trg, double* __restrict__ out) {
+                                                const double* __restrict__ trg, int64_t n_trg, int64_t chunk,
+                                                double* __restrict__ out, double* __restrict__ part) {
   constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1;
   __shared__ double sc[3 * 256];
   __shared__ double sd[K0 * 256];
@@ -138,11 +143,12 @@ __global__ __launch_bounds__(256) void k_direct(const double* __restrict__ src, 
   double u[K1];
 #pragma unroll
   for (int c = 0; c < K1; c++) u[c] = 0.0;
-  for (int64_t j0 = 0; j0 < n_src; j0 += 256) {
+  const int64_t s_begin = blockIdx.y * chunk, s_end = min(n_src, s_begin + chunk);
+  for (int64_t j0 = s_begin; j0 < s_end; j0 += 256) {
     const int64_t j = j0 + threadIdx.x;
-    const int nj = (int)min((int64_t)256, n_src - j0);
+    const int nj = (int)min((int64_t)256, s_end - j0);
     __syncthreads();
-    if (j < n_src) {
+    if (j < s_end) {
       for (int d = 0; d < 3; d++) sc[3 * threadIdx.x + d] = src[3 * j + d];
       for (int c = 0; c < K0; c++) sd[K0 * threadIdx.x + c] = den[K0 * j + c];
     }
@@ -150,22 +156,70 @@ __global__ __launch_bounds__(256) void k_direct(const double* __restrict__ src, 
     for (int jj = 0; jj < nj; jj++)
       interact<KIND, double>(tx - sc[3 * jj], ty - sc[3 * jj + 1], tz - sc[3 * jj + 2], &sd[K0 * jj], u);
   }
-  if (valid)
+  if (!valid) return;
+  if (gridDim.y == 1) {
 #pragma unroll
     for (int c = 0; c < K1; c++) out[K1 * i + c] = u[c] * kscale<KIND>();
+  } else {
+    double* p = part + (int64_t)blockIdx.y * K1 * n_trg + K1 * i;
+#pragma unroll
+    for (int c = 0; c < K1; c++) p[c] = u[c];
+  }
+}
+
+// out[i] = scale * sum over chunks c = 0, 1, ... of part[c][i], i < nv
+template <int KIND>
+__global__ __launch_bounds__(256) void k_direct_reduce(int64_t nv, int n_chunks, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < n_chunks; c++) s += part[(int64_t)c * nv + i];
+    out[i] = s * kscale<KIND>();
+  }
+}
+
+DirectSplit direct_split(int kind, int64_t n_src, int64_t n_trg) {
+  DirectSplit d;
+  if (n_trg <= 0 || n_src <= 0) return d;
+  const int k1 = kind == 0 ? 1 : 3;
+  const int64_t tblocks = (n_trg + 255) / 256;
+  // about 2048 blocks (8 per CU) when the targets alone give fewer; chunks of whole 256-source
+  // stages
+  const int64_t want = std::min<int64_t>((2048 + tblocks - 1) / tblocks, (n_src + 255) / 256);
+  if (want <= 1) {
+    d.chunk = n_src;
+    return d;
+  }
+  d.chunk = ((n_src + want - 1) / want + 255) / 256 * 256;
+  d.n_chunks = (int)((n_src + d.chunk - 1) / d.chunk);
+  d.scratch_bytes = d.n_chunks > 1 ? (size_t)d.n_chunks * k1 * n_trg * sizeof(double) : 0;
+  if (d.n_chunks == 1) d.chunk = n_src;
+  return d;
 }
 
 void direct_sum(hipStream_t s, int kind, const double* src, int64_t n_src, const double* den, const double* trg, int64_t n_trg,
-                double* out) {
+                double* out, double* part) {
   if (n_trg == 0) return;
-  const dim3 g((unsigned)((n_trg + 255) / 256));
-  if (kind == 0)
-    hipLaunchKernelGGL(k_direct<0>, g, dim3(256), 0, s, src, n_src, den, trg, n_trg, out);
-  else if (kind == 1)
-    hipLaunchKernelGGL(k_direct<1>, g, dim3(256), 0, s, src, n_src, den, trg, n_trg, out);
-  else
-    hipLaunchKernelGGL(k_direct<2>, g, dim3(256), 0, s, src, n_src, den, trg, n_trg, out);
-  HIP_LAUNCH_CHECK();
+  const DirectSplit d = direct_split(kind, n_src, n_trg);
+  if (d.n_chunks > 1 && !part) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+  const dim3 g((unsigned)((n_trg + 255) / 256), (unsigned)d.n_chunks);
+  const int64_t nv = (int64_t)(kind == 0 ? 1 : 3) * n_trg;
+  const unsigned gr = (unsigned)std::min<int64_t>((nv + 255) / 256, 4096);
+#define PVFMM_DIRECT(KK)                                                                                     \
+  hipLaunchKernelGGL(k_direct<KK>, g, dim3(256), 0, s, src, n_src, den, trg, n_trg, d.chunk, out, part);     \
+  HIP_LAUNCH_CHECK();                                                                                        \
+  if (d.n_chunks > 1) {                                                                                      \
+    hipLaunchKernelGGL(k_direct_reduce<KK>, dim3(gr), dim3(256), 0, s, nv, d.n_chunks, (const double*)part, out); \
+    HIP_LAUNCH_CHECK();                                                                                      \
+  }
+  if (kind == 0) {
+    PVFMM_DIRECT(0)
+  } else if (kind == 1) {
+    PVFMM_DIRECT(1)
+  } else {
+    PVFMM_DIRECT(2)
+  }
+#undef PVFMM_DIRECT
 }
 
 // ----------------------------------------------------------------------------- P2P, 256-thread leaf blocks

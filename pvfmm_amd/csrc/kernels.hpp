@@ -97,7 +97,17 @@ struct DeviceKernels {
 
 // FP64 all-pairs sum on the device (AoS src[3n], den[k0 n], trg[3n], out[k1 n] device arrays);
 // kind as in DeviceKernels (0 laplace potential, 1 laplace gradient, 2 stokes velocity).
+// The sources are split into n_chunks chunks of `chunk` points when the targets alone would
+// not fill the GPU; the chunks' partial sums go to a scratch buffer of scratch_bytes (0 for one
+// chunk) and are added in chunk order, so a result depends only on (kind, n_src, n_trg) and
+// the inputs, whichever buffer holds the partials.
+struct DirectSplit {
+  int64_t chunk = 0;
+  int n_chunks = 1;
+  size_t scratch_bytes = 0;
+};
+DirectSplit direct_split(int kind, int64_t n_src, int64_t n_trg);
+// part: device buffer of direct_split(...).scratch_bytes (may be null when that is 0)
 void direct_sum(hipStream_t s, int kind, const double* src, int64_t n_src, const double* den, const double* trg, int64_t n_trg,
-                double* out);
-
+                double* out, double* part);
 }  // namespace pvfmm_amd

@@ -116,8 +116,9 @@ def test_precomp_file_with_other_contents_is_kept(cache_dir):
                                            ("stokes_vel", 6, "fp64")])
 def test_device_precompute_matches_host(gpu, monkeypatch, kernel, m, prec):
     """BuildMatrix, the Jacobi SVD and the U2U/D2D products on the device against the host
-    path: the pseudo-inverses (F0 F1) and translations agree to the conditioning of the
-    check-to-equivalent systems, and FMM evaluations with either agree to that level."""
+    path: the translations act alike on the densities sources produce (the matrices
+    themselves may differ in directions the cutoff leaves undetermined), and FMM evaluations
+    with either operator set agree to rounding."""
     import time
 
     monkeypatch.delenv("PVFMM_DIR", raising=False)
@@ -131,11 +132,7 @@ def test_device_precompute_matches_host(gpu, monkeypatch, kernel, m, prec):
     td = time.perf_counter() - t0
     ns = 6 * (m - 1) ** 2 + 2
     K = ns * (3 if kernel == "stokes_vel" else 1)
-    ph = fh.operator("uc2ue0").reshape(K, -1) @ fh.operator("uc2ue1").reshape(-1, K)
-    pd = fd.operator("uc2ue0").reshape(K, -1) @ fd.operator("uc2ue1").reshape(-1, K)
-    e_p = float(np.linalg.norm(pd - ph) / np.linalg.norm(ph))
-    e_u = float(np.linalg.norm(fd.operator("u2u") - fh.operator("u2u")) / np.linalg.norm(fh.operator("u2u")))
-    e_d = float(np.linalg.norm(fd.operator("d2d") - fh.operator("d2d")) / np.linalg.norm(fh.operator("d2d")))
+    e_u, e_d = _translation_action(fh, fd, kernel, m)
     rng = np.random.default_rng(9)
     x = rng.random((6000, 3))
     k0 = 3 if kernel == "stokes_vel" else 1
@@ -143,7 +140,53 @@ def test_device_precompute_matches_host(gpu, monkeypatch, kernel, m, prec):
     uh = P.PtFMMTree(x, x, 64).setup(fh).evaluate(q)
     ud = P.PtFMMTree(x, x, 64).setup(fd).evaluate(q)
     e_fmm = float(np.linalg.norm(ud - uh) / np.linalg.norm(uh))
-    print(f"{kernel} m={m} {prec}: host {th:.2f} s, device {td:.2f} s; pinv {e_p:.2e}, u2u {e_u:.2e}, "
-          f"d2d {e_d:.2e}, FMM {e_fmm:.2e}")
+    print(f"{kernel} m={m} {prec}: host {th:.2f} s, device {td:.2f} s; U2U action {e_u:.2e}, D2D action {e_d:.2e}, "
+          f"FMM {e_fmm:.2e}")
     assert e_fmm < (1e-10 if prec == "fp64" else 1e-6)
-    assert e_u < 1e-6 and e_d < 1e-6
+    assert e_u < 1e-9 and e_d < 1e-9
+
+
+def _translation_action(fh, fd, kernel, m):
+    """The U2U and D2D translations of two operator sets applied to the data they meet in an
+    FMM, compared where it matters.  At m = 10 in FP64 the check-to-equivalent systems have
+    condition numbers near 1e16, so the singular vectors kept just above the 4 eps max_S
+    cutoff are not determined by the matrix: two correct SVDs (a host and a device Jacobi
+    sweep, whose sums round differently) give factors, and translation matrices, that differ by
+    1e-2 in norm.  What is fixed is their action on densities produced by sources:
+      U2U: random sources in each child box -> child equivalent density (child check potential
+           times the depth-1 check-to-equivalent map, half the depth-0 one for these kernels of
+           degree -1) -> U2U -> potential of the parent equivalent density at 200 points at
+           distance 3 from the parent centre;
+      D2D: sources at distance 3 from the parent -> potential on the parent's downward check
+           surface -> D2D -> child check potential.
+    Returns the relative differences of the two sets' results (host as reference)."""
+    import oracle as O
+
+    kid = {"laplace_poten": 0, "laplace_grad": 1, "stokes_vel": 2}[kernel]
+    ke = O.expansion_kernel(kid)
+    k0 = 3 if ke == 2 else 1
+    ns = 6 * (m - 1) ** 2 + 2
+    K = ns * k0
+    rng = np.random.default_rng(31)
+    d = rng.standard_normal((200, 3))
+    far = 0.5 + 3.0 * d / np.linalg.norm(d, axis=1)[:, None]
+    ue = O.surf_at(m, [0, 0, 0], 0, O.RAD0)
+    dc = O.surf_at(m, [0, 0, 0], 0, O.RAD0)
+    ku = np.zeros(2)
+    kd = np.zeros(2)
+    for c in range(8):
+        bits = np.array([c & 1, (c >> 1) & 1, (c >> 2) & 1], dtype=np.float64)
+        y = bits * 0.5 + 0.5 * rng.random((40, 3))
+        qy = rng.standard_normal(40 * k0)
+        child_check = qy @ O.kernel_matrix(ke, y, O.surf_at(m, bits * 0.5, 1, O.RAD1))
+        # the two factors applied in turn (their explicit product would lose digits at m = 10)
+        q_child = 0.5 * ((child_check @ fh.operator("uc2ue0").reshape(K, -1)) @ fh.operator("uc2ue1").reshape(-1, K))
+        kf = O.kernel_matrix(ke, ue, far)
+        ph = q_child @ fh.operator("u2u").reshape(8, K, K)[c] @ kf
+        pd = q_child @ fd.operator("u2u").reshape(8, K, K)[c] @ kf
+        ku += [np.sum((pd - ph) ** 2), np.sum(ph ** 2)]
+        parent_check = qy @ O.kernel_matrix(ke, far[:40], dc)
+        dh = parent_check @ fh.operator("d2d").reshape(K, 8, K)[:, c, :]
+        dd = parent_check @ fd.operator("d2d").reshape(K, 8, K)[:, c, :]
+        kd += [np.sum((dd - dh) ** 2), np.sum(dh ** 2)]
+    return float(np.sqrt(ku[0] / ku[1])), float(np.sqrt(kd[0] / kd[1]))

@@ -123,11 +123,17 @@ def test_adaptive_grad_2m_full_size(gpu):
     assert errs["fp64"] < 1e-8
 
 
-@pytest.mark.parametrize("prec,tol_or", [("mixed", 1e-5), ("fp64", 1e-9)])
+@pytest.mark.parametrize("prec,tol_or", [("mixed", 1e-5), ("fp64", 1e-8)])
 def test_ellipsoid_stokes_sample_vs_oracle(gpu, prec, tol_or):
     """stokes_vel on 2^18 points of the ellipsoid surface (semi-axes 0.45, 0.3, 0.15), m=8,
     max_pts=200, against the oracle on the same tree.  The V-list tiles of this surface hold
-    more than 80 distinct partners (the Stokes tile cap is 160), which the check asserts."""
+    more than 80 distinct partners (the Stokes tile cap is 160), which the check asserts.
+
+    In FP64 the reference's cutoff 4 eps_64 max_S keeps singular values of the Stokes
+    check-to-equivalent systems down to about 1e-16 max_S; the oracle (LAPACK SVD) and this
+    evaluator (its own Jacobi SVD) then differ by a few 1e-9 and both sit at the same rel-L2
+    against the direct sum (2.8e-5 on this sample), the error of the algorithm at that cutoff:
+    the comparison with the oracle's error is the check that matters there."""
     c = W.ellipsoid_stokes_1m(1 << 18)
     tree = P.PtFMMTree(c["src"], c["trg"], c["max_pts"]).setup(P.PtFMM(c["kernel"], c["m"], prec))
     st = tree.stats()
@@ -141,6 +147,7 @@ def test_ellipsoid_stokes_sample_vs_oracle(gpu, prec, tol_or):
     self_check(tree, u, c["src"], c["den"], 2, 3, f"stokes_vel {prec}")
     assert st["v_tile_max_partners"] > 80
     assert e_or < tol_or
+    assert abs(e_l2 - e_l2_or) < 0.05 * e_l2_or
     assert e_l2 < 5e-5
 
 
@@ -199,8 +206,12 @@ def test_energy_1m_gradients(gpu):
     at +h and -h and cancels, so (E(x + h e) - E(x - h e)) / 2h = q_i (phi(x_i + h e) -
     phi(x_i - h e)) / 2h with phi summed over every source.  Its truncation error is about
     (h / r)^2 for a pair at distance r: about 4e-4 for the nearest neighbours of this cloud
-    (r ~ 0.005), so the difference quotient is checked at 1e-3 and the direct-sum gradient at
-    the FMM's own accuracy."""
+    (r ~ 0.005), and among 2^20 points some of the 256 sampled ones have a much closer
+    neighbour, so the quotient itself can be off by 1e-3.  It is checked two ways: the FMM
+    gradient must be as close to the h = 1e-4 quotient as the exact (direct-sum) gradient is,
+    and it must match the Richardson combination (4 D(h/2) - D(h)) / 3 of the central
+    quotients at h and h/2, whose truncation error is O(h^4), at the FMM's accuracy; the
+    direct-sum gradient is checked at the FMM's own accuracy."""
     import torch
 
     from pvfmm_amd.autograd import LaplaceFMM
@@ -224,6 +235,10 @@ def test_energy_1m_gradients(gpu):
     pp = O.direct(0, x, q, x[ii] + shift)
     pm = O.direct(0, x, q, x[ii] - shift)
     fd_x = q[ii] * (pp - pm) / (2 * h)
+    pp2 = O.direct(0, x, q, x[ii] + 0.5 * shift)
+    pm2 = O.direct(0, x, q, x[ii] - 0.5 * shift)
+    fd_x2 = q[ii] * (pp2 - pm2) / h
+    fd_rich = (4 * fd_x2 - fd_x) / 3
     jq = rng.choice(n, 256, replace=False)
     fd_q = O.direct(0, x, q, x[jq])
     g = -q[ii] * O.direct(1, x, q, x[ii]).reshape(-1, 3)[np.arange(256), dd]
@@ -231,6 +246,9 @@ def test_energy_1m_gradients(gpu):
     e_fq = rel(dq[jq], fd_q)
     e_g = rel(dx[ii, dd], g)
     e_fd_vs_g = rel(fd_x, g)
+    e_rich = rel(dx[ii, dd], fd_rich)
     say(f"energy-1m mixed: dE/dx vs FD {e_fx:.2e} (FD vs direct-sum gradient {e_fd_vs_g:.2e}), "
+        f"vs Richardson FD {e_rich:.2e} (its own error vs direct-sum gradient {rel(fd_rich, g):.2e}), "
         f"dE/dx vs -q*laplace_grad {e_g:.2e}, dE/dq vs FD {e_fq:.2e}")
-    assert e_fx < 1e-3 and e_fq < 1e-5 and e_g < 1e-5
+    assert e_fx < e_fd_vs_g + 1e-5 and e_fd_vs_g < 1e-2
+    assert e_rich < 1e-5 and e_fq < 1e-5 and e_g < 1e-5
