@@ -232,6 +232,20 @@ int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out);
 int pvfmm_amd_direct(int kernel, const double* src, int64_t n_src, const double* den, const double* trg, int64_t n_trg,
                      double* out);
 
+/* pvfmm_amd_direct on device arrays with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc).  d_src [3 n_src], d_den [k0 n_src],
+ * d_trg [3 n_trg], d_out [k1 n_trg] are device arrays (AoS); the work is queued on `stream`
+ * (a hipStream_t; NULL is the null stream) and the call returns without waiting.  The
+ * partial sums of the source chunks live in one static 32 MiB device array per GPU; the chunk
+ * rule keeps them within it (at most 19 MB), larger needs are rejected with
+ * PVFMM_ERR_INVALID_ARGUMENT.  Every call on a GPU shares that array and overwrites it, so
+ * calls that overlap on one GPU (other streams, other threads) race.  The first call on a
+ * device looks the array up (hipGetSymbolAddress) and should run outside graph capture.
+ * Results equal pvfmm_amd_direct's bit for bit. */
+int pvfmm_amd_direct_device_noalloc(int kernel, const double* d_src, int64_t n_src, const double* d_den, const double* d_trg,
+                                    int64_t n_trg, double* d_out, void* stream);
+
 /* Number of GPUs visible; 0 when the HIP runtime finds none. */
 int pvfmm_amd_device_count(void);
 

@@ -669,6 +669,16 @@ int pvfmm_amd_direct(int kernel, const double* src, int64_t n_src, const double*
     HIP_CHECK(hipMemcpy(out, dout.p, sizeof(double) * ki.k1 * n_trg, hipMemcpyDeviceToHost));
   });
 }
+int pvfmm_amd_direct_device_noalloc(int kernel, const double* d_src, int64_t n_src, const double* d_den, const double* d_trg,
+                                    int64_t n_trg, double* d_out, void* stream) {
+  return guarded([&] {
+    const KernelInfo& ki = kernel_info(kernel);
+    if (n_src < 0 || n_trg < 0 || (n_src && (!d_src || !d_den)) || (n_trg && (!d_trg || !d_out)))
+      throw Error(kErrInvalidArgument, "bad direct-sum arguments");
+    if (n_trg == 0) return;
+    direct_sum_static(static_cast<hipStream_t>(stream), ki.pt_kind, d_src, n_src, d_den, d_trg, n_trg, d_out);
+  });
+}
 int pvfmm_amd_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;

@@ -110,4 +110,11 @@ DirectSplit direct_split(int kind, int64_t n_src, int64_t n_trg);
 // part: device buffer of direct_split(...).scratch_bytes (may be null when that is 0)
 void direct_sum(hipStream_t s, int kind, const double* src, int64_t n_src, const double* den, const double* trg, int64_t n_trg,
                 double* out, double* part);
+// The same sum with the partials in the library's static device array (direct_static.hip; no
+// allocation): kDirectScratchBytes per device, shared by every call on that device; a larger
+// need is rejected (kErrInvalidArgument).
+constexpr size_t kDirectScratchBytes = (size_t)32 << 20;
+void direct_sum_static(hipStream_t s, int kind, const double* src, int64_t n_src, const double* den, const double* trg,
+                       int64_t n_trg, double* out);
+
 }  // namespace pvfmm_amd

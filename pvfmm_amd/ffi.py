@@ -91,6 +91,7 @@ def _declare(L):
         "pvfmm_amd_fmm_operator": (i64, [vp, i, c_double_p]),
         "pvfmm_amd_device_count": (i, []),
         "pvfmm_amd_direct": (i, [i, c_double_p, i64, c_double_p, c_double_p, i64, c_double_p]),
+        "pvfmm_amd_direct_device_noalloc": (i, [i, vp, i64, vp, vp, i64, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
