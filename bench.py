@@ -400,23 +400,42 @@ def main():
         return
     mixed, fp64 = res["mixed"], res["fp64"]
 
-    # the FP64 evaluator at the smallest m whose error is at most the mixed evaluator's
+    # Equal-error comparison.  The error of each evaluator at m = 6, 8, 10 (FP64: truncation in
+    # m; mixed: truncation or FP32 rounding, whichever is larger) and its time; for an error
+    # level eps each evaluator takes the smallest m whose error is at most eps (the cheapest),
+    # and the ratio of those times is the speedup at equal error.  Reported for eps = the mixed
+    # evaluator's error at the workload's m, and eps = 1e-6.
     eq = None
     if not args.no_equal_error and mixed["err"] is not None:
-        for m_eq in (6, 8, 10):
-            if m_eq == c["m"]:
-                e_eq, step_eq = fp64["err"], fp64["step"]
-            else:
-                tree.setup(P.PtFMM(c["kernel"], m_eq, "fp64"))
+        frontier = {"fp64": {c["m"]: (fp64["step"], fp64["err"])}, "mixed": {c["m"]: (mixed["step"], mixed["err"])}}
+        for prec in ("fp64", "mixed"):
+            for m_eq in (6, 8, 10):
+                if m_eq in frontier[prec]:
+                    continue
+                tree.setup(P.PtFMM(c["kernel"], m_eq, prec))
                 if not args.no_tune:
                     tree.autotune(dq)
                 step_eq, out, _ = time_steps(tree, dq, max(3, args.steps // 2), 1, timing=False)
                 e_eq = rel_l2(out.cpu().numpy().reshape(n, 3)[idx], ud)
-            log(f"fp64 m={m_eq}: {1e3 * step_eq:.2f} ms/eval, rel-L2 {e_eq:.3e}")
-            if e_eq <= mixed["err"] or m_eq == 10:
-                eq = {"m": m_eq, "rel_l2": e_eq, "ms_per_step": 1e3 * step_eq, "mtargets_s": n / step_eq / 1e6,
-                      "mixed_over_fp64": step_eq / mixed["step"], "reached": bool(e_eq <= mixed["err"])}
-                break
+                frontier[prec][m_eq] = (step_eq, e_eq)
+                log(f"{prec} m={m_eq}: {1e3 * step_eq:.2f} ms/eval, rel-L2 {e_eq:.3e}")
+
+        def cheapest(prec, eps):
+            ok = [(t, mm, e) for mm, (t, e) in frontier[prec].items() if e <= eps]
+            return min(ok) if ok else None
+
+        levels = []
+        for eps in (mixed["err"], 1e-6):
+            f64, mx = cheapest("fp64", eps), cheapest("mixed", eps)
+            levels.append({"rel_l2_bound": eps,
+                           "fp64": None if f64 is None else {"m": f64[1], "ms_per_step": 1e3 * f64[0], "rel_l2": f64[2]},
+                           "mixed": None if mx is None else {"m": mx[1], "ms_per_step": 1e3 * mx[0], "rel_l2": mx[2]},
+                           "mixed_over_fp64": None if (f64 is None or mx is None) else f64[0] / mx[0]})
+            log(f"equal error <= {eps:.2e}: fp64 m={f64 and f64[1]}, mixed m={mx and mx[1]}, "
+                f"ratio {levels[-1]['mixed_over_fp64']}")
+        eq = {"frontier": {prec: {str(mm): {"ms_per_step": 1e3 * t, "rel_l2": e} for mm, (t, e) in sorted(d.items())}
+                           for prec, d in frontier.items()},
+              "levels": levels}
     st = mixed["stats"]
     m = c["m"]
     roof = roofline(mixed["phases"], st, m, real_bytes=4)
@@ -457,7 +476,7 @@ def main():
         "mixed_over_fp64": fp64["step"] / mixed["step"],
         "rel_l2_mixed": mixed["err"],
         "rel_l2_fp64": fp64["err"],
-        "fp64_equal_error": eq,
+        "equal_error": eq,
         "bwd_fwd_ratio": None if bf is None else bf["ratio"],
         "bwd_fwd": bf,
         "setup_s": {"tree": t_tree, "operators_mixed": mixed["t_ops"], "plan_mixed": mixed["t_setup"],

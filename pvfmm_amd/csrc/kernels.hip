@@ -498,78 +498,109 @@ __global__ __launch_bounds__(256) void k_far_leaf(const int32_t* __restrict__ tl
 }
 
 // ----------------------------------------------------------------------------- P2P: points -> surfaces
-template <typename Real, int KIND, int VB>
-__global__ __launch_bounds__(64) void k_p2p_surface(const int32_t* __restrict__ tnode, const int32_t* __restrict__ item_ptr,
-                                                    const int32_t* __restrict__ items, const int32_t* __restrict__ src_begin,
-                                                    const int32_t* __restrict__ src_end, const double* __restrict__ center,
-                                                    const float* __restrict__ node_size, Real rad,
-                                                    const Real* __restrict__ unit_surf, int ns, const Real* __restrict__ sx,
-                                                    const Real* __restrict__ sden, int64_t n_src, int64_t den_vs,
-                                                    Real* __restrict__ out, int64_t out_vs, const int32_t* __restrict__ orow,
-                                                    int64_t ld, bool accumulate) {
-  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1;
-  __shared__ Real ls[(3 + VB * K0) * 64];
-  const int row = blockIdx.x, lane = threadIdx.x;
-  const int p = blockIdx.y * 64 + lane;
+// Sources (point ranges of the row's items, leaf-local coordinates shifted into the target
+// node's frame) onto the ns surface points of the target node.  One 256-thread block per row
+// (and per 256 PPT surface points); thread tid takes the points tid, tid + 256, ... of its
+// block's range, PPT of them, so every staged source is read from LDS once for PPT points.
+// Sources are packed into LDS chunks of up to 256 points across consecutive items, one
+// barrier pair per chunk; partial sums over at most 64 sources are kept in Real and added
+// into an FP64 accumulator.
+template <typename Real, int KIND, int VB, int PPT>
+__global__ __launch_bounds__(256) void k_p2p_surface(const int32_t* __restrict__ tnode, const int32_t* __restrict__ item_ptr,
+                                                     const int32_t* __restrict__ items, const int32_t* __restrict__ src_begin,
+                                                     const int32_t* __restrict__ src_end, const double* __restrict__ center,
+                                                     const float* __restrict__ node_size, Real rad,
+                                                     const Real* __restrict__ unit_surf, int ns, const Real* __restrict__ sx,
+                                                     const Real* __restrict__ sden, int64_t n_src, int64_t den_vs,
+                                                     Real* __restrict__ out, int64_t out_vs, const int32_t* __restrict__ orow,
+                                                     int64_t ld, bool accumulate) {
+  constexpr int K0 = KDims<KIND>::k0, K1 = KDims<KIND>::k1, NT = 256, SC = 256;
+  __shared__ __align__(16) Real ls[(3 + VB * K0) * SC];
+  const int row = blockIdx.x, tid = threadIdx.x;
   const int T = tnode[row];
   const double ct0 = center[3 * T], ct1 = center[3 * T + 1], ct2 = center[3 * T + 2];
   const Real sc = rad * (Real)node_size[T];
-  Real p0 = 0, p1 = 0, p2 = 0;
-  if (p < ns) {
-    p0 = sc * unit_surf[3 * p];
-    p1 = sc * unit_surf[3 * p + 1];
-    p2 = sc * unit_surf[3 * p + 2];
+  Real px[PPT], py[PPT], pz[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; q++) {
+    const int p = (blockIdx.y * PPT + q) * NT + tid;
+    const bool ok = p < ns;
+    px[q] = ok ? sc * unit_surf[3 * p] : Real(0);
+    py[q] = ok ? sc * unit_surf[3 * p + 1] : Real(0);
+    pz[q] = ok ? sc * unit_surf[3 * p + 2] : Real(0);
   }
-  double acc[VB][K1];
+  double acc[PPT][VB][K1];
 #pragma unroll
-  for (int v = 0; v < VB; v++)
-#pragma unroll
-    for (int k = 0; k < K1; k++) acc[v][k] = 0;
-  const int ib = item_ptr[row], ie = item_ptr[row + 1];
-  for (int it = ib; it < ie; it++) {
-    const int S = items[it];
-    const int sb = src_begin[S], nsrc = src_end[S] - sb;
-    const Real o0 = (Real)(center[3 * S] - ct0), o1 = (Real)(center[3 * S + 1] - ct1), o2 = (Real)(center[3 * S + 2] - ct2);
-    Real part[VB][K1];
+  for (int q = 0; q < PPT; q++)
 #pragma unroll
     for (int v = 0; v < VB; v++)
 #pragma unroll
-      for (int k = 0; k < K1; k++) part[v][k] = 0;
-    for (int s0 = 0; s0 < nsrc; s0 += 64) {
-      const int nsc = min(64, nsrc - s0);
-      __syncthreads();
-      if (lane < nsc) {
-        const int64_t g = sb + s0 + lane;
-        ls[lane] = sx[g] + o0;
-        ls[64 + lane] = sx[n_src + g] + o1;
-        ls[128 + lane] = sx[2 * n_src + g] + o2;
+      for (int k = 0; k < K1; k++) acc[q][v][k] = 0;
+  const int ib = item_ptr[row], ie = item_ptr[row + 1];
+  int it = ib, off = 0;
+  while (it < ie) {
+    __syncthreads();  // the previous chunk is consumed
+    int pos = 0;
+    while (it < ie && pos < SC) {
+      const int S = items[it];
+      const int sb = src_begin[S], nsrc = src_end[S] - sb;
+      const int take = min(nsrc - off, SC - pos);
+      const Real o0 = (Real)(center[3 * S] - ct0), o1 = (Real)(center[3 * S + 1] - ct1), o2 = (Real)(center[3 * S + 2] - ct2);
+      for (int q = tid; q < take; q += NT) {
+        const int64_t g = sb + off + q;
+        ls[pos + q] = sx[g] + o0;
+        ls[SC + pos + q] = sx[n_src + g] + o1;
+        ls[2 * SC + pos + q] = sx[2 * n_src + g] + o2;
 #pragma unroll
         for (int v = 0; v < VB; v++)
 #pragma unroll
-          for (int c = 0; c < K0; c++) ls[(3 + v * K0 + c) * 64 + lane] = sden[v * den_vs + c * n_src + g];
+          for (int c = 0; c < K0; c++) ls[(3 + v * K0 + c) * SC + pos + q] = sden[v * den_vs + c * n_src + g];
       }
-      __syncthreads();
-      for (int s = 0; s < nsc; s++) {
+      pos += take;
+      off += take;
+      if (off == nsrc) {
+        it++;
+        off = 0;
+      }
+    }
+    __syncthreads();
+    for (int s0 = 0; s0 < pos; s0 += 64) {
+      Real part[PPT][VB][K1];
+#pragma unroll
+      for (int q = 0; q < PPT; q++)
+#pragma unroll
+        for (int v = 0; v < VB; v++)
+#pragma unroll
+          for (int k = 0; k < K1; k++) part[q][v][k] = 0;
+      const int s1 = min(pos, s0 + 64);
+      for (int s = s0; s < s1; s++) {
+        const Real x = ls[s], y = ls[SC + s], z = ls[2 * SC + s];
         Real f[VB][K0];
 #pragma unroll
         for (int v = 0; v < VB; v++)
 #pragma unroll
-          for (int c = 0; c < K0; c++) f[v][c] = ls[(3 + v * K0 + c) * 64 + s];
-        interact_v<KIND, VB, Real>(p0 - ls[s], p1 - ls[64 + s], p2 - ls[128 + s], f, part);
+          for (int c = 0; c < K0; c++) f[v][c] = ls[(3 + v * K0 + c) * SC + s];
+#pragma unroll
+        for (int q = 0; q < PPT; q++) interact_v<KIND, VB, Real>(px[q] - x, py[q] - y, pz[q] - z, f, part[q]);
       }
+#pragma unroll
+      for (int q = 0; q < PPT; q++)
+#pragma unroll
+        for (int v = 0; v < VB; v++)
+#pragma unroll
+          for (int k = 0; k < K1; k++) acc[q][v][k] += (double)part[q][v][k];
     }
-#pragma unroll
-    for (int v = 0; v < VB; v++)
-#pragma unroll
-      for (int k = 0; k < K1; k++) acc[v][k] += (double)part[v][k];
   }
-  if (p < ns) {
+#pragma unroll
+  for (int q = 0; q < PPT; q++) {
+    const int p = (blockIdx.y * PPT + q) * NT + tid;
+    if (p >= ns) continue;
 #pragma unroll
     for (int v = 0; v < VB; v++) {
       Real* o = out + v * out_vs + (int64_t)orow[row] * ld + p * K1;
 #pragma unroll
       for (int k = 0; k < K1; k++) {
-        const Real val = (Real)(acc[v][k] * kscale<KIND>());
+        const Real val = (Real)(acc[q][v][k] * kscale<KIND>());
         o[k] = accumulate ? o[k] + val : val;
       }
     }
@@ -1830,13 +1861,17 @@ void DeviceKernels<Real>::p2p_surface(hipStream_t s, int kind, int n_rows, const
                                       int64_t ld, bool accumulate, int nv, int64_t den_vs, int64_t out_vs) {
   if (n_rows == 0 || nv <= 0) return;
   if (kind != 0 && kind != 2) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-  const dim3 g(n_rows, (ns + 63) / 64);
   for_density_chunks(nv, [&](int v0, int vb) {
     const Real* d = sden + v0 * den_vs;
     Real* o = out + v0 * out_vs;
+    // two surface points per thread for one density vector, one for several
 #define SURF_LAUNCH(KD, VB)                                                                                                 \
-  hipLaunchKernelGGL((k_p2p_surface<Real, KD, VB>), g, dim3(64), 0, s, tnode, item_ptr, items, src_begin, src_end, center, \
-                     node_size, (Real)rad, unit_surf, ns, sx, d, n_src, den_vs, o, out_vs, orow, ld, accumulate)
+  {                                                                                                                         \
+    constexpr int PPT = VB == 1 ? 2 : 1;                                                                                    \
+    const dim3 g(n_rows, (ns + 256 * PPT - 1) / (256 * PPT));                                                               \
+    hipLaunchKernelGGL((k_p2p_surface<Real, KD, VB, PPT>), g, dim3(256), 0, s, tnode, item_ptr, items, src_begin, src_end, \
+                       center, node_size, (Real)rad, unit_surf, ns, sx, d, n_src, den_vs, o, out_vs, orow, ld, accumulate); \
+  }
 #define SURF_VB(KD)                   \
   switch (vb) {                       \
     case 8: SURF_LAUNCH(KD, 8); break; \
