@@ -404,7 +404,7 @@ def main():
     # m; mixed: truncation or FP32 rounding, whichever is larger) and its time; for an error
     # level eps each evaluator takes the smallest m whose error is at most eps (the cheapest),
     # and the ratio of those times is the speedup at equal error.  Reported for eps = the mixed
-    # evaluator's error at the workload's m, and eps = 1e-6.
+    # evaluator's error at each m, and eps = 1e-6.
     eq = None
     if not args.no_equal_error and mixed["err"] is not None:
         frontier = {"fp64": {c["m"]: (fp64["step"], fp64["err"])}, "mixed": {c["m"]: (mixed["step"], mixed["err"])}}
@@ -425,7 +425,7 @@ def main():
             return min(ok) if ok else None
 
         levels = []
-        for eps in (mixed["err"], 1e-6):
+        for eps in sorted({e for _, e in frontier["mixed"].values()} | {1e-6}, reverse=True):
             f64, mx = cheapest("fp64", eps), cheapest("mixed", eps)
             levels.append({"rel_l2_bound": eps,
                            "fp64": None if f64 is None else {"m": f64[1], "ms_per_step": 1e3 * f64[0], "rel_l2": f64[2]},

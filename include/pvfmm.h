@@ -175,8 +175,8 @@ void* pvfmm_amd_tree_stream(void* tree);
 /* Launch-shape tuning of a set-up tree; no counterpart in the reference (its CPU kernels have
  * no launch shapes).  params = {hadamard_tiles_per_block, gemm_split_blocks,
  * hadamard_vec_targets_per_wave} (tiles per block >= 1, used up to 64 by the FP32 kernel;
- * split blocks >= 0; targets per wave >= 1, used by the 16-vector FP32 kernel of plans set up
- * for 8 or more vectors).  set_tuning takes 2 or 3 values.  autotune times the candidates on
+ * split blocks >= 0; targets per wave >= 0, used by the 16-vector FP32 kernel of plans set up
+ * for 8 or more vectors, 0 running such passes one vector at a time through the V list).  set_tuning takes 2 or 3 values.  autotune times the candidates on
  * this tree with n_vec device-resident density vectors d_den (1 <= n_vec <= n_vec_max, as
  * pvfmm_amd_tree_evaluate_device; d_out is overwritten), keeps the fastest and writes the
  * chosen values to params[0..cap).  Returns 0 or an error code. */

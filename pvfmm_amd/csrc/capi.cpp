@@ -539,8 +539,8 @@ int pvfmm_amd_tree_set_tuning(void* tree, const int* params, int n) {
     TreeHandle* t = (TreeHandle*)tree;
     if (!t || !t->plan) throw Error(kErrState, "tree has no device plan; call setup first");
     if (n < 2 || !params) throw Error(kErrInvalidArgument, "at least two tuning values expected");
-    if (params[0] < 1 || params[1] < 0 || (n > 2 && params[2] < 1))
-      throw Error(kErrInvalidArgument, "tiles per block >= 1, split blocks >= 0, targets per wave >= 1");
+    if (params[0] < 1 || params[1] < 0 || (n > 2 && params[2] < 0))
+      throw Error(kErrInvalidArgument, "tiles per block >= 1, split blocks >= 0, targets per wave >= 0");
     t->plan->tuning.hadamard_tpb = params[0];
     t->plan->tuning.gemm_split_blocks = params[1];
     if (n > 2) t->plan->tuning.hadamard_vec_tpw = params[2];

@@ -153,8 +153,11 @@ class DevicePlan final : public DevicePlanBase {
     };
     try {
       if (sizeof(Real) == 4 && e0_ == 1 && vt_.n > 0) {
-        if (use_vec16(n_vec)) pick(tuning.hadamard_vec_tpw, {4, 8, 16, 32}, {kPhHadamard});
-        else pick(tuning.hadamard_tpb, {4, 8, 16, 32, 64}, {kPhHadamard});
+        // with 8 or more vectors: the 16-vector kernel (targets per wave) or, for 0, one vector
+        // at a time through the tiled kernel, compared over the whole V list (their spectra
+        // layouts differ)
+        if (vec16_ && n_vec >= 8) pick(tuning.hadamard_vec_tpw, {0, 4, 8, 16, 32}, {kPhFFT, kPhHadamard, kPhIFFT});
+        if (!use_vec16(n_vec)) pick(tuning.hadamard_tpb, {4, 8, 16, 32, 64}, {kPhHadamard});
       }
       pick(tuning.gemm_split_blocks, {0, 256, 512, 1024, 2048}, {kPhS2UGemm, kPhU2U, kPhD2D, kPhD2TGemm});
     } catch (...) {
@@ -209,7 +212,7 @@ class DevicePlan final : public DevicePlanBase {
   // plans set up for 8 or more vectors): [slot][f][vector 16][16 reals]
   DBuf<Real> Xv_, Yv_;
   bool vec16_ = false;
-  bool use_vec16(int nv) const { return vec16_ && nv >= 8; }
+  bool use_vec16(int nv) const { return vec16_ && nv >= 8 && tuning.hadamard_vec_tpw > 0; }
   int n_vsrc_ = 0, n_vtrg_ = 0, n_off_ = 0;
 
   // V-list target tiles for the LDS-staged Hadamard

@@ -43,7 +43,7 @@ struct PlanStats {
 struct Tuning {
   int hadamard_tpb = 16;        // tiles per block run of the FP32 scalar Hadamard
   int gemm_split_blocks = 1024;  // GEMMs with fewer tile blocks than this split K
-  int hadamard_vec_tpw = 16;    // targets per wave of the 16-vector FP32 scalar Hadamard
+  int hadamard_vec_tpw = 16;    // targets per wave of the 16-vector FP32 scalar Hadamard (0: one vector at a time)
 };
 
 class DevicePlanBase {

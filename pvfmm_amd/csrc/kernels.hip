@@ -1758,7 +1758,11 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_vec16(int M_dim, int n_trg,
                                                           const int32_t* __restrict__ partner, const uint16_t* __restrict__ Vb,
                                                           const float* __restrict__ X, float* __restrict__ Y) {
   constexpr int NOFF = 2 * NPAIR;
-  const int b = blockIdx.x, chunk = b % n_chunks, f = b / n_chunks;
+  // all target chunks of one frequency run on one XCD, back to back (block b runs on XCD b mod
+  // 8), so the partners' spectra at that frequency are read into that XCD's L2 once and served
+  // from there to every target that lists them
+  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
+  const int chunk = local % n_chunks, f = (local / n_chunks) * 8 + xcd;
   if (f >= M_dim) return;  // whole block
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int row = l & 15, kg = l >> 4, pa = kg >> 1, half = kg & 1;
@@ -2087,7 +2091,7 @@ void DeviceKernels<Real>::hadamard_vec16(hipStream_t s, int n_trg, int M_dim, co
     if (!Vb) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
     const int tpw = std::max(1, targets_per_wave);
     const int n_chunks = (n_trg + 4 * tpw - 1) / (4 * tpw);
-    const int64_t nb = (int64_t)n_chunks * M_dim;
+    const int64_t nb = (int64_t)n_chunks * ((M_dim + 7) / 8) * 8;
     if (nb >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
     hipLaunchKernelGGL(k_hadamard_vec16<13>, dim3((unsigned)nb), dim3(256), 0, s, M_dim, n_trg, tpw, n_chunks, partner, Vb,
                        X, Y);

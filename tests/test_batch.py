@@ -130,8 +130,9 @@ def test_sixteen_vector_pass_and_its_tuning(kernel):
     """A plan set up for 8 or more vectors runs the V list of a pass through the 16-vector
     Hadamard (the vectors of one target in the MFMA rows).  Its products and their order are
     those of the one-vector kernel (absent partners add zero products), so every vector equals
-    its own one-vector evaluation to rounding of the other phases; the nv > 1 autotuner picks a
-    targets-per-wave shape and keeps the results."""
+    its own one-vector evaluation to rounding of the other phases; targets per wave 0 (one
+    vector at a time through the tiled kernel) gives the same values; the nv > 1 autotuner picks
+    between them and keeps the results."""
     import torch
 
     rng = np.random.default_rng(8)
@@ -148,8 +149,12 @@ def test_sixteen_vector_pass_and_its_tuning(kernel):
         worst = max(worst, float(torch.linalg.norm(ub[v] - u1) / torch.linalg.norm(u1)))
     print(f"{kernel}: 20 vectors in one pass vs one at a time, worst rel {worst:.2e}")
     assert worst < 1e-6
+    # targets per wave 0: the same pass one vector at a time through the tiled kernel
+    tb.set_tuning(hadamard_vec_targets_per_wave=0)
+    u0 = tb.evaluate_device(q)
+    assert float(torch.linalg.norm(u0 - ub) / torch.linalg.norm(ub)) < 1e-6
     chosen = tb.autotune(q[:16].contiguous())
     print(f"autotuned for 16 vectors: {chosen}")
-    assert chosen["hadamard_vec_targets_per_wave"] in (4, 8, 16, 32)
+    assert chosen["hadamard_vec_targets_per_wave"] in (0, 4, 8, 16, 32)
     ub2 = tb.evaluate_device(q)
     assert float(torch.linalg.norm(ub2 - ub) / torch.linalg.norm(ub)) < 1e-6
