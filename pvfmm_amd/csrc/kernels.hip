@@ -304,13 +304,17 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
               for (int k = 0; k < K1; k++) part[v][k] = 0;
             const int s1 = min(je, s0 + 64);
             int s = s0;
-            for (; s + 4 <= s1; s += 4) {
+            // s is a multiple of 4 here: the 4 sources' coordinates and densities are 16-byte
+            // LDS words at immediate offsets from one base per array
+            const double2* lc2 = reinterpret_cast<const double2*>(lc) + (s >> 1);
+            const float4* ld4 = reinterpret_cast<const float4*>(ld) + (s >> 2);
+            for (; s + 4 <= s1; s += 4, lc2 += 2, ld4 += 1) {
               double xs[3][4];
 #pragma unroll
               for (int d = 0; d < 3; d++)
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                  const double2 w2 = *reinterpret_cast<const double2*>(&lc[d * SC + s + 2 * h]);
+                  const double2 w2 = lc2[(d * SC >> 1) + h];
                   xs[d][2 * h] = w2.x;
                   xs[d][2 * h + 1] = w2.y;
                 }
@@ -318,9 +322,13 @@ __global__ __launch_bounds__(256) void k_p2p_leaf(const int32_t* __restrict__ tl
 #pragma unroll
               for (int v = 0; v < VB; v++)
 #pragma unroll
-                for (int c = 0; c < K0; c++)
-#pragma unroll
-                  for (int u = 0; u < 4; u++) fs[v][c][u] = ld[(v * K0 + c) * SC + s + u];
+                for (int c = 0; c < K0; c++) {
+                  const float4 q4 = ld4[(v * K0 + c) * SC >> 2];
+                  fs[v][c][0] = q4.x;
+                  fs[v][c][1] = q4.y;
+                  fs[v][c][2] = q4.z;
+                  fs[v][c][3] = q4.w;
+                }
 #pragma unroll
               for (int u = 0; u < 4; u++) {
                 Real f[VB][K0];
