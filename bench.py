@@ -395,6 +395,9 @@ def main():
         log(f"{prec}: {1e3 * step_plain:.2f} ms/eval ({n / step_plain / 1e6:.1f} Mtargets/s), rel-L2 {res[prec]['err'] if res[prec]['err'] is None else format(res[prec]['err'], '.3e')}, "
             f"operators {t_ops:.1f}s, setup {t_setup:.1f}s, launch shapes {shapes} ({'default' if args.no_tune else f'autotuned in {t_tune:.1f}s'})")
         log("   phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
+        st = res[prec]["stats"]
+        log(f"   V-list tiles {st['v_tiles']}, partners {st['v_tile_partners']} (largest union "
+            f"{st['v_tile_max_partners']}), staged {st.get('v_tile_staged', st['v_tile_partners'])}")
 
     if args.prec != "both":
         return

@@ -216,7 +216,8 @@ int pvfmm_amd_tree_phase_times(void* tree, double* ms, int cap);
 const char* pvfmm_amd_phase_name(int phase);
 /* plan statistics: n_nodes, n_leaves, max_depth, s2m_rows, l2t_rows, m2m_rows, l2l_rows, x_rows,
  * x_items, v_src_slots, v_trg_slots, v_pairs, u_rows, u_items, u_pairs, far_items, far_pairs,
- * s2m_pairs, u_src_points_streamed, device_bytes, v_tiles, v_tile_partners, v_tile_max_partners;
+ * s2m_pairs, u_src_points_streamed, device_bytes, v_tiles, v_tile_partners, v_tile_max_partners,
+ * v_tile_staged (partner spectra a pass over the V-list tiles stages into LDS);
  * copies the first `cap` values and returns how many exist (negative error code on failure) */
 int pvfmm_amd_tree_stats(void* tree, int64_t* out, int cap);
 

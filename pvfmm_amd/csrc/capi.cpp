@@ -615,7 +615,7 @@ int pvfmm_amd_tree_stats(void* tree, int64_t* o, int cap) {
     const int64_t v[] = {s.n_nodes, s.n_leaves, s.max_depth, s.s2m_rows, s.l2t_rows, s.m2m_rows, s.l2l_rows, s.x_rows,
                          s.x_items, s.v_src_slots, s.v_trg_slots, s.v_pairs, s.u_rows, s.u_items, s.u_pairs, s.far_items,
                          s.far_pairs, s.s2m_pairs, s.u_src_points_streamed, s.device_bytes, s.v_tiles, s.v_tile_partners,
-                         s.v_tile_max_partners};
+                         s.v_tile_max_partners, s.v_tile_staged};
     count = (int)(sizeof(v) / sizeof(v[0]));
     std::copy(v, v + std::min(cap, count), o);
   });

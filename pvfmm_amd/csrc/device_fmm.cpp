@@ -51,57 +51,101 @@ struct DBuf {
   }
 };
 
-// Order of a V-list tile's staged partners for the bf16 kernel, whose 16 lanes reading one
-// staged entry each (the 16 targets of a group at one offset) are served together: entry
-// position p lands in LDS bank group 7p mod 16, so the positions of the partners a group reads
-// at one offset should differ mod 16.  Greedy colouring by residue mod 16, most-read entries
-// first, residue classes filled to their share of positions; pos[] (union position of each
-// source slot) and the tile's usrc range are permuted in place.
-static void color_union(int k, int n_off, const std::vector<int32_t>& t0, const std::vector<int32_t>& u0,
-                        std::vector<int32_t>& usrc, const std::vector<int32_t>& partner, std::vector<int32_t>& pos) {
+// LDS pool slots of the V-list tiles' staged partners for the bf16 kernel (FP32 scalar
+// kernels), which keeps one pool of `pool` staged entries across the run of tiles a block
+// handles: a partner of tile k that tile k-1 also lists keeps its slot, and only the others
+// (the delta) are read and staged, into slots that no partner of tile k-1 holds, while tile
+// k-1 is computed.  A block's first tile stages its whole union into the same slots.
+// Among the free slots, a new partner takes one by greedy colouring, most-read partners
+// first: the 16 lanes that read one staged entry each (the 16 targets of a group at one
+// offset) are served together, and slot s lands in LDS bank group 7s mod 16, so the slots of
+// the partners a group reads at one offset should differ mod 16.  Output: kslot[u] the slot of
+// union entry u, and per tile k the delta dent[d0[k] .. d0[k+1]) as (source slot << 9 | slot).
+static void assign_pool_slots(int n_off, int pool, int n_s, const std::vector<int32_t>& t0, const std::vector<int32_t>& u0,
+                              const std::vector<int32_t>& usrc, const std::vector<int32_t>& partner,
+                              std::vector<int32_t>& kslot, std::vector<int32_t>& d0, std::vector<int32_t>& dent) {
   constexpr int G = 16;
-  const int nu = u0[k + 1] - u0[k];
-  if (nu <= 1) return;
-  std::vector<std::vector<int>> grp;         // local indices read together
-  std::vector<std::vector<int>> memb(nu);    // groups of each local index
-  for (int g = 0; g < 4; g++)
-    for (int o = 0; o < n_off; o++) {
-      std::vector<int> m;
-      for (int r = 0; r < 16; r++) {
-        const int t = t0[k] + 16 * g + r;
-        if (t >= t0[k + 1]) break;
-        const int32_t sl = partner[(size_t)t * n_off + o];
-        if (sl >= 0) m.push_back(pos[sl]);
+  const int n_tiles = (int)t0.size() - 1;
+  kslot.assign(usrc.size(), -1);
+  d0.assign(1, 0);
+  dent.clear();
+  std::vector<int32_t> slot_of(std::max(1, n_s), -1);  // slots of the partners of the last tile
+  std::vector<int32_t> loc(std::max(1, n_s), -1);      // local index in the current tile
+  std::vector<char> busy(pool);
+  for (int k = 0; k < n_tiles; k++) {
+    const int nu = u0[k + 1] - u0[k];
+    for (int u = u0[k]; u < u0[k + 1]; u++) loc[usrc[u]] = u - u0[k];
+    std::fill(busy.begin(), busy.end(), 0);
+    if (k > 0)
+      for (int u = u0[k - 1]; u < u0[k]; u++) {
+        const int32_t sl = usrc[u];
+        busy[slot_of[sl]] = 1;
+        if (loc[sl] < 0) slot_of[sl] = -1;  // dropped: a later listing starts a new run
       }
-      std::sort(m.begin(), m.end());
-      m.erase(std::unique(m.begin(), m.end()), m.end());
-      if (m.size() < 2) continue;
-      for (int li : m) memb[li].push_back((int)grp.size());
-      grp.push_back(std::move(m));
+    // groups of local indices read together, as (group, offset) of the tile
+    std::vector<std::vector<int>> grp, memb(nu);
+    for (int g = 0; g < 4; g++)
+      for (int o = 0; o < n_off; o++) {
+        std::vector<int> m;
+        for (int r = 0; r < 16; r++) {
+          const int t = t0[k] + 16 * g + r;
+          if (t >= t0[k + 1]) break;
+          const int32_t sl = partner[(size_t)t * n_off + o];
+          if (sl >= 0) m.push_back(loc[sl]);
+        }
+        std::sort(m.begin(), m.end());
+        m.erase(std::unique(m.begin(), m.end()), m.end());
+        if (m.size() < 2) continue;
+        for (int li : m) memb[li].push_back((int)grp.size());
+        grp.push_back(std::move(m));
+      }
+    std::vector<int> col(nu, -1), order;
+    for (int li = 0; li < nu; li++) {
+      const int32_t sl = usrc[u0[k] + li];
+      if (slot_of[sl] >= 0) col[li] = slot_of[sl] % G;  // kept
+      else order.push_back(li);
     }
-  std::vector<int> order(nu);
-  for (int i = 0; i < nu; i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return memb[a].size() > memb[b].size(); });
-  int cap[G], next[G];
-  for (int c = 0; c < G; c++) { cap[c] = (nu - c + G - 1) / G; next[c] = c; }
-  std::vector<int> col(nu, -1);
-  for (int li : order) {
-    int bad[G] = {0};
-    for (int gi : memb[li])
-      for (int lj : grp[gi])
-        if (col[lj] >= 0) bad[col[lj]]++;
-    int best = -1;
-    for (int c = 0; c < G; c++)
-      if (cap[c] > 0 && (best < 0 || bad[c] < bad[best])) best = c;
-    col[li] = best;
-    cap[best]--;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return memb[x].size() > memb[y].size(); });
+    for (int li : order) {
+      int bad[G] = {0};
+      for (int gi : memb[li])
+        for (int lj : grp[gi])
+          if (col[lj] >= 0) bad[col[lj]]++;
+      int best = -1, best_slot = -1;
+      for (int c = 0; c < G; c++) {
+        int s = c;
+        while (s < pool && busy[s]) s += G;
+        if (s >= pool) continue;
+        if (best < 0 || bad[c] < bad[best] || (bad[c] == bad[best] && s < best_slot)) { best = c; best_slot = s; }
+      }
+      if (best < 0) throw Error(kErrInternal, "V-list tiles exceed the staging pool");
+      const int32_t sl = usrc[u0[k] + li];
+      busy[best_slot] = 1;
+      slot_of[sl] = best_slot;
+      col[li] = best;
+      dent.push_back((sl << 9) | best_slot);
+    }
+    for (int u = u0[k]; u < u0[k + 1]; u++) {
+      kslot[u] = slot_of[usrc[u]];
+      loc[usrc[u]] = -1;
+    }
+    d0.push_back((int32_t)dent.size());
   }
-  std::vector<int32_t> old(usrc.begin() + u0[k], usrc.begin() + u0[k + 1]);
-  for (int li = 0; li < nu; li++) {
-    const int p = next[col[li]];
-    next[col[li]] += G;
-    usrc[u0[k] + p] = old[li];
-    pos[old[li]] = p;
+  // replay the staging as the kernel does it: each tile's delta goes to slots its predecessor
+  // does not read, after which every slot of the tile's table holds the listed source
+  std::vector<int32_t> held(pool, -1);
+  for (int k = 0; k < n_tiles; k++) {
+    if (k > 0) std::fill(busy.begin(), busy.end(), 0);
+    if (k > 0)
+      for (int u = u0[k - 1]; u < u0[k]; u++) busy[kslot[u]] = 1;
+    for (int d = d0[k]; d < d0[k + 1]; d++) {
+      const int s = dent[d] & 511;
+      if (s >= pool || (k > 0 && busy[s])) throw Error(kErrInternal, "V-list pool slot overwritten while in use");
+      held[s] = dent[d] >> 9;
+    }
+    for (int u = u0[k]; u < u0[k + 1]; u++)
+      if (kslot[u] < 0 || kslot[u] >= pool || held[kslot[u]] != usrc[u])
+        throw Error(kErrInternal, "V-list pool slot does not hold its partner");
   }
 }
 
@@ -220,57 +264,76 @@ class DevicePlan final : public DevicePlanBase {
     DBuf<int32_t> t0, u0, usrc;
     DBuf<uint16_t> uidx;
     DBuf<uint32_t> gmask;
+    DBuf<int32_t> d0, dent;  // FP32 scalar kernel: per tile, the partners new since the previous tile
     int n = 0;
-    int64_t partners = 0, max_partners = 0;
+    int64_t partners = 0, max_partners = 0, staged_delta = 0;
   };
   TileSet vt_;
 
   // Tiles of consecutive target slots: at most 64 targets, at most umax distinct partners
   // (greedy in slot order, which is Morton order within a level).  partner[t * n_off + o] is
-  // the source slot of target slot t at offset o, or -1.
+  // the source slot of target slot t at offset o, or -1.  For the FP32 scalar kernel (LDS pool
+  // of assign_pool_slots) a tile's partners together with the previous tile's must also fit
+  // the pool, and usrc holds each tile's union as (source slot << 9 | pool slot).
   void build_tiles(const std::vector<int32_t>& v_partner, int n_t, int n_s, TileSet& ts, int64_t* total) {
     const int umax = K::hadamard_union_max(e0_);
+    const bool pair_layout = std::is_same<Real, float>::value && e0_ == 1 && e1_ == 1;
+    const int pool = pair_layout ? K::hadamard_pool_max() : (1 << 30);
     std::vector<int32_t> t0{0}, u0{0}, usrc;
     std::vector<uint16_t> uidx;
     std::vector<int32_t> pos(std::max(1, n_s), -1);
-    std::vector<int32_t> cur;  // distinct partners of the open tile
+    std::vector<char> in_prev(std::max(1, n_s), 0);  // partner of the last closed tile
+    std::vector<int32_t> cur, prev;  // distinct partners of the open tile and of the last closed one
+    int cur_new = 0;                 // partners of the open tile that the last closed one lacks
     auto close_tile = [&](int t_end) {
       for (int32_t sl : cur) pos[sl] = -1;
       usrc.insert(usrc.end(), cur.begin(), cur.end());
+      for (int32_t sl : prev) in_prev[sl] = 0;
+      prev.swap(cur);
+      for (int32_t sl : prev) in_prev[sl] = 1;
       cur.clear();
+      cur_new = 0;
       t0.push_back(t_end);
       u0.push_back((int32_t)usrc.size());
     };
     // first pass: tile boundaries and union lists
     for (int t = 0; t < n_t; t++) {
-      int add = 0;
+      int add = 0, add_new = 0;
       for (int k = 0; k < n_off_; k++) {
         const int32_t sl = v_partner[(size_t)t * n_off_ + k];
-        if (sl >= 0 && pos[sl] < 0) { pos[sl] = -2; add++; }  // provisional mark
+        if (sl >= 0 && pos[sl] < 0) { pos[sl] = -2; add++; add_new += in_prev[sl] ? 0 : 1; }  // provisional mark
       }
       for (int k = 0; k < n_off_; k++) {
         const int32_t sl = v_partner[(size_t)t * n_off_ + k];
         if (sl >= 0 && pos[sl] == -2) pos[sl] = -1;
       }
-      if (t > t0.back() && (t - t0.back() == 64 || (int)cur.size() + add > umax)) close_tile(t);
+      if (t > t0.back() && (t - t0.back() == 64 || (int)cur.size() + add > umax ||
+                            (int64_t)prev.size() + cur_new + add_new > pool))
+        close_tile(t);
       for (int k = 0; k < n_off_; k++) {
         const int32_t sl = v_partner[(size_t)t * n_off_ + k];
-        if (sl >= 0 && pos[sl] < 0) { pos[sl] = (int32_t)cur.size(); cur.push_back(sl); }
+        if (sl >= 0 && pos[sl] < 0) {
+          pos[sl] = (int32_t)cur.size();
+          cur.push_back(sl);
+          cur_new += in_prev[sl] ? 0 : 1;
+        }
       }
     }
     if (n_t > t0.back()) close_tile(n_t);
     const int n_tiles = (int)t0.size() - 1;
-    // a missing partner points at the staging area's zero entry (index umax).  The FP32
-    // scalar kernel reads its table as [k][group][row][o % 2][o / 2 (padded to 16)], the
-    // others as [k][o][row][group].
-    const bool pair_layout = std::is_same<Real, float>::value && e0_ == 1 && e1_ == 1;
+    // a missing partner points at the staging area's zero entry (index umax, or the pool size
+    // for the FP32 scalar kernel).  The FP32 scalar kernel reads its table as [k][group][row]
+    // [o % 2][o / 2 (padded to 16)], the others as [k][o][row][group].
     const size_t per_tile = pair_layout ? (size_t)4 * 16 * 2 * 16 : (size_t)64 * n_off_;
     if (pair_layout && n_off_ > 32) throw Error(kErrInternal, "V-list offsets exceed the pair table");
-    uidx.assign((size_t)n_tiles * per_tile, (uint16_t)umax);
+    if (pair_layout && (pool + 1 > 511 || (int64_t)n_s >= (int64_t(1) << 22)))
+      throw Error(kErrInternal, "V-list staging pool or source slots exceed the packed entry");
+    std::vector<int32_t> kslot, d0, dent;
+    if (pair_layout) assign_pool_slots(n_off_, pool, n_s, t0, u0, usrc, v_partner, kslot, d0, dent);
+    uidx.assign((size_t)n_tiles * per_tile, (uint16_t)(pair_layout ? pool : umax));
     std::vector<uint32_t> gmask((size_t)n_tiles * 4, 0u);  // offsets present per 16-target group
     for (int k = 0; k < n_tiles; k++) {
-      for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = u - u0[k];
-      if (pair_layout) color_union(k, n_off_, t0, u0, usrc, v_partner, pos);
+      for (int u = u0[k]; u < u0[k + 1]; u++) pos[usrc[u]] = pair_layout ? kslot[u] : u - u0[k];
       for (int t = t0[k]; t < t0[k + 1]; t++)
         for (int o = 0; o < n_off_; o++) {
           const int32_t sl = v_partner[(size_t)t * n_off_ + o];
@@ -289,7 +352,19 @@ class DevicePlan final : public DevicePlanBase {
     }
     ts.n = n_tiles;
     ts.partners = (int64_t)usrc.size();
-    usrc.push_back(0);  // one readable slot past the end: the kernels read ahead branch-free
+    if (pair_layout) {
+      for (size_t u = 0; u < usrc.size(); u++) usrc[u] = (usrc[u] << 9) | kslot[u];
+      ts.staged_delta = (int64_t)dent.size();
+      // one readable entry past each list's end (source 0 into the unread trash slot): the
+      // kernel reads ahead branch-free
+      dent.push_back(pool + 1);
+      usrc.push_back(pool + 1);
+      ts.d0.upload(d0, total);
+      ts.dent.upload(dent, total);
+    } else {
+      ts.staged_delta = ts.partners;
+      usrc.push_back(0);  // one readable slot past the end: the kernels read ahead branch-free
+    }
     ts.t0.upload(t0, total); ts.u0.upload(u0, total); ts.usrc.upload(usrc, total); ts.uidx.upload(uidx, total);
     ts.gmask.upload(gmask, total);
   }
@@ -600,6 +675,7 @@ class DevicePlan final : public DevicePlanBase {
     stats.v_tiles = vt_.n;
     stats.v_tile_partners = vt_.partners;
     stats.v_tile_max_partners = vt_.max_partners;
+    stats.v_tile_staged = vt_.staged_delta;
     X_.alloc((int64_t)n_vsrc_ * M_dim_ * 16 * e0_, &total);
     Y_.alloc((int64_t)n_vtrg_ * M_dim_ * 16 * e1_, &total);
     {
@@ -877,7 +953,8 @@ class DevicePlan final : public DevicePlanBase {
     // (the spectra buffers swap roles: Y_ holds the new sources, X_ the new targets)
     for (int v = 0; v < nv; v++) {
       K::fft_up(st_, n_vtrg_, v_trg_fc_.p, dn_.p + v * dn_vs_, K1_, e1_, m_, lattice_.p, lattice_inv_.p, a.fft_scal.p, Y_.p);
-      K::hadamard_tiled(st_, a.vt.n, M_dim_, e0_, e1_, n_off_, a.vt.t0.p, a.vt.u0.p, a.vt.usrc.p, a.vt.uidx.p, a.vt.gmask.p,
+      K::hadamard_tiled(st_, a.vt.n, M_dim_, e0_, e1_, n_off_, a.vt.t0.p, a.vt.u0.p, a.vt.usrc.p, a.vt.d0.p, a.vt.dent.p,
+                        a.vt.uidx.p, a.vt.gmask.p,
                         v_offsets_.p, V_.p, Y_.p, X_.p, tuning.hadamard_tpb, Vb_.p);
       K::ifft_down(st_, a.n_vt, a.v_fc.p, X_.p, eq_.p + v * eq_vs_, K0_, e0_, m_, lattice_.p, lattice_inv_.p, ns_,
                    a.ifft_scal.p);
@@ -964,7 +1041,8 @@ class DevicePlan final : public DevicePlanBase {
       if (grouped)
         K::hadamard_vec16(st_, n_vtrg_, M_dim_, v_partner_.p, Vb_.p, Xv_.p, Yv_.p, tuning.hadamard_vec_tpw);
       else
-        K::hadamard_tiled(st_, vt_.n, M_dim_, e0_, e1_, n_off_, vt_.t0.p, vt_.u0.p, vt_.usrc.p, vt_.uidx.p, vt_.gmask.p,
+        K::hadamard_tiled(st_, vt_.n, M_dim_, e0_, e1_, n_off_, vt_.t0.p, vt_.u0.p, vt_.usrc.p, vt_.d0.p, vt_.dent.p,
+                          vt_.uidx.p, vt_.gmask.p,
                           v_offsets_.p, V_.p, X_.p, Y_.p, tuning.hadamard_tpb, Vb_.p);
       if (sub) HIP_CHECK(hipEventRecord(vev_[4 * g + 2], st_));
       if (n_pass == 1) mark(7);

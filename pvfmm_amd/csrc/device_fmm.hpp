@@ -36,6 +36,7 @@ struct PlanStats {
   int64_t u_src_points_streamed = 0;
   int64_t device_bytes = 0;
   int64_t v_tiles = 0, v_tile_partners = 0, v_tile_max_partners = 0;
+  int64_t v_tile_staged = 0;  // partner spectra staged per pass (the FP32 scalar kernel stages deltas)
 };
 
 // Launch shapes an evaluation uses; the defaults suit large trees, and autotune() picks them

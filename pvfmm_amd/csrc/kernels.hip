@@ -1558,8 +1558,14 @@ __global__ __launch_bounds__(256, (hadamard_blocks_per_cu<Real, E0, FB, UMAX>())
 // while a tile's MFMAs run from one buffer, the next tile's partner table and spectra are read
 // from global memory and written to the other, one barrier per tile.  Pairs at which none of
 // the group's targets has a partner are skipped.  Two blocks run per CU.
+// The staged entries form one LDS pool of POOL entries for the block's run of tiles (entry
+// POOL stays zero, entry POOL + 1 takes the writes of idle lanes): a partner that consecutive
+// tiles list stays in its slot, so after the run's first tile only the partners new since the
+// previous tile are read and staged, into slots that the previous tile does not read (the host
+// assigns the slots).  Staging passes past the length of that list are skipped.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kSplitPool = 496;
 
 // the three bf16 parts of an FP32 value, by truncation (each step exact), as the high halves
 // of three words
@@ -1575,28 +1581,29 @@ __device__ __forceinline__ void split3(float x, uint32_t& h0, uint32_t& h1, uint
 // two bf16 (the high halves of a and b, low halves zero) packed into one word, a in the low half
 __device__ __forceinline__ uint32_t pack_hi(uint32_t a, uint32_t b) { return (a >> 16) | b; }
 
-template <int UMAX>
+template <int UMAX, int POOL>
 __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tiles, int tiles_per_block, int n_chunks,
                                                           const int32_t* __restrict__ tile_t0,
                                                           const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
+                                                          const int32_t* __restrict__ tile_d0, const int32_t* __restrict__ dent,
                                                           const uint16_t* __restrict__ uidx, const uint32_t* __restrict__ gmask,
                                                           const uint16_t* __restrict__ Vb, const float* __restrict__ X,
                                                           float* __restrict__ Y) {
   constexpr int NOFF = 26, NPAIR = NOFF / 2;
   constexpr int RW = 16;                  // reals per staged entry and frequency
   constexpr int EST = 3 * RW / 2 + 4;     // words per staged entry: 3 parts x 16 bf16, padded
-  constexpr int XBUF = (UMAX + 2) * EST;  // entry UMAX stays zero, UMAX + 1 takes unused lanes
+  constexpr int XBUF = (POOL + 2) * EST;  // entry POOL stays zero, POOL + 1 takes unused lanes
   constexpr int UT = 4 * 16 * 2 * 16;     // partner table of a tile: [group][row][a][pair (padded to 16)]
   constexpr int UW = UT / 4;              // as 8-byte words
   constexpr int UWP = UW / 256;
   constexpr int EPP = 256 / (RW / 4);     // staged entries per pass (a thread stages 4 reals)
   constexpr int NPASS = (UMAX + EPP - 1) / EPP;
   constexpr int TMAX = 64;                // longest run of tiles (the launcher clamps to it)
-  static_assert(NPASS < NPAIR && UW % 256 == 0, "staging shape");
-  __shared__ __align__(16) uint32_t Xl[2][XBUF];
+  static_assert(NPASS < NPAIR && UW % 256 == 0 && POOL + 1 < 512, "staging shape");
+  __shared__ __align__(16) uint32_t Xl[XBUF];
   __shared__ __align__(16) uint16_t ul[2][UT];
-  __shared__ uint32_t m_g[4 * TMAX];  // the run's skip masks, union starts and target starts
-  __shared__ int32_t m_u[TMAX + 1], m_t[TMAX + 1];
+  __shared__ uint32_t m_g[4 * TMAX];  // the run's skip masks, delta starts and target starts
+  __shared__ int32_t m_d[TMAX + 1], m_t[TMAX + 1];
   // blocks b and b + 8 run on one XCD one after the other and take the two frequencies of one
   // 128-byte line of X and Y (a frequency's 16 reals are 64 bytes) for the same run of tiles,
   // so the second block's reads hit that XCD's L2
@@ -1618,41 +1625,33 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
 #pragma unroll
     for (int p = 0; p < 3; p++)
       bq[j][p] = *reinterpret_cast<const bf16x8*>(Vb + ((((int64_t)j * 3 + p) * M_dim + f) * 64 + l) * 8);
-  for (int e = tid; e < EST; e += 256) {
-    Xl[0][UMAX * EST + e] = 0u;
-    Xl[1][UMAX * EST + e] = 0u;
-  }
+  for (int e = tid; e < EST; e += 256) Xl[POOL * EST + e] = 0u;
   const int nk = k1 - k0;
   for (int i = tid; i < 4 * nk; i += 256) m_g[i] = gmask[4 * k0 + i];
   for (int i = tid; i <= nk; i += 256) {
-    m_u[i] = tile_u0[k0 + i];
+    m_d[i] = tile_d0[k0 + i];
     m_t[i] = tile_t0[k0 + i];
   }
 
-  int src[NPASS];
+  int src[NPASS];  // staged entries (source slot << 9 | pool slot)
+  int dst[NPASS];  // their pool slots, kept from the global loads to the LDS stores
   floatx4 val[NPASS];
   uint2 uw[UWP];
-  // union start and size of tile k (size 0 past the run's end), from global memory (prologue)
-  // or from the run's copy in LDS (loop)
-  auto u0_g = [&](int k) __attribute__((always_inline)) { return tile_u0[min(k, k1 - 1)]; };
-  auto nu_g = [&](int k) __attribute__((always_inline)) {
-    const int kk = min(k, k1 - 1);
-    const int n = tile_u0[kk + 1] - tile_u0[kk];
-    return k < k1 ? n : 0;
-  };
-  auto u0_l = [&](int k) __attribute__((always_inline)) { return m_u[min(k, k1 - 1) - k0]; };
-  auto nu_l = [&](int k) __attribute__((always_inline)) {
+  // start and length of tile k's delta list (length 0 past the run's end), from the run's
+  // copy in LDS, the length uniform over the block
+  auto d0_l = [&](int k) __attribute__((always_inline)) { return m_d[min(k, k1 - 1) - k0]; };
+  auto nd_l = [&](int k) __attribute__((always_inline)) {
     const int kk = min(k, k1 - 1) - k0;
-    const int n = m_u[kk + 1] - m_u[kk];
-    return k < k1 ? n : 0;
+    const int n = m_d[kk + 1] - m_d[kk];
+    return __builtin_amdgcn_readfirstlane(k < k1 ? n : 0);
   };
-  auto load_src = [&](int i, int u0, int nu) __attribute__((always_inline)) {
-    src[i] = usrc[u0 + min(su + EPP * i, max(nu - 1, 0))];
+  auto load_src = [&](int i, const int32_t* list, int at, int n) __attribute__((always_inline)) {
+    src[i] = list[at + min(su + EPP * i, max(n - 1, 0))];
   };
-  auto load_val = [&](int i, int nu) __attribute__((always_inline)) {
-    const bool ok = su + EPP * i < nu;
-    const int sl = ok ? src[i] : 0;  // a select on the input keeps it branch-free
-    val[i] = *reinterpret_cast<const floatx4*>(X + ((int64_t)sl * M_dim + f) * RW + 4 * sr);
+  auto load_val = [&](int i, int n) __attribute__((always_inline)) {
+    const int e = su + EPP * i < n ? src[i] : POOL + 1;  // a select on the input keeps it branch-free
+    dst[i] = e & 511;
+    val[i] = *reinterpret_cast<const floatx4*>(X + ((int64_t)(e >> 9) * M_dim + f) * RW + 4 * sr);
   };
   auto load_uw = [&](int k) __attribute__((always_inline)) {
     const uint2* ug = reinterpret_cast<const uint2*>(uidx + (int64_t)min(k, k1 - 1) * UT);
@@ -1660,14 +1659,14 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
     for (int i = 0; i < UWP; i++) uw[i] = ug[tid + 256 * i];
   };
   // entry layout (words): part p at 8p, reals 2q, 2q+1 in word 8p + q
-  auto store_lds = [&](int buf, int nu) __attribute__((always_inline)) {
+  auto store_lds = [&](int buf, int n) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < NPASS; i++) {
-      const int u = su + EPP * i;
+      if (EPP * i >= n) break;  // uniform
       uint32_t h[3][4];
 #pragma unroll
       for (int q = 0; q < 4; q++) split3(val[i][q], h[0][q], h[1][q], h[2][q]);
-      uint32_t* d = &Xl[buf][(u < nu ? u : UMAX + 1) * EST + 2 * sr];  // others: trash entry
+      uint32_t* d = &Xl[dst[i] * EST + 2 * sr];
 #pragma unroll
       for (int p = 0; p < 3; p++)
         *reinterpret_cast<uint2*>(d + 8 * p) = uint2{pack_hi(h[p][0], h[p][1]), pack_hi(h[p][2], h[p][3])};
@@ -1676,15 +1675,17 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
     for (int i = 0; i < UWP; i++) reinterpret_cast<uint2*>(ul[buf])[tid + 256 * i] = uw[i];
   };
   {
-    const int nu0 = nu_g(k0), u00 = u0_g(k0);
+    // the run's first tile: its whole union
+    const int u00 = tile_u0[k0], nu0 = __builtin_amdgcn_readfirstlane(tile_u0[k0 + 1] - u00);
 #pragma unroll
-    for (int i = 0; i < NPASS; i++) load_src(i, u00, nu0);
+    for (int i = 0; i < NPASS; i++) load_src(i, usrc, u00, nu0);
 #pragma unroll
     for (int i = 0; i < NPASS; i++) load_val(i, nu0);
     load_uw(k0);
-    const int nu01 = nu_g(k0 + 1), u01 = u0_g(k0 + 1);
+    __syncthreads();  // m_d
+    const int nd1 = nd_l(k0 + 1), d01 = d0_l(k0 + 1);
 #pragma unroll
-    for (int i = 0; i < NPASS; i++) load_src(i, u01, nu01);
+    for (int i = 0; i < NPASS; i++) load_src(i, dent, d01, nd1);
     store_lds(0, nu0);
   }
 #pragma unroll
@@ -1695,10 +1696,10 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
 
   for (int k = k0; k < k1; k++) {
     const int buf = (k - k0) & 1;
-    const int nu1 = nu_l(k + 1), nu2 = nu_l(k + 2), u02 = u0_l(k + 2);
+    const int nd1 = nd_l(k + 1), nd2 = nd_l(k + 2), d02 = d0_l(k + 2);
     // offsets at which a target of the wave's group has a partner
     const uint32_t wm = __builtin_amdgcn_readfirstlane(m_g[4 * (k - k0) + w]);
-    const uint32_t* xb = Xl[buf];
+    const uint32_t* xb = Xl;
     // staged positions of the partners of target (w, row) at offsets 2j + pa, j < 13, read at
     // once so that the operand reads of a pair wait for no other LDS read
     u32x4 pw[2];
@@ -1725,9 +1726,11 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
 #pragma unroll
         for (int p = 0; p < 3; p++) an[p] = aop(pn, p);
       }
-      if (j < NPASS) {
-        load_val(j, nu1);       // tile k+1, from the slots read during tile k-1
-        load_src(j, u02, nu2);  // tile k+2
+      if (j < NPASS && EPP * j < nd1) {
+        load_val(j, nd1);  // tile k+1, from the entries read during tile k-1
+      }
+      if (j < NPASS && EPP * j < nd2) {
+        load_src(j, dent, d02, nd2);  // tile k+2
       }
       if (j == NPASS) load_uw(k + 1);
       __builtin_amdgcn_sched_barrier(0);
@@ -1741,7 +1744,7 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
       }
       __builtin_amdgcn_sched_barrier(0);
     }
-    store_lds(buf ^ 1, nu1);  // tile k+1 (the other buffer is free since the last barrier)
+    store_lds(buf ^ 1, nd1);  // tile k+1 (slots that tile k does not read; the other table buffer is free since the last barrier)
     {
       const int t0 = m_t[k - k0], nt = m_t[k - k0 + 1] - t0;
       const floatx4 sum = acc0 + acc1;
@@ -2052,9 +2055,14 @@ int DeviceKernels<Real>::hadamard_union_max(int e0) {
 }
 
 template <typename Real>
+int DeviceKernels<Real>::hadamard_pool_max() {
+  return kSplitPool;
+}
+
+template <typename Real>
 void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
-                                         const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx,
-                                         const uint32_t* gmask, const int32_t* offsets, const Real* Vp, const Real* X, Real* Y, int tiles_per_block,
+                                         const int32_t* tile_u0, const int32_t* usrc, const int32_t* tile_d0,
+                                         const int32_t* dent, const uint16_t* uidx, const uint32_t* gmask, const int32_t* offsets, const Real* Vp, const Real* X, Real* Y, int tiles_per_block,
                                          const uint16_t* Vb) {
   if (n_tiles == 0) return;
   if (n_off != 26) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // the 26 V1 parent offsets
@@ -2070,8 +2078,8 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
       const int n_chunks = (n_tiles + TPB - 1) / TPB;
       const int64_t nbs = (int64_t)n_chunks * ((M_dim + 15) / 16) * 16;
       if (nbs >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-      hipLaunchKernelGGL(k_hadamard_split<250>, dim3((unsigned)nbs), dim3(256), 0, s, M_dim, n_tiles, TPB, n_chunks, tile_t0,
-                         tile_u0, usrc, uidx, gmask, Vb, X, Y);
+      hipLaunchKernelGGL((k_hadamard_split<250, kSplitPool>), dim3((unsigned)nbs), dim3(256), 0, s, M_dim, n_tiles, TPB,
+                         n_chunks, tile_t0, tile_u0, usrc, tile_d0, dent, uidx, gmask, Vb, X, Y);
     } else {
       const int n_fgt = (M_dim + 1) / 2;
       const int64_t nbt = (int64_t)n_tiles * ((n_fgt + 7) / 8) * 8;

@@ -77,11 +77,17 @@ struct DeviceKernels {
   // run of tiles one block of the FP32 scalar kernel handles.  Vb (FP32 scalar kernels only,
   // else null): the transfer blocks as three bf16 parts, layout [pair j][part p][f][lane 64][8],
   // lane l holding part p of B_{2j + l/32}[8 ((l/16) % 2) + i][l % 16], i < 8, for the offsets
-  // in the order of uidx; when given, the bf16 matrix-core kernel runs instead of the FP32 one.
+  // in the order of uidx; the FP32 scalar kernels run on the bf16 matrix cores and need it.
+  // They stage into an LDS pool of hadamard_pool_max() entries kept across a block's run of
+  // tiles: usrc entries are (source slot << 9 | pool slot), uidx holds pool slots (the pool
+  // size for none), and tile k's partners new since tile k-1 are dent[tile_d0[k] ..
+  // tile_d0[k+1]) in the same packing; each list has one readable entry past its end.  The
+  // other kernels ignore tile_d0 and dent.
   static int hadamard_union_max(int e0);
+  static int hadamard_pool_max();
   static void hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
-                             const int32_t* tile_u0, const int32_t* usrc, const uint16_t* uidx, const uint32_t* gmask,
-                             const int32_t* offsets,
+                             const int32_t* tile_u0, const int32_t* usrc, const int32_t* tile_d0, const int32_t* dent,
+                             const uint16_t* uidx, const uint32_t* gmask, const int32_t* offsets,
                              const Real* Vp, const Real* X, Real* Y, int tiles_per_block = 16, const uint16_t* Vb = nullptr);
   // inverse DFT, pick the check-surface points, dn[child] += scal * value
   static void ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn,

@@ -356,7 +356,7 @@ class PtFMMTree:
         names = ["n_nodes", "n_leaves", "max_depth", "s2m_rows", "l2t_rows", "m2m_rows", "l2l_rows", "x_rows",
                  "x_items", "v_src_slots", "v_trg_slots", "v_pairs", "u_rows", "u_items", "u_pairs", "far_items",
                  "far_pairs", "s2m_pairs", "u_src_points_streamed", "device_bytes", "v_tiles", "v_tile_partners",
-                 "v_tile_max_partners"]
+                 "v_tile_max_partners", "v_tile_staged"]
         out = np.zeros(len(names), dtype=np.int64)
         n = ffi.lib().pvfmm_amd_tree_stats(self._h, out.ctypes.data_as(ffi.c_int64_p), len(names))
         if n < 0:
