@@ -51,6 +51,41 @@ struct DBuf {
   }
 };
 
+// U2U and D2D through the rank of the check-to-equivalent factors.  The composite translations
+// are u2u = T uc2ue1 (T: 8 K0 x r, uc2ue1: r x K0 with orthonormal rows) and d2d = dc2de0 B
+// (dc2de0: K1 x r with orthogonal columns of norms 1/s_j, B: r x 8 K1), so T = u2u uc2ue1^T and
+// B = diag(s^2) dc2de0^T d2d (include/fmm_pts.txx:582-643 forms the same products).  Applied as
+// two GEMMs they cost 9 K r instead of 8 K^2 multiply-adds per row; the results are returned in
+// the padded layouts of the plan (r -> rp columns of T, rows of B, zero-filled).
+static std::vector<double> u2u_rank_factor(const Operators& O, int rp) {
+  const int K0 = O.K0(), r = O.rank_u;
+  std::vector<double> T((size_t)8 * K0 * rp, 0.0);
+  #pragma omp parallel for schedule(static)
+  for (int i = 0; i < 8 * K0; i++)
+    for (int j = 0; j < r; j++) {
+      double acc = 0;
+      for (int k = 0; k < K0; k++) acc += O.u2u[(size_t)i * K0 + k] * O.uc2ue1[(size_t)j * K0 + k];
+      T[(size_t)i * rp + j] = acc;
+    }
+  return T;
+}
+static std::vector<double> d2d_rank_factor(const Operators& O, int rp) {
+  const int K1 = O.K1(), r = O.rank_d;
+  std::vector<double> B((size_t)rp * 8 * K1, 0.0);
+  #pragma omp parallel for schedule(static)
+  for (int j = 0; j < r; j++) {
+    double n2 = 0;
+    for (int i = 0; i < K1; i++) n2 += O.dc2de0[(size_t)i * r + j] * O.dc2de0[(size_t)i * r + j];
+    double* b = &B[(size_t)j * 8 * K1];
+    for (int i = 0; i < K1; i++) {
+      const double c = O.dc2de0[(size_t)i * r + j] / n2;
+      const double* d = &O.d2d[(size_t)i * 8 * K1];
+      for (int n = 0; n < 8 * K1; n++) b[n] += c * d[n];
+    }
+  }
+  return B;
+}
+
 // LDS pool slots of the V-list tiles' staged partners for the bf16 kernel (FP32 scalar
 // kernels), which keeps one pool of `pool` staged entries across the run of tiles a block
 // handles: a partner of tile k that tile k-1 also lists keeps its slot, and only the others
@@ -232,6 +267,11 @@ class DevicePlan final : public DevicePlanBase {
   DBuf<Real> eq_, dn_, chk_;
   // operators
   DBuf<Real> uc2ue0_, uc2ue1_, dc2de0_, dc2de1_, u2u_, d2d_, V_, unit_surf_, ctmp_;
+  // U2U / D2D through the check-to-equivalent rank (u2u_rank_factor, d2d_rank_factor) when
+  // 9 r < 8 K: u2ur_ 8 K0 x ru_, d2dr_ rd_ x 8 K1; lvl_seq_ = 0, 1, ... (rows of ctmp_)
+  bool u2u_fact_ = false, d2d_fact_ = false;
+  DBuf<Real> u2ur_, d2dr_;
+  DBuf<int32_t> lvl_seq_;
   DBuf<uint16_t> Vb_;  // transfer blocks as bf16 parts (FP32 scalar kernels)
   DBuf<Real> gemm_work_;  // split-K partial tiles of the small coarse-level GEMMs
   int64_t gemm_work_elems_ = 0;
@@ -469,8 +509,12 @@ class DevicePlan final : public DevicePlanBase {
     uc2ue1_.upload(pad_rows(O_.uc2ue1, O_.rank_u, K0_, ru_), &total);
     dc2de0_.upload(pad_cols(O_.dc2de0, K1_, O_.rank_d, rd_), &total);
     dc2de1_.upload(pad_rows(O_.dc2de1, O_.rank_d, K0_, rd_), &total);
-    u2u_.upload(conv(O_.u2u), &total);
-    d2d_.upload(conv(O_.d2d), &total);
+    u2u_fact_ = 9 * ru_ < 8 * K0_;
+    d2d_fact_ = 9 * rd_ < 8 * K1_;
+    if (u2u_fact_) u2ur_.upload(conv(u2u_rank_factor(O_, ru_)), &total);
+    else u2u_.upload(conv(O_.u2u), &total);
+    if (d2d_fact_) d2dr_.upload(conv(d2d_rank_factor(O_, rd_)), &total);
+    else d2d_.upload(conv(O_.d2d), &total);
     {
       // Transfer blocks in real form for y = x B with x, y interleaved (re, im):
       //   B[2s][2t] = Re M[t][s], B[2s+1][2t] = -Im M[t][s], B[2s][2t+1] = Im M[t][s],
@@ -579,7 +623,24 @@ class DevicePlan final : public DevicePlanBase {
     n_l2t_ = n_tleaf_ = (int)tleaf.size();
     l2t_ra_.upload(l2t_ra, &total); l2t_rc_.upload(l2t_rc, &total); l2t_alpha_.upload(l2t_alpha, &total);
     l2t_seq_.upload(l2t_seq, &total);
-    ctmp_vs_ = (int64_t)std::max(1, std::max(n_s2m_, n_l2t_)) * std::max(1, std::max(ru_, rd_));
+    // ctmp_ rows: the check-to-equivalent GEMMs' leaves, and the parents of one level of the
+    // rank-factored U2U / D2D
+    int max_level_rows = 0;
+    for (int l = 0; l <= max_depth_; l++) {
+      int na = 0, nb = 0;
+      for (int i = T.level_begin[l]; i < T.level_begin[l + 1]; i++) {
+        if (T.is_leaf(i)) continue;
+        na += T.pt_cnt0(i) > 0;
+        nb += T.pt_cnt1(i) > 0;
+      }
+      max_level_rows = std::max(max_level_rows, std::max(na, nb));
+    }
+    {
+      std::vector<int32_t> seq(std::max(1, max_level_rows));
+      for (size_t i = 0; i < seq.size(); i++) seq[i] = (int32_t)i;
+      lvl_seq_.upload(seq, &total);
+    }
+    ctmp_vs_ = (int64_t)std::max(1, std::max(std::max(n_s2m_, n_l2t_), max_level_rows)) * std::max(1, std::max(ru_, rd_));
     ctmp_.alloc(ctmp_vs_ * nvb_, &total);
     t_leaf_.upload(tleaf, &total);
     eq_vs_ = (int64_t)(N + n_l2t_) * K0_;
@@ -760,7 +821,7 @@ class DevicePlan final : public DevicePlanBase {
   int64_t device_bytes_ = 0;
 
   struct AdjointPlan {
-    DBuf<Real> uc2ue0T, uc2ue1T, dc2de0T, dc2de1T, u2uT, d2dT;
+    DBuf<Real> uc2ue0T, uc2ue1T, dc2de0T, dc2de1T, u2uT, d2dT, u2urT, d2drT;
     DBuf<Real> tden;     // adjoint input gathered into target order (k1 x n_trg per vector)
     DBuf<double> sval;   // adjoint output accumulated in source order (k0 x n_src per vector)
     DBuf<int32_t> iota;  // 0, 1, 2, ...: one item per row
@@ -831,8 +892,10 @@ class DevicePlan final : public DevicePlanBase {
     a->uc2ue1T.upload(transposed(pad_rows(O_.uc2ue1, O_.rank_u, K0_, ru_), ru_, K0_), &total);  // K0 x ru
     a->dc2de0T.upload(transposed(pad_cols(O_.dc2de0, K1_, O_.rank_d, rd_), K1_, rd_), &total);  // rd x K1
     a->dc2de1T.upload(transposed(pad_rows(O_.dc2de1, O_.rank_d, K0_, rd_), rd_, K0_), &total);  // K0 x rd
-    a->u2uT.upload(transposed(conv(O_.u2u), 8 * K0_, K0_), &total);                            // K0 x 8K0
-    a->d2dT.upload(transposed(conv(O_.d2d), K1_, 8 * K1_), &total);                            // 8K1 x K1
+    if (u2u_fact_) a->u2urT.upload(transposed(conv(u2u_rank_factor(O_, ru_)), 8 * K0_, ru_), &total);  // ru x 8K0
+    else a->u2uT.upload(transposed(conv(O_.u2u), 8 * K0_, K0_), &total);                                // K0 x 8K0
+    if (d2d_fact_) a->d2drT.upload(transposed(conv(d2d_rank_factor(O_, rd_)), rd_, 8 * K1_), &total);  // 8K1 x rd
+    else a->d2dT.upload(transposed(conv(O_.d2d), K1_, 8 * K1_), &total);                                // 8K1 x K1
     a->tden.alloc((int64_t)k1_ * n_trg_ * nvb_, &total);
     a->sval.alloc((int64_t)k0_ * n_src_ * nvb_, &total);
     {
@@ -946,8 +1009,15 @@ class DevicePlan final : public DevicePlanBase {
     // D2D^T, bottom-up: parent += [8 children] x [D2D_0 .. D2D_7]^T
     for (int l = max_depth_; l >= 1; l--) {
       const LevelRows& L = *l2l_[l - 1];
-      K::gemm_rows(st_, L.n, K1_, 8 * K1_, dn_.p, L.rc.p, K1_, a.d2dT.p, dn_.p, L.ra.p, K1_, nullptr, true, gemm_work_.p,
-                   gemm_work_elems_, sp, nv, dn_vs_, dn_vs_);
+      if (d2d_fact_) {
+        K::gemm_rows(st_, L.n, rd_, 8 * K1_, dn_.p, L.rc.p, K1_, a.d2drT.p, ctmp_.p, lvl_seq_.p, rd_, nullptr, false,
+                     gemm_work_.p, gemm_work_elems_, sp, nv, dn_vs_, ctmp_vs_);
+        K::gemm_rows(st_, L.n, K1_, rd_, ctmp_.p, lvl_seq_.p, rd_, a.dc2de0T.p, dn_.p, L.ra.p, K1_, nullptr, true,
+                     gemm_work_.p, gemm_work_elems_, sp, nv, ctmp_vs_, dn_vs_);
+      } else {
+        K::gemm_rows(st_, L.n, K1_, 8 * K1_, dn_.p, L.rc.p, K1_, a.d2dT.p, dn_.p, L.ra.p, K1_, nullptr, true, gemm_work_.p,
+                     gemm_work_elems_, sp, nv, dn_vs_, dn_vs_);
+      }
     }
     // V list^T: the M2L of offset -R from the target parents' children to the source parents'
     // (the spectra buffers swap roles: Y_ holds the new sources, X_ the new targets)
@@ -965,8 +1035,15 @@ class DevicePlan final : public DevicePlanBase {
     // U2U^T, top-down: [8 children] += parent x [U2U_0 .. U2U_7]^T
     for (int l = 0; l <= max_depth_ - 1; l++) {
       const LevelRows& L = *m2m_[l];
-      K::gemm_rows(st_, L.n, 8 * K0_, K0_, eq_.p, L.rc.p, K0_, a.u2uT.p, eq_.p, L.ra.p, K0_, nullptr, true, gemm_work_.p,
-                   gemm_work_elems_, sp, nv, eq_vs_, eq_vs_);
+      if (u2u_fact_) {
+        K::gemm_rows(st_, L.n, ru_, K0_, eq_.p, L.rc.p, K0_, a.uc2ue1T.p, ctmp_.p, lvl_seq_.p, ru_, nullptr, false,
+                     gemm_work_.p, gemm_work_elems_, sp, nv, eq_vs_, ctmp_vs_);
+        K::gemm_rows(st_, L.n, 8 * K0_, ru_, ctmp_.p, lvl_seq_.p, ru_, a.u2urT.p, eq_.p, L.ra.p, K0_, nullptr, true,
+                     gemm_work_.p, gemm_work_elems_, sp, nv, ctmp_vs_, eq_vs_);
+      } else {
+        K::gemm_rows(st_, L.n, 8 * K0_, K0_, eq_.p, L.rc.p, K0_, a.u2uT.p, eq_.p, L.ra.p, K0_, nullptr, true, gemm_work_.p,
+                     gemm_work_elems_, sp, nv, eq_vs_, eq_vs_);
+      }
     }
     // S2U GEMMs^T: upward equivalent -> check potential of each source leaf
     K::gemm_rows(st_, n_s2m_, ru_, K0_, eq_.p, s2m_node_.p, K0_, a.uc2ue1T.p, ctmp_.p, s2m_orow_.p, ru_, nullptr, false,
@@ -1015,8 +1092,15 @@ class DevicePlan final : public DevicePlanBase {
     // U2U, bottom-up: parent row = [8 children] x stacked U2U
     for (int l = max_depth_ - 1; l >= 0; l--) {
       const LevelRows& L = *m2m_[l];
-      K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false, gemm_work_.p,
-                   gemm_work_elems_, sp, nv, eq_vs_, eq_vs_);
+      if (u2u_fact_) {  // [8 children] -> rank coordinates -> parent
+        K::gemm_rows(st_, L.n, ru_, 8 * K0_, eq_.p, L.ra.p, K0_, u2ur_.p, ctmp_.p, lvl_seq_.p, ru_, nullptr, false,
+                     gemm_work_.p, gemm_work_elems_, sp, nv, eq_vs_, ctmp_vs_);
+        K::gemm_rows(st_, L.n, K0_, ru_, ctmp_.p, lvl_seq_.p, ru_, uc2ue1_.p, eq_.p, L.rc.p, K0_, nullptr, false,
+                     gemm_work_.p, gemm_work_elems_, sp, nv, ctmp_vs_, eq_vs_);
+      } else {
+        K::gemm_rows(st_, L.n, K0_, 8 * K0_, eq_.p, L.ra.p, K0_, u2u_.p, eq_.p, L.rc.p, K0_, nullptr, false, gemm_work_.p,
+                     gemm_work_elems_, sp, nv, eq_vs_, eq_vs_);
+      }
     }
     mark(4);
     // X list: source points -> downward check surface (RAD0)
@@ -1059,8 +1143,15 @@ class DevicePlan final : public DevicePlanBase {
     // D2D, top-down: 8 children rows += parent x [D2D_0 .. D2D_7]
     for (int l = 1; l <= max_depth_; l++) {
       const LevelRows& L = *l2l_[l - 1];
-      K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true, gemm_work_.p,
-                   gemm_work_elems_, sp, nv, dn_vs_, dn_vs_);
+      if (d2d_fact_) {  // parent check -> rank coordinates -> [8 children]
+        K::gemm_rows(st_, L.n, rd_, K1_, dn_.p, L.ra.p, K1_, dc2de0_.p, ctmp_.p, lvl_seq_.p, rd_, nullptr, false,
+                     gemm_work_.p, gemm_work_elems_, sp, nv, dn_vs_, ctmp_vs_);
+        K::gemm_rows(st_, L.n, 8 * K1_, rd_, ctmp_.p, lvl_seq_.p, rd_, d2dr_.p, dn_.p, L.rc.p, K1_, nullptr, true,
+                     gemm_work_.p, gemm_work_elems_, sp, nv, ctmp_vs_, dn_vs_);
+      } else {
+        K::gemm_rows(st_, L.n, 8 * K1_, K1_, dn_.p, L.ra.p, K1_, d2d_.p, dn_.p, L.rc.p, K1_, nullptr, true, gemm_work_.p,
+                     gemm_work_elems_, sp, nv, dn_vs_, dn_vs_);
+      }
     }
     mark(9);
     // D2T: check -> equivalent

@@ -1648,10 +1648,14 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
   auto load_src = [&](int i, const int32_t* list, int at, int n) __attribute__((always_inline)) {
     src[i] = list[at + min(su + EPP * i, max(n - 1, 0))];
   };
+  // X row of source slot s at frequency f: xf + s * xstride bytes (M_dim * 64 < 2^32: the
+  // host bounds M_dim)
+  const char* xf = reinterpret_cast<const char*>(X + (int64_t)f * RW + 4 * sr);
+  const uint32_t xstride = (uint32_t)M_dim * RW * sizeof(float);
   auto load_val = [&](int i, int n) __attribute__((always_inline)) {
     const int e = su + EPP * i < n ? src[i] : POOL + 1;  // a select on the input keeps it branch-free
     dst[i] = e & 511;
-    val[i] = *reinterpret_cast<const floatx4*>(X + ((int64_t)(e >> 9) * M_dim + f) * RW + 4 * sr);
+    val[i] = *reinterpret_cast<const floatx4*>(xf + (uint64_t)((uint32_t)e >> 9) * xstride);
   };
   auto load_uw = [&](int k) __attribute__((always_inline)) {
     const uint2* ug = reinterpret_cast<const uint2*>(uidx + (int64_t)min(k, k1 - 1) * UT);
@@ -1687,6 +1691,10 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
 #pragma unroll
     for (int i = 0; i < NPASS; i++) load_src(i, dent, d01, nd1);
     store_lds(0, nu0);
+    // the loop's first use of src[] then waits for nothing: without this wait the compiler
+    // carries the prologue's outstanding loads into the loop's wait counts, where they would
+    // make every tile wait for the previous tile's Y stores
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   }
 #pragma unroll
   for (int j = 0; j < NPAIR; j++)
@@ -1748,11 +1756,10 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
     {
       const int t0 = m_t[k - k0], nt = m_t[k - k0 + 1] - t0;
       const floatx4 sum = acc0 + acc1;
+      float* yr = Y + ((int64_t)(t0 + 16 * w + 4 * kg) * M_dim + f) * RW + row;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int tg = 16 * w + 4 * kg + r;
-        if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * RW + row] = sum[r];
-      }
+      for (int r = 0; r < 4; r++)
+        if (16 * w + 4 * kg + r < nt) yr[(int64_t)r * M_dim * RW] = sum[r];
     }
     __syncthreads();
   }
@@ -2073,7 +2080,7 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
   if (e0 == 1 && e1 == 1) {
     if constexpr (std::is_same<Real, float>::value) {
       // runs of tiles_per_block (at most 64) tiles per block, one frequency per block
-      if (!Vb) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+      if (!Vb || (int64_t)M_dim * 64 >= (1ll << 32)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
       const int TPB = std::min(64, std::max(1, tiles_per_block));
       const int n_chunks = (n_tiles + TPB - 1) / TPB;
       const int64_t nbs = (int64_t)n_chunks * ((M_dim + 15) / 16) * 16;
