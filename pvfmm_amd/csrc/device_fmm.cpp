@@ -268,7 +268,10 @@ class DevicePlan final : public DevicePlanBase {
   // operators
   DBuf<Real> uc2ue0_, uc2ue1_, dc2de0_, dc2de1_, u2u_, d2d_, V_, unit_surf_, ctmp_;
   // U2U / D2D through the check-to-equivalent rank (u2u_rank_factor, d2d_rank_factor) when
-  // 9 r < 8 K: u2ur_ 8 K0 x ru_, d2dr_ rd_ x 8 K1; lvl_seq_ = 0, 1, ... (rows of ctmp_)
+  // 9 r < 8 K, at levels of at least kFactorRows parents (a smaller level costs less as one
+  // GEMM than as two launches): u2ur_ 8 K0 x ru_, d2dr_ rd_ x 8 K1; lvl_seq_ = 0, 1, ... (rows
+  // of ctmp_)
+  static constexpr int kFactorRows = 256;
   bool u2u_fact_ = false, d2d_fact_ = false;
   DBuf<Real> u2ur_, d2dr_;
   DBuf<int32_t> lvl_seq_;
@@ -512,9 +515,9 @@ class DevicePlan final : public DevicePlanBase {
     u2u_fact_ = 9 * ru_ < 8 * K0_;
     d2d_fact_ = 9 * rd_ < 8 * K1_;
     if (u2u_fact_) u2ur_.upload(conv(u2u_rank_factor(O_, ru_)), &total);
-    else u2u_.upload(conv(O_.u2u), &total);
     if (d2d_fact_) d2dr_.upload(conv(d2d_rank_factor(O_, rd_)), &total);
-    else d2d_.upload(conv(O_.d2d), &total);
+    u2u_.upload(conv(O_.u2u), &total);
+    d2d_.upload(conv(O_.d2d), &total);
     {
       // Transfer blocks in real form for y = x B with x, y interleaved (re, im):
       //   B[2s][2t] = Re M[t][s], B[2s+1][2t] = -Im M[t][s], B[2s][2t+1] = Im M[t][s],
@@ -893,9 +896,9 @@ class DevicePlan final : public DevicePlanBase {
     a->dc2de0T.upload(transposed(pad_cols(O_.dc2de0, K1_, O_.rank_d, rd_), K1_, rd_), &total);  // rd x K1
     a->dc2de1T.upload(transposed(pad_rows(O_.dc2de1, O_.rank_d, K0_, rd_), rd_, K0_), &total);  // K0 x rd
     if (u2u_fact_) a->u2urT.upload(transposed(conv(u2u_rank_factor(O_, ru_)), 8 * K0_, ru_), &total);  // ru x 8K0
-    else a->u2uT.upload(transposed(conv(O_.u2u), 8 * K0_, K0_), &total);                                // K0 x 8K0
     if (d2d_fact_) a->d2drT.upload(transposed(conv(d2d_rank_factor(O_, rd_)), rd_, 8 * K1_), &total);  // 8K1 x rd
-    else a->d2dT.upload(transposed(conv(O_.d2d), K1_, 8 * K1_), &total);                                // 8K1 x K1
+    a->u2uT.upload(transposed(conv(O_.u2u), 8 * K0_, K0_), &total);                                     // K0 x 8K0
+    a->d2dT.upload(transposed(conv(O_.d2d), K1_, 8 * K1_), &total);                                     // 8K1 x K1
     a->tden.alloc((int64_t)k1_ * n_trg_ * nvb_, &total);
     a->sval.alloc((int64_t)k0_ * n_src_ * nvb_, &total);
     {
@@ -1009,7 +1012,7 @@ class DevicePlan final : public DevicePlanBase {
     // D2D^T, bottom-up: parent += [8 children] x [D2D_0 .. D2D_7]^T
     for (int l = max_depth_; l >= 1; l--) {
       const LevelRows& L = *l2l_[l - 1];
-      if (d2d_fact_) {
+      if (d2d_fact_ && L.n >= kFactorRows) {
         K::gemm_rows(st_, L.n, rd_, 8 * K1_, dn_.p, L.rc.p, K1_, a.d2drT.p, ctmp_.p, lvl_seq_.p, rd_, nullptr, false,
                      gemm_work_.p, gemm_work_elems_, sp, nv, dn_vs_, ctmp_vs_);
         K::gemm_rows(st_, L.n, K1_, rd_, ctmp_.p, lvl_seq_.p, rd_, a.dc2de0T.p, dn_.p, L.ra.p, K1_, nullptr, true,
@@ -1035,7 +1038,7 @@ class DevicePlan final : public DevicePlanBase {
     // U2U^T, top-down: [8 children] += parent x [U2U_0 .. U2U_7]^T
     for (int l = 0; l <= max_depth_ - 1; l++) {
       const LevelRows& L = *m2m_[l];
-      if (u2u_fact_) {
+      if (u2u_fact_ && L.n >= kFactorRows) {
         K::gemm_rows(st_, L.n, ru_, K0_, eq_.p, L.rc.p, K0_, a.uc2ue1T.p, ctmp_.p, lvl_seq_.p, ru_, nullptr, false,
                      gemm_work_.p, gemm_work_elems_, sp, nv, eq_vs_, ctmp_vs_);
         K::gemm_rows(st_, L.n, 8 * K0_, ru_, ctmp_.p, lvl_seq_.p, ru_, a.u2urT.p, eq_.p, L.ra.p, K0_, nullptr, true,
@@ -1092,7 +1095,7 @@ class DevicePlan final : public DevicePlanBase {
     // U2U, bottom-up: parent row = [8 children] x stacked U2U
     for (int l = max_depth_ - 1; l >= 0; l--) {
       const LevelRows& L = *m2m_[l];
-      if (u2u_fact_) {  // [8 children] -> rank coordinates -> parent
+      if (u2u_fact_ && L.n >= kFactorRows) {  // [8 children] -> rank coordinates -> parent
         K::gemm_rows(st_, L.n, ru_, 8 * K0_, eq_.p, L.ra.p, K0_, u2ur_.p, ctmp_.p, lvl_seq_.p, ru_, nullptr, false,
                      gemm_work_.p, gemm_work_elems_, sp, nv, eq_vs_, ctmp_vs_);
         K::gemm_rows(st_, L.n, K0_, ru_, ctmp_.p, lvl_seq_.p, ru_, uc2ue1_.p, eq_.p, L.rc.p, K0_, nullptr, false,
@@ -1143,7 +1146,7 @@ class DevicePlan final : public DevicePlanBase {
     // D2D, top-down: 8 children rows += parent x [D2D_0 .. D2D_7]
     for (int l = 1; l <= max_depth_; l++) {
       const LevelRows& L = *l2l_[l - 1];
-      if (d2d_fact_) {  // parent check -> rank coordinates -> [8 children]
+      if (d2d_fact_ && L.n >= kFactorRows) {  // parent check -> rank coordinates -> [8 children]
         K::gemm_rows(st_, L.n, rd_, K1_, dn_.p, L.ra.p, K1_, dc2de0_.p, ctmp_.p, lvl_seq_.p, rd_, nullptr, false,
                      gemm_work_.p, gemm_work_elems_, sp, nv, dn_vs_, ctmp_vs_);
         K::gemm_rows(st_, L.n, 8 * K1_, rd_, ctmp_.p, lvl_seq_.p, rd_, d2dr_.p, dn_.p, L.rc.p, K1_, nullptr, true,

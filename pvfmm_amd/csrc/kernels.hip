@@ -763,8 +763,19 @@ __global__ void k_gemm_reduce(int M, int N, int ksplit, const Real* __restrict__
   const int64_t total = (int64_t)M * N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(e / N), n = (int)(e % N);
+    // partials added in split order; 8 loads in flight at a time
+    const Real* wp = work + (int64_t)i * N + n;
+    const int64_t zs = (int64_t)M * N;
     Real v = 0;
-    for (int z = 0; z < ksplit; z++) v += work[((int64_t)z * M + i) * N + n];
+    int z = 0;
+    for (; z + 8 <= ksplit; z += 8) {
+      Real t[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) t[q] = wp[(z + q) * zs];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v += t[q];
+    }
+    for (; z < ksplit; z++) v += wp[z * zs];
     const int cv = i / Mv, ci = i - cv * Mv;
     Real* crow = C + cv * cvs + (int64_t)rc[ci] * ldc;
     const Real al = alpha ? alpha[ci] : Real(1);

@@ -123,11 +123,15 @@ def test_adaptive_grad_2m_full_size(gpu):
     assert errs["fp64"] < 1e-8
 
 
-@pytest.mark.parametrize("prec,tol_or", [("mixed", 1e-5), ("fp64", 1e-8)])
+@pytest.mark.parametrize("prec,tol_or", [("mixed", 2e-6), ("fp64", 1e-8)])
 def test_ellipsoid_stokes_sample_vs_oracle(gpu, prec, tol_or):
     """stokes_vel on 2^18 points of the ellipsoid surface (semi-axes 0.45, 0.3, 0.15), m=8,
     max_pts=200, against the oracle on the same tree.  The V-list tiles of this surface hold
     more than 80 distinct partners (the Stokes tile cap is 160), which the check asserts.
+
+    The mixed oracle is the same algorithm (same cutoff) in FP64: the evaluator's FP32 rounding
+    is its distance from the oracle (3.4e-7 measured), which adds to the algorithm's rel-L2
+    (1.06e-6) in quadrature; the check bounds it to at most a quarter of that error.
 
     In FP64 the reference's cutoff 4 eps_64 max_S keeps singular values of the Stokes
     check-to-equivalent systems down to about 1e-16 max_S; the oracle (LAPACK SVD) and this
@@ -147,7 +151,10 @@ def test_ellipsoid_stokes_sample_vs_oracle(gpu, prec, tol_or):
     self_check(tree, u, c["src"], c["den"], 2, 3, f"stokes_vel {prec}")
     assert st["v_tile_max_partners"] > 80
     assert e_or < tol_or
-    assert abs(e_l2 - e_l2_or) < 0.05 * e_l2_or
+    if prec == "fp64":
+        assert abs(e_l2 - e_l2_or) < 0.05 * e_l2_or
+    else:
+        assert 0.95 * e_l2_or < e_l2 < 1.25 * e_l2_or
     assert e_l2 < 5e-5
 
 
