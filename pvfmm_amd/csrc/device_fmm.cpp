@@ -275,7 +275,12 @@ class DevicePlan final : public DevicePlanBase {
   bool u2u_fact_ = false, d2d_fact_ = false;
   DBuf<Real> u2ur_, d2dr_;
   DBuf<int32_t> lvl_seq_;
-  DBuf<uint16_t> Vb_;  // transfer blocks as bf16 parts (FP32 scalar kernels)
+  DBuf<uint16_t> Vb_;  // transfer blocks as bf16 parts (FP32 scalar kernels, 16-vector passes)
+  // the fp16 form of the FP32 scalar Hadamard: transfer parts, 1 / tau_f, the spectrum bound of
+  // each slot of X (or, in the adjoint, of Y) and the scale of each run of tiles
+  DBuf<uint16_t> Vh_;
+  DBuf<float> vtinv_, vbound_, vrscale_;
+  bool split_ = false;
   DBuf<Real> gemm_work_;  // split-K partial tiles of the small coarse-level GEMMs
   int64_t gemm_work_elems_ = 0;
   int nvb_ = 1;  // density vectors per evaluation pass (buffers hold this many)
@@ -312,6 +317,16 @@ class DevicePlan final : public DevicePlanBase {
     int64_t partners = 0, max_partners = 0, staged_delta = 0;
   };
   TileSet vt_;
+  // operands of the fp16 Hadamard with run scales in `rs` (kept until the launch returns)
+  HadamardParts hp_;
+  const HadamardParts& hparts(DBuf<float>& rs) {
+    hp_.Vh = Vh_.p;
+    hp_.tinv = vtinv_.p;
+    hp_.bound = vbound_.p;
+    hp_.run_scale = rs.p;
+    hp_.run_scale_cap = (int)rs.n;
+    return hp_;
+  }
 
   // Tiles of consecutive target slots: at most 64 targets, at most umax distinct partners
   // (greedy in slot order, which is Morton order within a level).  partner[t * n_off + o] is
@@ -579,6 +594,40 @@ class DevicePlan final : public DevicePlanBase {
           }
         }
         Vb_.upload(Vb, &total);
+        // two fp16 parts of tau_f B for k_hadamard_split: tau_f = 2^(14 - e) with every entry
+        // of frequency f's blocks below 2^e (so below 2^14 after scaling); part 0 rounds tau_f t
+        // to nearest, part 1 the remainder
+        std::vector<uint16_t> Vh((size_t)n_pair * 2 * M_dim_ * 64 * 8);
+        std::vector<float> tinv(M_dim_);
+        auto entry = [&](int j, int f, int l, int i) {
+          const int R = O_.v1_offsets[2 * j + (l >> 5)];
+          const std::complex<double> v = O_.v1[((size_t)R * M_dim_ + f) * NR * NS + (size_t)((l & 15) >> 1) * NS +
+                                               ((8 * ((l >> 4) & 1) + i) >> 1)];
+          const int k = 8 * ((l >> 4) & 1) + i, n = l & 15;
+          return ((k & 1) == 0) ? ((n & 1) == 0 ? v.real() : v.imag()) : ((n & 1) == 0 ? -v.imag() : v.real());
+        };
+        #pragma omp parallel for schedule(static)
+        for (int f = 0; f < M_dim_; f++) {
+          double mx = 0;
+          for (int j = 0; j < n_pair; j++)
+            for (int l = 0; l < 64; l++)
+              for (int i = 0; i < 8; i++) mx = std::max(mx, std::fabs(entry(j, f, l, i)));
+          int e = 0;
+          if (mx > 0) (void)std::frexp(mx, &e);
+          const double tau = mx > 0 ? std::ldexp(1.0, std::min(100, std::max(-100, 14 - e))) : 1.0;
+          tinv[f] = (float)(1.0 / tau);
+          for (int j = 0; j < n_pair; j++)
+            for (int l = 0; l < 64; l++)
+              for (int i = 0; i < 8; i++) {
+                const double x = entry(j, f, l, i) * tau;
+                const _Float16 h0 = (_Float16)x, h1 = (_Float16)(x - (double)h0);
+                std::memcpy(&Vh[((((size_t)j * 2 + 0) * M_dim_ + f) * 64 + l) * 8 + i], &h0, 2);
+                std::memcpy(&Vh[((((size_t)j * 2 + 1) * M_dim_ + f) * 64 + l) * 8 + i], &h1, 2);
+              }
+        }
+        Vh_.upload(Vh, &total);
+        vtinv_.upload(tinv, &total);
+        split_ = true;
       }
     }
     unit_surf_.upload(conv(O_.unit_surf), &total);
@@ -736,6 +785,10 @@ class DevicePlan final : public DevicePlanBase {
     v_partner_.upload(v_partner, &total); v_offsets_.upload(offs, &total);
     h_.v_partner = v_partner; h_.v_src_fc = v_src_fc; h_.v_trg_fc = v_trg_fc; h_.offs = offs;
     build_tiles(v_partner, n_vtrg_, n_vsrc_, vt_, &total);
+    if (split_) {
+      vbound_.alloc(std::max(1, std::max(n_vsrc_, n_vtrg_)), &total);  // the adjoint's sources are the target slots
+      vrscale_.alloc(std::max(1, vt_.n), &total);
+    }
     stats.v_tiles = vt_.n;
     stats.v_tile_partners = vt_.partners;
     stats.v_tile_max_partners = vt_.max_partners;
@@ -839,6 +892,7 @@ class DevicePlan final : public DevicePlanBase {
     DBuf<int64_t> s2u_den;                 // S2U: source leaf <- its RAD1 check surface
     DBuf<double> s2u_rad;
     TileSet vt;                            // V list: source parents <- target parents
+    DBuf<float> vrscale;                   // run scales of vt (fp16 Hadamard)
     DBuf<int32_t> v_fc;                    // first child of each transposed target slot
     DBuf<Real> fft_scal, ifft_scal;
     int n_vt = 0;
@@ -976,6 +1030,7 @@ class DevicePlan final : public DevicePlanBase {
       for (int t = 0; t < n_vtrg_; t++) fscal[t] = (Real)std::pow(2.0, ki.src_scal * (h_.vtrg_depth[t] + 1));
       a->n_vt = (int)fc.size();
       build_tiles(partner, a->n_vt, n_vtrg_, a->vt, &total);
+      if (split_) a->vrscale.alloc(std::max(1, a->vt.n), &total);
       a->v_fc.upload(fc, &total);
       a->fft_scal.upload(fscal, &total);
       a->ifft_scal.upload(iscal, &total);
@@ -1025,10 +1080,11 @@ class DevicePlan final : public DevicePlanBase {
     // V list^T: the M2L of offset -R from the target parents' children to the source parents'
     // (the spectra buffers swap roles: Y_ holds the new sources, X_ the new targets)
     for (int v = 0; v < nv; v++) {
-      K::fft_up(st_, n_vtrg_, v_trg_fc_.p, dn_.p + v * dn_vs_, K1_, e1_, m_, lattice_.p, lattice_inv_.p, a.fft_scal.p, Y_.p);
+      K::fft_up(st_, n_vtrg_, v_trg_fc_.p, dn_.p + v * dn_vs_, K1_, e1_, m_, lattice_.p, lattice_inv_.p, a.fft_scal.p, Y_.p,
+                1, 0, split_ ? vbound_.p : nullptr);
       K::hadamard_tiled(st_, a.vt.n, M_dim_, e0_, e1_, n_off_, a.vt.t0.p, a.vt.u0.p, a.vt.usrc.p, a.vt.d0.p, a.vt.dent.p,
-                        a.vt.uidx.p, a.vt.gmask.p,
-                        v_offsets_.p, V_.p, Y_.p, X_.p, tuning.hadamard_tpb, Vb_.p);
+                        a.vt.uidx.p, a.vt.gmask.p, v_offsets_.p, V_.p, Y_.p, X_.p, tuning.hadamard_tpb,
+                        split_ ? &hparts(a.vrscale) : nullptr);
       K::ifft_down(st_, a.n_vt, a.v_fc.p, X_.p, eq_.p + v * eq_vs_, K0_, e0_, m_, lattice_.p, lattice_inv_.p, ns_,
                    a.ifft_scal.p);
     }
@@ -1122,15 +1178,16 @@ class DevicePlan final : public DevicePlanBase {
       if (sub) HIP_CHECK(hipEventRecord(vev_[4 * g], st_));
       for (int v = v0; v < v0 + gn; v++)
         K::fft_up(st_, n_vsrc_, v_src_fc_.p, eq_.p + v * eq_vs_, K0_, e0_, m_, lattice_.p, lattice_inv_.p, v_src_scal_.p,
-                  grouped ? Xv_.p : X_.p, grouped ? 16 : 1, grouped ? v - v0 : 0);
+                  grouped ? Xv_.p : X_.p, grouped ? 16 : 1, grouped ? v - v0 : 0,
+                  split_ && !grouped ? vbound_.p : nullptr);
       if (sub) HIP_CHECK(hipEventRecord(vev_[4 * g + 1], st_));
       if (n_pass == 1) mark(6);
       if (grouped)
         K::hadamard_vec16(st_, n_vtrg_, M_dim_, v_partner_.p, Vb_.p, Xv_.p, Yv_.p, tuning.hadamard_vec_tpw);
       else
         K::hadamard_tiled(st_, vt_.n, M_dim_, e0_, e1_, n_off_, vt_.t0.p, vt_.u0.p, vt_.usrc.p, vt_.d0.p, vt_.dent.p,
-                          vt_.uidx.p, vt_.gmask.p,
-                          v_offsets_.p, V_.p, X_.p, Y_.p, tuning.hadamard_tpb, Vb_.p);
+                          vt_.uidx.p, vt_.gmask.p, v_offsets_.p, V_.p, X_.p, Y_.p, tuning.hadamard_tpb,
+                          split_ ? &hparts(vrscale_) : nullptr);
       if (sub) HIP_CHECK(hipEventRecord(vev_[4 * g + 2], st_));
       if (n_pass == 1) mark(7);
       for (int v = v0; v < v0 + gn; v++)

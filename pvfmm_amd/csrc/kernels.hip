@@ -1038,6 +1038,60 @@ __device__ __forceinline__ void cmac_tw(Real gr, Real gi, Real& sr, Real& si) {
   }
 }
 
+// The spectrum bound of k_fft_up_m for the generic-m transform: bound[slot] = |scal[slot]| *
+// sum of |up| over the 8 children's ns values (component 0), in a fixed order.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_spectrum_bound(int n_slots, const int32_t* __restrict__ first_child,
+                                                        const Real* __restrict__ up, int K0, int ns,
+                                                        const Real* __restrict__ scal, float* __restrict__ bound) {
+  __shared__ Real red[256];
+  const int slot = blockIdx.x, t = threadIdx.x;
+  if (slot >= n_slots) return;  // whole block
+  const int fc = first_child[slot];
+  Real a = 0;
+  for (int q = t; q < 8 * ns; q += 256) a += fabs(up[(int64_t)(fc + q / ns) * K0 + q % ns]);
+  red[t] = a;
+  __syncthreads();
+  if (t < 64) {
+    Real b = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
+    if (t == 0) bound[slot] = (float)(b * fabs(scal[slot]));
+  }
+}
+
+// Scale of each run of tpb consecutive V-list tiles for the fp16 Hadamard: the power of two s
+// with s * b < 2^14 <= 2 s * b, b the largest spectrum bound of the sources the run stages (its
+// first tile's partners and every later tile's new ones, as (slot << 9 | pool slot)); 1 when b
+// is 0 (or not finite).
+__global__ __launch_bounds__(256) void k_run_scale(int n_tiles, int tpb, const int32_t* __restrict__ tile_u0,
+                                                   const int32_t* __restrict__ usrc, const int32_t* __restrict__ tile_d0,
+                                                   const int32_t* __restrict__ dent, const float* __restrict__ bound,
+                                                   float* __restrict__ run_scale) {
+  __shared__ float red[256];
+  const int r = blockIdx.x, t = threadIdx.x, k0 = r * tpb, k1 = min(n_tiles, k0 + tpb);
+  float mx = 0.f;
+  for (int u = tile_u0[k0] + t; u < tile_u0[k0 + 1]; u += 256) mx = fmaxf(mx, bound[usrc[u] >> 9]);
+  if (k0 + 1 < k1)
+    for (int d = tile_d0[k0 + 1] + t; d < tile_d0[k1]; d += 256) mx = fmaxf(mx, bound[dent[d] >> 9]);
+  red[t] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] = fmaxf(red[t], red[t + o]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    const float b = red[0];
+    float sc = 1.f;
+    if (b > 0.f && b <= 3.4e38f) {
+      int e = 0;
+      (void)frexpf(b, &e);  // b < 2^e
+      sc = ldexpf(1.f, min(100, max(-100, 14 - e)));
+    }
+    run_scale[r] = sc;
+  }
+}
+
 // Shape of the fixed-m V-list transforms.  Lattice lines (c, y, z) are owned by threads:
 // NT = m^2 * CG threads, thread t -> (y, z) = t % m^2 and children cg + CG*q (q < LPT).
 // KC x-frequency planes of the intermediate are staged in LDS at a time, sized so that two
@@ -1070,7 +1124,7 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
                                                                          const Real* __restrict__ up, int K0, int e0,
                                                                          const int32_t* __restrict__ lat_inv,
                                                                          const Real* __restrict__ scal, Real* __restrict__ X,
-                                                                         int n_slots, int vg, int vi) {
+                                                                         int n_slots, int vg, int vi, float* __restrict__ bound) {
   using S = DftShape<Real, M>;
   constexpr int N1 = S::N1, NH = S::NH, MD = S::MD, M2 = S::M2, NT = S::NT, LPT = S::LPT, CG = S::CG, KC = S::KC_UP;
   constexpr int CST = KC * M2 + 1;  // child stride (odd, spreads the children over LDS banks)
@@ -1092,6 +1146,24 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
     const int p = lat_inv[(z * M + y) * M + x];
 #pragma unroll
     for (int q = 0; q < LPT; q++) v[q][x] = p >= 0 ? up[(int64_t)(fc + cg + CG * q) * K0 + p * e0 + i] * sc : Real(0);
+  }
+  if (bound) {
+    // sum of |v| over the slot's lattice values: no frequency of the spectrum exceeds it (unit
+    // twiddles); added in a fixed order (per thread, then the first wave, NT >= 64)
+    Real a = 0;
+#pragma unroll
+    for (int q = 0; q < LPT; q++)
+#pragma unroll
+      for (int x = 0; x < M; x++) a += fabs(v[q][x]);
+    s1r[t] = a;  // s1r is free until the x stage
+    __syncthreads();
+    if (t < 64) {
+      Real b = 0;
+      for (int u = t; u < NT; u += 64) b += s1r[u];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
+      if (t == 0) bound[slot] = (float)b;
+    }
   }
   __syncthreads();
   Real* Xs = X + (((int64_t)slot * MD * vg + vi) * 8 * e0 + i) * 2;
@@ -1553,28 +1625,40 @@ __global__ __launch_bounds__(256, (hadamard_blocks_per_cu<Real, E0, FB, UMAX>())
       }
 }
 
-// FP32 frequency-domain M2L for the scalar kernels on the bf16 matrix cores: every operand x
-// is split exactly into three bf16 parts x = x0 + x1 + x2 (x0 the top 8 significant bits, x1
-// the next 8, x2 the rest; |x1| < 2^-7 |x|, |x2| < 2^-14 |x|), each product of parts is exact
-// in FP32, and the six largest products are summed in FP32:
-//   x.t ~ x0 t0 + (x0 t1 + x1 t0) + (x0 t2 + x1 t1 + x2 t0);
-// the dropped x1 t2 + x2 t1 + x2 t2 stay below 2^-21 |x t| (transfer parts are rounded to
-// nearest: |t1| <= 2^-8 |t|, |t2| <= 2^-16 |t|).
-// One v_mfma_f32_16x16x32_bf16 covers two parent offsets (k = 16 a + real index, a the offset
-// of the pair), so an offset pair of a 16-target group costs 6 MFMAs of K = 32 instead of the 8
+// The three bf16 parts of an FP32 value x = x0 + x1 + x2 (x0 the top 8 significant bits, x1
+// the next 8, x2 the rest; each step exact): the operand split of k_hadamard_vec16, whose six
+// largest products x0 t0 + (x0 t1 + x1 t0) + (x0 t2 + x1 t1 + x2 t0) leave out less than
+// 2^-21 |x t| (transfer parts rounded to nearest: |t1| <= 2^-8 |t|, |t2| <= 2^-16 |t|).
+//
+// FP32 frequency-domain M2L for the scalar kernels (k_hadamard_split) on the fp16 matrix
+// cores: operands are scaled by powers of two into fp16 range and split exactly into two fp16
+// parts x = x0 + x1 (x0 = RN(x), x1 = RN(x - x0): 22 significant bits, |x - x0 - x1| <=
+// 2^-22 |x|), and the three largest products are summed in FP32,
+//   x.t ~ x0 t0 + (x0 t1 + x1 t0),
+// each exact in FP32 (11 x 11 bits); the dropped x1 t1 stays below 2^-22 |x t|.  The spectra
+// of the sources a block stages are scaled by its run's power of two s (k_run_scale: s times
+// the largest bound of their spectra is below 2^14), the transfer blocks of frequency f by
+// tau_f (host: tau_f times the block's largest entry below 2^14), and the result is divided by
+// s tau_f (exact).  Spectra far below their run's largest bound lose relative precision in x1
+// (fp16 subnormals) but stay within about 2^-24 of that bound, the FP32 rounding of the run's
+// largest values.
+// One v_mfma_f32_16x16x32_f16 covers two parent offsets (k = 16 a + real index, a the offset
+// of the pair), so an offset pair of a 16-target group costs 3 MFMAs of K = 32 instead of the 8
 // FP32 MFMAs of K = 4 per offset.  A block owns one frequency and a run of consecutive tiles;
 // wave w computes target group w (targets 16w..16w+15 of a tile) over the 13 offset pairs, with
 // the transfer-block parts of its frequency held in registers for the run.  The partners'
-// spectra are staged in LDS as their three bf16 parts (split while staging) in two buffers:
-// while a tile's MFMAs run from one buffer, the next tile's partner table and spectra are read
-// from global memory and written to the other, one barrier per tile.  Pairs at which none of
-// the group's targets has a partner are skipped.  Two blocks run per CU.
+// spectra are staged in LDS as their two fp16 parts (scaled and split while staging): while a
+// tile's MFMAs run, the next tile's partner table and spectra are read from global memory and
+// written to LDS, one barrier per tile.  Pairs at which none of the group's targets has a
+// partner are skipped.  Two blocks run per CU.
 // The staged entries form one LDS pool of POOL entries for the block's run of tiles (entry
 // POOL stays zero, entry POOL + 1 takes the writes of idle lanes): a partner that consecutive
 // tiles list stays in its slot, so after the run's first tile only the partners new since the
 // previous tile are read and staged, into slots that the previous tile does not read (the host
 // assigns the slots).  Staging passes past the length of that list are skipped.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kSplitPool = 496;
 
@@ -1598,11 +1682,12 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
                                                           const int32_t* __restrict__ tile_u0, const int32_t* __restrict__ usrc,
                                                           const int32_t* __restrict__ tile_d0, const int32_t* __restrict__ dent,
                                                           const uint16_t* __restrict__ uidx, const uint32_t* __restrict__ gmask,
-                                                          const uint16_t* __restrict__ Vb, const float* __restrict__ X,
+                                                          const uint16_t* __restrict__ Vh, const float* __restrict__ tinv,
+                                                          const float* __restrict__ run_scale, const float* __restrict__ X,
                                                           float* __restrict__ Y) {
   constexpr int NOFF = 26, NPAIR = NOFF / 2;
   constexpr int RW = 16;                  // reals per staged entry and frequency
-  constexpr int EST = 3 * RW / 2 + 4;     // words per staged entry: 3 parts x 16 bf16, padded
+  constexpr int EST = 2 * RW / 2 + 4;     // words per staged entry: 2 parts x 16 fp16, padded
   constexpr int XBUF = (POOL + 2) * EST;  // entry POOL stays zero, POOL + 1 takes unused lanes
   constexpr int UT = 4 * 16 * 2 * 16;     // partner table of a tile: [group][row][a][pair (padded to 16)]
   constexpr int UW = UT / 4;              // as 8-byte words
@@ -1628,14 +1713,16 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
   const int lane_off = 4 * (kg & 1);        // word offset of its 8 reals within a part
   const int sr = tid & 3, su = tid >> 2;    // staging: reals 4sr..4sr+3 of entries su + EPP * i
 
-  // transfer-block parts of frequency f: bq[j][p] holds, for lane (n = l%16, kg), part p of
-  // B_{2j + kg/2}[8 (kg%2) + i][n], i < 8
-  bf16x8 bq[NPAIR][3];
+  // transfer-block parts of frequency f (scaled by tau_f): bq[j][p] holds, for lane
+  // (n = l%16, kg), part p of B_{2j + kg/2}[8 (kg%2) + i][n], i < 8
+  halfx8 bq[NPAIR][2];
 #pragma unroll
   for (int j = 0; j < NPAIR; j++)
 #pragma unroll
-    for (int p = 0; p < 3; p++)
-      bq[j][p] = *reinterpret_cast<const bf16x8*>(Vb + ((((int64_t)j * 3 + p) * M_dim + f) * 64 + l) * 8);
+    for (int p = 0; p < 2; p++)
+      bq[j][p] = *reinterpret_cast<const halfx8*>(Vh + ((((int64_t)j * 2 + p) * M_dim + f) * 64 + l) * 8);
+  // the run's scale, and the factor that undoes both scales
+  const float xsc = run_scale[chunk], ysc = tinv[f] / xsc;
   for (int e = tid; e < EST; e += 256) Xl[POOL * EST + e] = 0u;
   const int nk = k1 - k0;
   for (int i = tid; i < 4 * nk; i += 256) m_g[i] = gmask[4 * k0 + i];
@@ -1678,13 +1765,16 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
 #pragma unroll
     for (int i = 0; i < NPASS; i++) {
       if (EPP * i >= n) break;  // uniform
-      uint32_t h[3][4];
+      halfx4 h0, h1;
 #pragma unroll
-      for (int q = 0; q < 4; q++) split3(val[i][q], h[0][q], h[1][q], h[2][q]);
+      for (int q = 0; q < 4; q++) {
+        const float x = val[i][q] * xsc;  // exact (power of two)
+        h0[q] = (_Float16)x;
+        h1[q] = (_Float16)(x - (float)h0[q]);
+      }
       uint32_t* d = &Xl[dst[i] * EST + 2 * sr];
-#pragma unroll
-      for (int p = 0; p < 3; p++)
-        *reinterpret_cast<uint2*>(d + 8 * p) = uint2{pack_hi(h[p][0], h[p][1]), pack_hi(h[p][2], h[p][3])};
+      *reinterpret_cast<halfx4*>(d) = h0;
+      *reinterpret_cast<halfx4*>(d + 8) = h1;
     }
 #pragma unroll
     for (int i = 0; i < UWP; i++) reinterpret_cast<uint2*>(ul[buf])[tid + 256 * i] = uw[i];
@@ -1710,7 +1800,7 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
 #pragma unroll
   for (int j = 0; j < NPAIR; j++)
 #pragma unroll
-    for (int p = 0; p < 3; p++) asm volatile("" : "+v"(bq[j][p]));
+    for (int p = 0; p < 2; p++) asm volatile("" : "+v"(bq[j][p]));
   __syncthreads();
 
   for (int k = k0; k < k1; k++) {
@@ -1726,24 +1816,24 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
     for (int h = 0; h < 2; h++) pw[h] = *reinterpret_cast<const u32x4*>(&ul[buf][((w * 16 + row) * 2 + pa) * 16 + 8 * h]);
     auto pidx = [&](int j) __attribute__((always_inline)) { return (int)((pw[j >> 3][(j >> 1) & 3] >> (16 * (j & 1))) & 0xFFFFu); };
     auto aop = [&](int pos, int p) __attribute__((always_inline)) {
-      return *reinterpret_cast<const bf16x8*>(&xb[pos * EST + 8 * p + lane_off]);
+      return *reinterpret_cast<const halfx8*>(&xb[pos * EST + 8 * p + lane_off]);
     };
     floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    bf16x8 an[3];
+    halfx8 an[2];
     {
       const int p0 = pidx(0);
 #pragma unroll
-      for (int p = 0; p < 3; p++) an[p] = aop(p0, p);
+      for (int p = 0; p < 2; p++) an[p] = aop(p0, p);
     }
 #pragma unroll
     for (int j = 0; j < NPAIR; j++) {
-      bf16x8 ac[3];
+      halfx8 ac[2];
 #pragma unroll
-      for (int p = 0; p < 3; p++) ac[p] = an[p];
+      for (int p = 0; p < 2; p++) ac[p] = an[p];
       if (j + 1 < NPAIR) {
         const int pn = pidx(j + 1);
 #pragma unroll
-        for (int p = 0; p < 3; p++) an[p] = aop(pn, p);
+        for (int p = 0; p < 2; p++) an[p] = aop(pn, p);
       }
       if (j < NPASS && EPP * j < nd1) {
         load_val(j, nd1);  // tile k+1, from the entries read during tile k-1
@@ -1754,19 +1844,16 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
       if (j == NPASS) load_uw(k + 1);
       __builtin_amdgcn_sched_barrier(0);
       if ((wm >> (2 * j)) & 3u) {
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[2], bq[j][0], acc1, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[1], bq[j][0], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[1], bq[j][1], acc1, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[0], bq[j][1], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[0], bq[j][2], acc1, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[0], bq[j][0], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ac[1], bq[j][0], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ac[0], bq[j][0], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ac[0], bq[j][1], acc1, 0, 0, 0);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
     store_lds(buf ^ 1, nd1);  // tile k+1 (slots that tile k does not read; the other table buffer is free since the last barrier)
     {
       const int t0 = m_t[k - k0], nt = m_t[k - k0 + 1] - t0;
-      const floatx4 sum = acc0 + acc1;
+      const floatx4 sum = (acc0 + acc1) * ysc;
       float* yr = Y + ((int64_t)(t0 + 16 * w + 4 * kg) * M_dim + f) * RW + row;
 #pragma unroll
       for (int r = 0; r < 4; r++)
@@ -2036,13 +2123,15 @@ static int fft_children_per_block(size_t per_child, size_t fixed) {
 
 template <typename Real>
 void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0, int m,
-                                 const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X, int vg, int vi) {
+                                 const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X, int vg, int vi,
+                                 float* bound) {
   if (n_slots == 0) return;
+  if (bound && (e0 != 1 || vg != 1)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   const dim3 g((unsigned)((n_slots + 7) / 8 * 8 * e0));  // whole XCD rounds of slots
 #define PVFMM_FFT_UP_M(MM)                                                                                        \
   case MM:                                                                                                        \
     hipLaunchKernelGGL((k_fft_up_m<Real, MM>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, up, K0, e0, lat_inv, \
-                       scal, X, n_slots, vg, vi);                                                                 \
+                       scal, X, n_slots, vg, vi, bound);                                                          \
     HIP_LAUNCH_CHECK();                                                                                           \
     return;
   switch (m) {
@@ -2061,6 +2150,11 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
   hipLaunchKernelGGL(k_fft_up<Real>, dim3(n_slots * e0 * (8 / cb)), dim3(512), lds, s, first_child, up, K0, e0, m, cb, lattice,
                      scal, X, vg, vi);
   HIP_LAUNCH_CHECK();
+  if (bound) {
+    hipLaunchKernelGGL(k_spectrum_bound<Real>, dim3(n_slots), dim3(256), 0, s, n_slots, first_child, up, K0,
+                       6 * (m - 1) * (m - 1) + 2, scal, bound);
+    HIP_LAUNCH_CHECK();
+  }
 }
 
 template <typename Real>
@@ -2080,8 +2174,9 @@ int DeviceKernels<Real>::hadamard_pool_max() {
 template <typename Real>
 void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
                                          const int32_t* tile_u0, const int32_t* usrc, const int32_t* tile_d0,
-                                         const int32_t* dent, const uint16_t* uidx, const uint32_t* gmask, const int32_t* offsets, const Real* Vp, const Real* X, Real* Y, int tiles_per_block,
-                                         const uint16_t* Vb) {
+                                         const int32_t* dent, const uint16_t* uidx, const uint32_t* gmask,
+                                         const int32_t* offsets, const Real* Vp, const Real* X, Real* Y, int tiles_per_block,
+                                         const HadamardParts* hp) {
   if (n_tiles == 0) return;
   if (n_off != 26) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);  // the 26 V1 parent offsets
   constexpr int FB = sizeof(Real) == 4 ? 4 : 2;
@@ -2091,13 +2186,17 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
   if (e0 == 1 && e1 == 1) {
     if constexpr (std::is_same<Real, float>::value) {
       // runs of tiles_per_block (at most 64) tiles per block, one frequency per block
-      if (!Vb || (int64_t)M_dim * 64 >= (1ll << 32)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+      if (!hp || !hp->Vh || !hp->tinv || !hp->bound || !hp->run_scale || (int64_t)M_dim * 64 >= (1ll << 32))
+        throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
       const int TPB = std::min(64, std::max(1, tiles_per_block));
       const int n_chunks = (n_tiles + TPB - 1) / TPB;
       const int64_t nbs = (int64_t)n_chunks * ((M_dim + 15) / 16) * 16;
-      if (nbs >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+      if (nbs >= (1ll << 31) || n_chunks > hp->run_scale_cap) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+      hipLaunchKernelGGL(k_run_scale, dim3((unsigned)n_chunks), dim3(256), 0, s, n_tiles, TPB, tile_u0, usrc, tile_d0, dent,
+                         hp->bound, hp->run_scale);
+      HIP_LAUNCH_CHECK();
       hipLaunchKernelGGL((k_hadamard_split<250, kSplitPool>), dim3((unsigned)nbs), dim3(256), 0, s, M_dim, n_tiles, TPB,
-                         n_chunks, tile_t0, tile_u0, usrc, tile_d0, dent, uidx, gmask, Vb, X, Y);
+                         n_chunks, tile_t0, tile_u0, usrc, tile_d0, dent, uidx, gmask, hp->Vh, hp->tinv, hp->run_scale, X, Y);
     } else {
       const int n_fgt = (M_dim + 1) / 2;
       const int64_t nbt = (int64_t)n_tiles * ((n_fgt + 7) / 8) * 8;

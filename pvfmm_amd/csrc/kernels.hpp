@@ -6,6 +6,15 @@
 
 namespace pvfmm_amd {
 
+// Operands of the FP32 scalar Hadamard (fp16 matrix cores; see hadamard_tiled).
+struct HadamardParts {
+  const uint16_t* Vh = nullptr;  // transfer blocks as two fp16 parts, scaled by tau_f
+  const float* tinv = nullptr;   // 1 / tau_f per frequency (powers of two)
+  const float* bound = nullptr;  // spectrum bound per source slot (fft_up's `bound`)
+  float* run_scale = nullptr;    // scratch: one scale per run of tiles
+  int run_scale_cap = 0;         // its length
+};
+
 // Point-kernel kinds of the device code: 0 laplace potential (1x1), 1 laplace gradient
 // (1x3, r = x_trg - x_src, i.e. minus the gradient of the potential), 2 stokes velocity (3x3).
 template <typename Real>
@@ -61,10 +70,12 @@ struct DeviceKernels {
   // X layout [slot][f][8*e0] complex.  lattice[3p..3p+2] is the m^3 lattice cell of surface
   // point p; lat_inv[(z*m + y)*m + x] is its inverse (-1 inside the cube).
   // vg, vi: X holds vector groups of vg density vectors, [slot][f][vector vg][8*e0 complex];
-  // this call writes vector vi of the group (vg = 1, vi = 0: one vector).
+  // this call writes vector vi of the group (vg = 1, vi = 0: one vector).  bound (e0 = 1,
+  // vg = 1 only; may be null): bound[slot] = |scal[slot]| * sum of |up| over the slot's children,
+  // which no frequency of its spectrum exceeds.
   static void fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0,
                      int m, const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X, int vg = 1,
-                     int vi = 0);
+                     int vi = 0, float* bound = nullptr);
   // Y[slot][f][8*e1] = sum_R V[R][f] X[partner[slot][R]][f] on MFMA; Vp is the transfer block
   // in real form, pre-permuted per lane (layout [R][f][kb][nb][lane 64][4]).  The target slots
   // form tiles of at most 64 consecutive slots [tile_t0[k], tile_t0[k+1])
@@ -74,10 +85,12 @@ struct DeviceKernels {
   // entry) for none, at [k][o][r][g] ([k][n_off][16][4]), or for the FP32 scalar kernel at
   // [k][g][r][o % 2][o / 2] ([k][4][16][2][16]); bit o of gmask[k][g] is set when a
   // target of group g of tile k has a partner at offset o.  tiles_per_block: length of the
-  // run of tiles one block of the FP32 scalar kernel handles.  Vb (FP32 scalar kernels only,
-  // else null): the transfer blocks as three bf16 parts, layout [pair j][part p][f][lane 64][8],
-  // lane l holding part p of B_{2j + l/32}[8 ((l/16) % 2) + i][l % 16], i < 8, for the offsets
-  // in the order of uidx; the FP32 scalar kernels run on the bf16 matrix cores and need it.
+  // run of tiles one block of the FP32 scalar kernel handles.  hp (FP32 scalar kernels only,
+  // else null): hp->Vh the transfer blocks as two fp16 parts of tau_f B, layout [pair j][part
+  // p][f][lane 64][8], lane l holding part p of B_{2j + l/32}[8 ((l/16) % 2) + i][l % 16], i < 8,
+  // for the offsets in the order of uidx; hp->tinv 1 / tau_f; hp->bound the spectrum bounds of
+  // X's slots (fft_up); hp->run_scale room for ceil(n_tiles / tiles_per_block) scales.  The FP32
+  // scalar kernels run on the fp16 matrix cores and need it.
   // They stage into an LDS pool of hadamard_pool_max() entries kept across a block's run of
   // tiles: usrc entries are (source slot << 9 | pool slot), uidx holds pool slots (the pool
   // size for none), and tile k's partners new since tile k-1 are dent[tile_d0[k] ..
@@ -88,7 +101,8 @@ struct DeviceKernels {
   static void hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, int e0, int e1, int n_off, const int32_t* tile_t0,
                              const int32_t* tile_u0, const int32_t* usrc, const int32_t* tile_d0, const int32_t* dent,
                              const uint16_t* uidx, const uint32_t* gmask, const int32_t* offsets,
-                             const Real* Vp, const Real* X, Real* Y, int tiles_per_block = 16, const uint16_t* Vb = nullptr);
+                             const Real* Vp, const Real* X, Real* Y, int tiles_per_block = 16,
+                             const HadamardParts* hp = nullptr);
   // inverse DFT, pick the check-surface points, dn[child] += scal * value
   static void ifft_down(hipStream_t s, int n_slots, const int32_t* first_child, const Real* Y, Real* dn,
                         int K1, int e1, int m, const int32_t* lattice, const int32_t* lat_inv, int ns,
@@ -96,7 +110,9 @@ struct DeviceKernels {
   // The FP32 scalar-kernel Hadamard for a group of 16 density vectors: the MFMA rows are the
   // 16 vectors of one target, X and Y in the vector-group layout of fft_up / ifft_down with
   // vg = 16, partner[t * 26 + o] the source slot of target slot t at offset v1_offsets[o] (or
-  // -1), Vb as for hadamard_tiled.  targets_per_wave: targets one wave computes in turn.
+  // -1), Vb the transfer blocks as three bf16 parts ([pair j][part p][f][lane 64][8], the lane
+  // layout of hadamard_tiled's fp16 parts, unscaled).  targets_per_wave: targets one wave
+  // computes in turn.
   static void hadamard_vec16(hipStream_t s, int n_trg, int M_dim, const int32_t* partner, const uint16_t* Vb,
                              const Real* X, Real* Y, int targets_per_wave = 16);
 };
