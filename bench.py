@@ -6,12 +6,12 @@ points on a strongly adaptive octree (50% on the sphere |x-0.5|=0.4, 30% in a Ga
 sigma 0.02 at (0.3,0.3,0.7), 20% uniform), densities N(0,1), m = 10, max_pts = 256.
 
 The reported value is the mixed-precision evaluator's Mtargets/s (FP32 P2P, DFTs and up/down
-GEMMs, the M2L Hadamard as exact three-part bf16 splits on the matrix cores; FP64 operator
+GEMMs, the M2L Hadamard as scaled two-part fp16 splits on the matrix cores; FP64 operator
 precompute and per-target merge).  The FP64 GPU evaluator is timed the same way and the ratio
 reported; both errors are relative L2 against the FP64 all-pairs direct sum on the GPU (which
 the tests check against the CPU direct sum) on 4096 sampled targets.  The CPU baseline is the
-FP64 oracle (a restatement of the reference algorithm) on a 2^18-point sample of the same
-distribution, on the host cores; it is the only use of the oracle here.
+FP64 oracle (a restatement of the reference algorithm) on the same workload (--cpu-sample
+points, by default all of them), on the host cores; it is the only use of the oracle here.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--n N] [--workload NAME]
 """
@@ -29,8 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MI355X_FP32_MFMA_TFLOPS = 157.3
-MI355X_BF16_MFMA_TFLOPS = 2500.0  # dense
-BF16_PRODUCTS_PER_FP32 = 6  # three-way bf16 split of both operands, six partial products kept
+MI355X_F16_MFMA_TFLOPS = 2500.0  # dense fp16 (and bf16)
+F16_PRODUCTS_PER_FP32 = 3  # two-way fp16 split of both operands, three partial products kept
 MI355X_FP32_VALU_TFLOPS = 78.6  # non-packed FP32 FMA (packed FP32 is not used)
 MI355X_FP64_TFLOPS = 78.6
 MI355X_HBM_GBS = 8000.0
@@ -117,36 +117,44 @@ def roofline(ph: dict, st: dict, m: int, real_bytes: int):
     """Roofline of the dominant kernel of the mixed evaluator from algorithmic counts.
 
     V-list Hadamard (scalar kernels): 512*k0*k1 flop per (V1 pair, frequency) (the reference's
-    own count, include/fmm_pts.txx:3336) in FP32, which the kernel computes on the bf16 matrix
-    cores as six bf16 products per FP32 product (each operand split exactly into three bf16
-    parts); its roofline is 6x that count against the dense bf16 MFMA peak, and the FP32
-    figure is reported beside it.  The kernel stages each tile's partner spectra in LDS once,
-    so it moves far fewer bytes than the target-stationary model (per pair and frequency
-    16*k0 reals read, per target parent 16*k1 written, the 26 V1 matrices read once), which it
-    would otherwise be bound by; that model's GB/s is reported beside it, and `traffic` is the
-    measured HBM bytes per launch.  U-list P2P: 16 flop per pair for laplace_grad, VALU-bound."""
+    own count, include/fmm_pts.txx:3336) in FP32, which the kernel computes on the fp16 matrix
+    cores as three fp16 products per FP32 product (both operands scaled and split exactly into
+    two fp16 parts).  At the dense fp16 peak those products take less time than the kernel's
+    compulsory HBM traffic, every source spectrum read once and every target spectrum written
+    once (16*M_dim*k0 reals per source parent, 16*M_dim*k1 per target parent, the 26 V1
+    matrices once), so with t_bound = max(bytes / HBM peak, flops / MFMA peak) the bound is
+    HBM: `achieved` is those bytes over the kernel's time.  Beside it: the FP32-equivalent rate
+    against the FP32 matrix peak, the fp16 product rate against the fp16 peak, and the
+    target-stationary model's bytes (each pair re-reading its partner's spectrum), which the
+    kernel avoids by staging each tile's partners in LDS once; `traffic` is the measured HBM
+    bytes per launch.  U-list P2P: 16 flop per pair for laplace_grad, VALU-bound."""
     n1 = 2 * m
     M_dim = n1 * n1 * (n1 // 2 + 1)
     dom = max(ph, key=ph.get)
     t = ph[dom] * 1e-3
     if dom == "V-List:Hadamard":
-        name = "k_hadamard_split<250>"
+        name = "k_hadamard_split<250, 496>"
         bytes_model = (16.0 * M_dim * real_bytes * st["v_pairs"] + 16.0 * M_dim * real_bytes * st["v_trg_slots"]
                        + 26.0 * M_dim * 128 * real_bytes)
+        bytes_alg = (16.0 * M_dim * real_bytes * (st["v_src_slots"] + st["v_trg_slots"]) + 26.0 * M_dim * 128 * real_bytes)
         flops = 512.0 * st["v_pairs"] * M_dim
-        achieved = BF16_PRODUCTS_PER_FP32 * flops / t / 1e12
-        return {"kernel": name + " (V-list frequency-domain M2L, FP32 via split bf16 MFMA)", "bound": "mfma",
-                "achieved": achieved, "peak": MI355X_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / MI355X_BF16_MFMA_TFLOPS,
-                "traffic": read_traffic(name), "ms_per_launch": ph[dom], "algorithmic_flops": flops,
-                "bf16_flops": BF16_PRODUCTS_PER_FP32 * flops,
+        achieved = bytes_alg / t / 1e9
+        return {"kernel": name + " (V-list frequency-domain M2L, FP32 via split fp16 MFMA)", "bound": "hbm",
+                "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS,
+                "traffic": read_traffic(name), "ms_per_launch": ph[dom], "algorithmic_bytes": bytes_alg,
+                "algorithmic_flops": flops,
+                "t_bound_ms": {"hbm": 1e3 * bytes_alg / (MI355X_HBM_GBS * 1e9),
+                               "fp16_mfma": 1e3 * F16_PRODUCTS_PER_FP32 * flops / (MI355X_F16_MFMA_TFLOPS * 1e12)},
                 "fp32_equivalent_tflops": flops / t / 1e12,
                 "fp32_equivalent_frac_of_fp32_mfma_peak": flops / t / 1e12 / MI355X_FP32_MFMA_TFLOPS,
+                "fp16_tflops": F16_PRODUCTS_PER_FP32 * flops / t / 1e12,
+                "fp16_frac_of_fp16_mfma_peak": F16_PRODUCTS_PER_FP32 * flops / t / 1e12 / MI355X_F16_MFMA_TFLOPS,
                 "target_stationary_model_bytes": bytes_model,
                 "target_stationary_model_gbs": bytes_model / t / 1e9,
-                "work": f"512 flop x {st['v_pairs']} V1 pairs x {M_dim} frequencies in FP32, each FP32 product "
-                        f"as {BF16_PRODUCTS_PER_FP32} bf16 products; the kernel runs dense 16-target MFMA groups "
-                        f"(offset pairs without partners skipped) from LDS-staged spectra"}
+                "work": f"{st['v_src_slots']} source and {st['v_trg_slots']} target parents x {M_dim} frequencies x "
+                        f"16 reals, read and written once; 512 flop x {st['v_pairs']} V1 pairs x {M_dim} frequencies "
+                        f"in FP32, each FP32 product as {F16_PRODUCTS_PER_FP32} fp16 products on dense 16-target MFMA "
+                        f"groups (offset pairs without partners skipped) from LDS-staged spectra"}
     if dom == "U-List":
         name = "k_p2p_leaf<float, 1, 1>"
         flops = 16.0 * st["u_pairs"]
@@ -472,7 +480,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"laplace_grad FMM evaluation, N={n} adaptive (50% sphere r=0.4, 30% gaussian "
                                f"sigma=0.02, 20% uniform), m={m}, max_pts={c['max_pts']}, mixed FP32 bulk (V-list "
-                               f"Hadamard products as exact 3-part bf16 splits on the matrix cores) / FP64 "
+                               f"Hadamard products as scaled 2-part fp16 splits on the matrix cores) / FP64 "
                                f"precompute and merge", "n_targets": n, "m": m, "max_pts": c["max_pts"]},
         "fp64_mtargets_s": n / fp64["step"] / 1e6,
         "fp64_ms_per_step": 1e3 * fp64["step"],

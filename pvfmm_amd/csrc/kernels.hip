@@ -1153,7 +1153,11 @@ struct DftShape {
 // per (c, kx, ky < m) runs the y stage for ky and ky+m (w^{-(ky+m)y} = (-1)^y w^{-ky y}) and the
 // z stage of both (X[kz] = E + O, X[kz+m] = E - O over even/odd z), writing the spectra as
 // X[slot][f][c*e0+i] complex, f = kx + (m+1)(ky + 2m kz).
-template <typename Real, int M>
+// NTS: non-temporal stores of X, which is read back only by the Hadamard, long after its lines
+// would have left the caches (1.20 -> 1.04 ms for the 2M adaptive workload).  Only where each
+// block writes whole lines: with several components (or vectors in one group) the blocks of one
+// parent interleave their stores within a line, which must merge in L2 first.
+template <typename Real, int M, bool NTS>
 __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const int32_t* __restrict__ first_child,
                                                                          const Real* __restrict__ up, int K0, int e0,
                                                                          const int32_t* __restrict__ lat_inv,
@@ -1246,12 +1250,15 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
           asm volatile("" : "+s"(xb));
           Real* d0 = xb + (f0 * vg * 8 * e0 + c * e0) * 2;
           Real* d1 = xb + (f1 * vg * 8 * e0 + c * e0) * 2;
-          // X is read back only by the Hadamard, long after these lines would have left the
-          // caches: non-temporal stores (1.20 -> 1.04 ms for the 2M adaptive workload)
-          __builtin_nontemporal_store(er + orr, &d0[0]);
-          __builtin_nontemporal_store(ei + oi, &d0[1]);
-          __builtin_nontemporal_store(er - orr, &d1[0]);
-          __builtin_nontemporal_store(ei - oi, &d1[1]);
+          if constexpr (NTS) {
+            __builtin_nontemporal_store(er + orr, &d0[0]);
+            __builtin_nontemporal_store(ei + oi, &d0[1]);
+            __builtin_nontemporal_store(er - orr, &d1[0]);
+            __builtin_nontemporal_store(ei - oi, &d1[1]);
+          } else {
+            d0[0] = er + orr; d0[1] = ei + oi;
+            d1[0] = er - orr; d1[1] = ei - oi;
+          }
         });
       };
       // y stage: column zz of ky (E + O) and of ky + m (E - O)
@@ -2168,8 +2175,12 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
   const dim3 g((unsigned)((n_slots + 7) / 8 * 8 * e0));  // whole XCD rounds of slots
 #define PVFMM_FFT_UP_M(MM)                                                                                        \
   case MM:                                                                                                        \
-    hipLaunchKernelGGL((k_fft_up_m<Real, MM>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, up, K0, e0, lat_inv, \
-                       scal, X, n_slots, vg, vi, bound);                                                          \
+    if (e0 == 1 && vg == 1)                                                                                       \
+      hipLaunchKernelGGL((k_fft_up_m<Real, MM, true>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, up, K0, e0, \
+                         lat_inv, scal, X, n_slots, vg, vi, bound);                                               \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_fft_up_m<Real, MM, false>), g, dim3(DftShape<Real, MM>::NT), 0, s, first_child, up, K0, e0, \
+                         lat_inv, scal, X, n_slots, vg, vi, bound);                                               \
     HIP_LAUNCH_CHECK();                                                                                           \
     return;
   switch (m) {
