@@ -55,7 +55,8 @@ enum PVFMMError {
 /* ---------------------------------------------------------------- reference particle C ABI */
 
 /* include/pvfmm.h:254 -- double-precision context; box_size <= 0 selects free space with an
- * automatic bounding box (src/pvfmm-wrapper.cpp:457-514).  Periodic boxes are unsupported. */
+ * automatic bounding box (src/pvfmm-wrapper.cpp:457-514), box_size > 0 the periodic box
+ * [0, box_size)^3 (BoundaryType Periodic, src/pvfmm-wrapper.cpp:397-403,514-519). */
 void* PVFMMCreateContextD(double box_size, int n, int m, enum PVFMMKernel kernel, pvfmm_comm_t comm);
 /* include/pvfmm.h:255 -- single-precision context (FP32 inputs/outputs, FP32 evaluation). */
 void* PVFMMCreateContextF(float box_size, int n, int m, enum PVFMMKernel kernel, pvfmm_comm_t comm);
@@ -141,6 +142,12 @@ void pvfmm_amd_fmm_destroy(void* fmm);
 /* PtFMM_CreateTree(src_coord, src_value, trg_coord, comm, max_pts, FreeSpace)
  * (include/pvfmm.txx:215-245): coordinates in [0,1)^3, AoS. */
 void* pvfmm_amd_tree_create(const double* src, int64_t n_src, const double* trg, int64_t n_trg, int max_pts);
+/* PtFMM_CreateTree(..., max_pts, bndry) with the reference's boundary types (BoundaryType,
+ * include/mpi_tree.hpp:22-29; include/pvfmm.txx:215-245): 0 FreeSpace, 1 PX, 2 PXY, 3 PXYZ
+ * (= Periodic, unit period along the periodic axes).  pt: tree-defining points, or NULL for
+ * the targets (n_pt is then ignored). */
+void* pvfmm_amd_tree_create_bndry(const double* src, int64_t n_src, const double* trg, int64_t n_trg, const double* pt,
+                                  int64_t n_pt, int max_pts, int bndry);
 /* Variant with an explicit tree-defining point set (MPI_Node::pt_coord). */
 void* pvfmm_amd_tree_create_pts(const double* src, int64_t n_src, const double* trg, int64_t n_trg, const double* pt,
                                 int64_t n_pt, int max_pts);
@@ -223,7 +230,8 @@ int pvfmm_amd_tree_stats(void* tree, int64_t* out, int cap);
 
 /* Operator matrices of an fmm handle (FP64), for parity checks against the oracle:
  * which 0 uc2ue0 (V S^+), 1 uc2ue1 (U^T), 2 dc2de0, 3 dc2de1, 4 u2u (8 stacked),
- * 5 d2d (8 side by side), 6 v1 (complex interleaved).
+ * 5 d2d (8 side by side), 6 v1 (complex interleaved), 7/8/9 the periodic boundary operator
+ * (BC_Type, include/fmm_pts.txx:790-1163) of PX / PXY / PXYZ, computed on first request.
  * Returns the element count; copies when out != NULL. */
 int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out);
 

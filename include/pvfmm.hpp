@@ -14,9 +14,10 @@
 // Header-only over the C ABI of include/pvfmm.h (link with -lpvfmm).  The work runs on one
 // GPU as the communicator's single rank, so `comm` may be any value (MPI_COMM_WORLD,
 // MPI_COMM_SELF or 0) and is not used.  Differences from the reference, each reported by
-// throwing pvfmm::Error rather than aborting: Periodic boundaries, double-layer sources,
-// init_depth > 0 and kernels other than laplace potential / gradient and stokes velocity are
-// rejected.  Real = float evaluates in the mixed FP32/FP64 mode of the library.
+// throwing pvfmm::Error rather than aborting: double-layer sources, init_depth > 0 and kernels
+// other than laplace potential / gradient and stokes velocity are rejected.  Periodic trees
+// (PX, PXY, PXYZ = Periodic, unit period) are supported.  Real = float evaluates in the mixed
+// FP32/FP64 mode of the library.
 #ifndef PVFMM_AMD_PVFMM_HPP
 #define PVFMM_AMD_PVFMM_HPP
 
@@ -30,8 +31,8 @@
 
 namespace pvfmm {
 
-// include/pvfmm_common.hpp: boundary condition of the tree
-enum BoundaryType { FreeSpace, Periodic };
+// include/mpi_tree.hpp:22-29: boundary condition of the tree
+enum BoundaryType { FreeSpace = 0, PX, PXY, PXYZ, Periodic = PXYZ };
 
 // Failure of a library call: code is a PVFMMError value.
 class Error : public std::runtime_error {
@@ -148,11 +149,11 @@ template <class Real>
 class PtFMM_Tree {
  public:
   PtFMM_Tree(const std::vector<Real>& sl_coord, const std::vector<Real>& sl_density, const std::vector<Real>& trg_coord,
-             int max_pts)
+             int max_pts, BoundaryType bndry = FreeSpace)
       : n_src_((int64_t)(sl_coord.size() / 3)), n_trg_((int64_t)(trg_coord.size() / 3)), sl_density_(sl_density) {
     if (sl_coord.size() % 3 || trg_coord.size() % 3) throw Error(PVFMM_ERR_INVALID_ARGUMENT, "coordinates are not [x y z]*n");
     const std::vector<double> s = detail::to_double(sl_coord), t = detail::to_double(trg_coord);
-    h_ = pvfmm_amd_tree_create(s.data(), n_src_, t.data(), n_trg_, max_pts);
+    h_ = pvfmm_amd_tree_create_bndry(s.data(), n_src_, t.data(), n_trg_, nullptr, 0, max_pts, (int)bndry);
     if (!h_) detail::check_last();
   }
   PtFMM_Tree(const PtFMM_Tree&) = delete;
@@ -183,14 +184,13 @@ class PtFMM_Tree {
 };
 
 // PtFMM_CreateTree (include/pvfmm.hpp:262-265, include/pvfmm.txx:215-245).  Coordinates in
-// [0,1)^3; the caller deletes the tree.
+// [0,1)^3; the caller deletes the tree.  bndry: FreeSpace or a periodic type (PX, PXY, PXYZ).
 template <class Real, class Comm>
 PtFMM_Tree<Real>* PtFMM_CreateTree(const std::vector<Real>& sl_coord, const std::vector<Real>& sl_density,
                                    const std::vector<Real>& trg_coord, Comm, int max_pts = 100, BoundaryType bndry = FreeSpace,
                                    int init_depth = 0) {
-  if (bndry != FreeSpace) throw Error(PVFMM_ERR_UNSUPPORTED, "periodic boundaries are not supported");
   if (init_depth != 0) throw Error(PVFMM_ERR_UNSUPPORTED, "init_depth > 0 is not supported");
-  return new PtFMM_Tree<Real>(sl_coord, sl_density, trg_coord, max_pts);
+  return new PtFMM_Tree<Real>(sl_coord, sl_density, trg_coord, max_pts, bndry);
 }
 
 // Variant with double-layer sources (include/pvfmm.hpp:262-265); they must be empty here.

@@ -18,7 +18,11 @@ What pins each part (tests/test_oracle.py unless noted):
   * pseudo-inverses: the C2E*E2C projector property;
   * tree and lists: brute-force geometric list definitions, 2:1 balance and tiling
     invariants, reference table counts (include/interac_list.txx:39-56);
-  * whole FMM: relative L2 against an FP64 direct sum, convergence in m.
+  * whole FMM: relative L2 against an FP64 direct sum, convergence in m;
+  * periodic boundary (bc_matrix, cube_integral, the periodic tree): Ewald sums for Laplace
+    (tests/ewald.py; potential, field, net charge), geometric convergence in m, translation
+    invariance for Stokes, brute-force quadrature and identities for the closed-form cube
+    integrals, a brute-force 2:1 check across the wrap (tests/test_periodic.py).
 Bitwise agreement with the reference's floating point (sctl::approx_rsqrt, LAPACK SVD signs,
 OpenMP summation order) is not pinned: SCTL is not vendored in the reference checkout, so no
 reference build exists to compare with.
@@ -64,7 +68,8 @@ def lib():
         vp, i64, i32, dp = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
         P = ctypes.POINTER
         L.oracle_tree_create.restype = vp
-        L.oracle_tree_create.argtypes = [P(dp), i64, P(dp), i64, P(dp), i64, i32, i32]
+        L.oracle_tree_create.argtypes = [P(dp), i64, P(dp), i64, P(dp), i64, i32, i32, i32]
+        L.oracle_set_bc.argtypes = [vp, P(dp)]
         L.oracle_tree_destroy.argtypes = [vp]
         L.oracle_tree_num_nodes.restype = i64
         L.oracle_tree_num_nodes.argtypes = [vp]
@@ -222,17 +227,237 @@ def precompute(kernel: str | int, m: int, real: str = "fp64") -> dict:
                 u2u=np.stack(u2u), d2d=np.stack(d2d), v1=v1, M_dim=M_dim)
 
 
+# ----------------------------------------------------------------------------- periodic boundary
+BC_LEVELS = 45  # PVFMM_BC_LEVELS, include/pvfmm_common.hpp:31
+BNDRY = {"free": 0, "px": 1, "pxy": 2, "pxyz": 3}  # BoundaryType, include/mpi_tree.hpp:22-29
+
+
+def u_equiv_surf(m, c, depth):
+    return surf_at(m, c, depth, RAD0)
+
+
+def u_check_surf(m, c, depth):
+    return surf_at(m, c, depth, RAD1)
+
+
+def d_check_surf(m, c, depth):
+    return surf_at(m, c, depth, RAD0)
+
+
+def d_equiv_surf(m, c, depth):
+    return surf_at(m, c, depth, RAD1)
+
+
+def _prism_inv_r(rel_lo, rel_hi):
+    """Closed form of the integral of 1/|y - x| over a box, corners relative to x: the
+    Newtonian potential of a rectangular prism (sum over the 8 corners with signs of
+    X Y asinh(Z/rho_xy) + ... - X^2/2 atan(Y Z / (X R)) - ...).  rel_lo, rel_hi: (n, 3)."""
+    out = np.zeros(len(rel_lo))
+    for sx in (0, 1):
+        for sy in (0, 1):
+            for sz in (0, 1):
+                X = np.where(sx, rel_hi[:, 0], rel_lo[:, 0])
+                Y = np.where(sy, rel_hi[:, 1], rel_lo[:, 1])
+                Z = np.where(sz, rel_hi[:, 2], rel_lo[:, 2])
+                sign = (1 if sx else -1) * (1 if sy else -1) * (1 if sz else -1)
+                R = np.sqrt(X * X + Y * Y + Z * Z)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    def ash(a, b, c):  # asinh(a / hypot(b, c)), 0 when b = c = 0
+                        h = np.hypot(b, c)
+                        return np.where(h > 0, np.arcsinh(a / np.where(h > 0, h, 1.0)), 0.0)
+
+                    def at(a, b, c):  # a^2 / 2 * atan(b c / (a R)), 0 when a = 0
+                        return np.where(a != 0, 0.5 * a * a * np.arctan(b * c / np.where(a != 0, a * R, 1.0)), 0.0)
+                    F = X * Y * ash(Z, X, Y) + Y * Z * ash(X, Y, Z) + Z * X * ash(Y, Z, X) - at(X, Y, Z) - at(Y, Z, X) - at(Z, X, Y)
+                out += sign * F
+    return out
+
+
+def _face_r_over_r(i, j, rel_lo, rel_hi):
+    """sum over the two faces normal to axis j of n_j * integral over the face of
+    (y_i - x_i) / |y - x| dA, corners relative to x.  Closed forms: in-plane component
+    int int w / R dw dz = sum +- (z R + a^2 asinh(z / a)) / 2 with a^2 = w^2 + d^2; normal component
+    d * int int 1 / R = d * sum +- (u asinh(v / hypot(u, d)) + v asinh(u / hypot(v, d)) - d atan(u v / (d R)))."""
+    p, q = [a for a in range(3) if a != j]
+    out = np.zeros(len(rel_lo))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for sj in (0, 1):
+            d = rel_hi[:, j] if sj else rel_lo[:, j]
+            nrm = 1.0 if sj else -1.0
+            acc = np.zeros(len(rel_lo))
+            for sp in (0, 1):
+                for sq in (0, 1):
+                    u = rel_hi[:, p] if sp else rel_lo[:, p]
+                    v = rel_hi[:, q] if sq else rel_lo[:, q]
+                    sign = (1 if sp else -1) * (1 if sq else -1)
+                    R = np.sqrt(u * u + v * v + d * d)
+                    if i == j:
+                        hu, hv = np.hypot(u, d), np.hypot(v, d)
+                        t1 = np.where(hu > 0, u * np.arcsinh(v / np.where(hu > 0, hu, 1.0)), 0.0)
+                        t2 = np.where(hv > 0, v * np.arcsinh(u / np.where(hv > 0, hv, 1.0)), 0.0)
+                        t3 = np.where(d != 0, d * np.arctan(u * v / np.where(d != 0, d * R, 1.0)), 0.0)
+                        acc += sign * d * (t1 + t2 - t3)
+                    else:
+                        w, z = (u, v) if i == p else (v, u)  # w: the in-plane axis i, z: the other
+                        a = np.hypot(w, d)  # int dw w / R = R, then int dz R with a^2 = w^2 + d^2
+                        h = 0.5 * (z * R + np.where(a > 0, a * a * np.arcsinh(z / np.where(a > 0, a, 1.0)), 0.0))
+                        acc += sign * h
+            out += nrm * acc
+    return out
+
+
+def cube_integral(kernel, pts, lo, size) -> np.ndarray:
+    """(n, k0, k1): integral over the cube [lo, lo + size]^3 of the expansion kernel from a
+    uniform unit density (component a) to each point (component b), in closed form (the
+    reference's cheb_integ of Chebyshev degree 0, include/cheb_utils.txx:1037-1081, as used by
+    include/fmm_pts.txx:865-880,1009-1020).  Stokes: int S_ab = (2 delta_ab P - faces_b(a)) / (8 pi)
+    from r_a r_b / r^3 = delta_ab / r - d_b((y_a - x_a) / r) and the divergence theorem."""
+    kid = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    pts = np.asarray(pts, dtype=np.float64).reshape(-1, 3)
+    rel_lo = lo - pts
+    rel_hi = lo + size - pts
+    P = _prism_inv_r(rel_lo, rel_hi)
+    if kid in (0, 1):
+        return (P / (4 * np.pi)).reshape(-1, 1, 1)
+    G = np.zeros((len(pts), 3, 3))
+    for a in range(3):
+        for b in range(3):
+            G[:, a, b] = ((2.0 * P if a == b else 0.0) - _face_r_over_r(a, b, rel_lo, rel_hi)) / (8 * np.pi)
+    return G
+
+
+def vol_poten(kid, pts) -> np.ndarray:
+    """(k0, n*k1): the kernels' analytic potential of a uniform unit density
+    (laplace_vol_poten / stokes_vol_poten, include/kernel.txx:1289-1295,1456-1467)."""
+    pts = np.asarray(pts, dtype=np.float64).reshape(-1, 3)
+    n = len(pts)
+    r2 = (pts * pts).sum(1)
+    if kid == 0:
+        return (-r2 / 6).reshape(1, n)
+    out = np.zeros((3, n, 3))
+    for k in range(3):
+        out[k, :, k] = -(r2 - pts[:, k] ** 2) / 4
+    return out.reshape(3, n * 3)
+
+
+def bc_matrix(kernel, m: int, bndry="pxyz", real: str = "fp64", upsample: int = 2) -> np.ndarray:
+    """Periodic boundary operator of FMM_Pts::Precomp(BC_Type) (include/fmm_pts.txx:790-1163,
+    the default build: no PVFMM_EXTENDED_BC): root upward equivalent density -> root downward
+    check potential of every periodic image beyond the root's colleagues, (n_surf*k0, n_surf*k1)
+    in the expansion kernel at order m.  Computed at order m + upsample and downsampled."""
+    kid = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    bt = BNDRY[bndry] if isinstance(bndry, str) else int(bndry)
+    ke = expansion_kernel(kid)
+    k0, k1 = KDIM[ke]
+    eps = float(np.finfo({"fp64": np.float64, "fp32": np.float32}[real]).eps)
+    mo = m + upsample
+    n = 6 * (mo - 1) ** 2 + 2
+    K = lambda a, b: kernel_matrix(ke, a, b)  # noqa: E731
+
+    def c2e(level, up):
+        c = np.full(3, -(2.0 ** level) / 2)
+        if up:
+            return _c2e(ke, u_equiv_surf(mo, c, level), u_check_surf(mo, c, level), eps)
+        return _c2e(ke, d_equiv_surf(mo, c, level), d_check_surf(mo, c, level), eps)
+
+    uc2ue0, uc2ue1 = c2e(0, True)
+    uc2ue0_, uc2ue1_ = c2e(-1, True)
+    dc2de0, dc2de1 = c2e(0, False)
+    dc2de0_, dc2de1_ = c2e(-1, False)
+    has_vol = bt == 3 and kid != 1  # laplace_grad's M2L kernel carries no volume potential
+    # zero-average projections (include/fmm_pts.txx:862-913)
+    G = cube_integral(ke, u_check_surf(mo, np.zeros(3), 0), 0.0, 1.0)  # (n, a, b)
+    M_s2c = np.transpose(G, (2, 0, 1)).reshape(k0, n * k1)  # [j][i*k1 + k] = int K_{k,j}(uc_i)
+    M_s2e = (M_s2c @ uc2ue0) @ uc2ue1
+    M_s2e = M_s2e / M_s2e.sum(1, keepdims=True)
+    Z = np.eye(n * k0) - np.tile(M_s2e, (n, 1))
+    C = np.eye(n * k1)
+    for k in range(k1):
+        C[k::k1, k::k1] -= 1.0 / n
+    keep = lambda a, b, c: not ((bt == 1 and (b != 0 or c != 0)) or (bt == 2 and c != 0))  # noqa: E731
+    # M2M: 8 unit boxes -> their parent of size 2 (include/fmm_pts.txx:917-941)
+    tb = np.full(3, -1.0)
+    chk = u_check_surf(mo, tb, -1)
+    M2M = np.zeros((n * k0, n * k1))
+    for a in (0, 1):
+        for b in (0, 1):
+            for c in (0, 1):
+                if keep(a, b, c):
+                    M2M += K(u_equiv_surf(mo, tb + np.array([a, b, c]), 0), chk)
+    M2M = (M2M @ uc2ue0_) @ uc2ue1_
+    # L2L: parent of size 2 -> its lower-corner child (include/fmm_pts.txx:942-951)
+    L2L = dc2de0_ @ (dc2de1_ @ K(d_equiv_surf(mo, tb, -1), d_check_surf(mo, tb, 0)))
+    # M2L: unit boxes at offsets in [-2, 3]^3 not adjacent to the root (include/fmm_pts.txx:952-975)
+    dnc = d_check_surf(mo, np.zeros(3), 0)
+    M2L = np.zeros((n * k0, n * k1))
+    for a in range(-2, 4):
+        for b in range(-2, 4):
+            for c in range(-2, 4):
+                if not keep(a, b, c) or max(abs(a), abs(b), abs(c)) <= 1:
+                    continue
+                M2L += K(u_equiv_surf(mo, np.array([a, b, c], dtype=np.float64), 0), dnc)
+    M2M = Z @ M2M @ Z
+    L2L = C @ L2L @ C
+    M2L = Z @ M2L @ C
+    # one level up: the same map for a box of size 2 is 2^-(src_scal + trg_scal) = 1/2 of it
+    # (Pr, Pc of include/fmm_pts.txx:980-993 for the potential-type expansion kernels)
+    M = M2L.copy()
+    for _ in range(1, BC_LEVELS):
+        M = M2L + M2M @ (0.5 * M) @ L2L
+    if has_vol:  # far field of the analytic volume potential (include/fmm_pts.txx:1001-1033)
+        dc = d_check_surf(mo, np.ones(3), 0)
+        G3 = cube_integral(ke, dc, 0.0, 3.0)
+        M_near = np.transpose(G3, (2, 0, 1)).reshape(k0, n * k1)
+        M_far = vol_poten(ke, dc) - M_near
+        M += np.tile(M_far, (n, 1))
+    # corner correction (include/fmm_pts.txx:1036-1138)
+    enable = {3: [1, 1, 1, 1, 0, 0, 0, 0], 2: [1, 1, 1, 0, 0, 0, 0, 0], 1: [1, 1, 0, 0, 0, 0, 0, 0]}[bt]
+    corners = np.array([[i % 2, (i // 2) % 2, (i // 4) % 2] for i in range(8)], dtype=np.float64)
+    z3 = np.zeros(3)
+    ue, de = u_equiv_surf(mo, z3, 0), d_equiv_surf(mo, z3, 0)
+    cv = (M @ dc2de0) @ (dc2de1 @ K(de, corners))  # (n*k0, 8*k1)
+    for kk in range(8):
+        for j0 in (-1, 0, 1):
+            for j1 in (-1, 0, 1):
+                for j2 in (-1, 0, 1):
+                    if (bt == 2 and j2 != 0) or (bt == 1 and (j1 != 0 or j2 != 0)):
+                        continue
+                    pt = corners[kk] - np.array([j0, j1, j2])
+                    if np.all(np.abs(pt - 0.5) < 1):
+                        continue
+                    cv[:, kk * k1:(kk + 1) * k1] += K(ue, pt.reshape(1, 3))
+    if has_vol:
+        cv -= np.tile(vol_poten(ke, corners), (n, 1))
+    basis = lambda p: np.array([np.ones(len(p)), p[:, 0], p[:, 1], p[:, 2], p[:, 0] * p[:, 1],  # noqa: E731
+                                p[:, 0] * p[:, 2], p[:, 1] * p[:, 2], p[:, 0] * p[:, 1] * p[:, 2]])
+    V = basis(corners)  # (coeff, corner)
+    Vp = np.linalg.pinv(V)  # (corner, coeff)
+    cvec = basis(dnc) * np.array(enable, dtype=np.float64)[:, None]  # (coeff, n)
+    coeff = np.einsum("ikj,kc->icj", cv.reshape(n * k0, 8, k1), Vp)
+    M = M - np.einsum("icj,cs->isj", coeff, cvec).reshape(n * k0, n * k1)
+    if upsample:  # back to order m (include/fmm_pts.txx:1141-1160)
+        rc = np.full(3, -0.5)
+        M2Md = (K(u_equiv_surf(m, rc, 0), u_check_surf(mo, rc, 0)) @ uc2ue0) @ uc2ue1
+        L2Ld = dc2de0 @ (dc2de1 @ K(d_equiv_surf(mo, rc, 0), d_check_surf(m, rc, 0)))
+        M = M2Md @ M @ L2Ld
+    return M
+
+
 # ----------------------------------------------------------------------------- tree + FMM
 class OracleTree:
-    """PtFMM_CreateTree + FMM_Tree::SetupFMM restated (include/pvfmm.txx:215-245)."""
+    """PtFMM_CreateTree + FMM_Tree::SetupFMM restated (include/pvfmm.txx:215-245).
 
-    def __init__(self, src, trg, max_pts, pt=None, refine=True):
+    bndry: "free" or a periodic BoundaryType ("px", "pxy", "pxyz"); periodic trees need the
+    operators' "bc" entry (bc_matrix) for the lattice beyond the root's colleagues."""
+
+    def __init__(self, src, trg, max_pts, pt=None, refine=True, bndry="free"):
         L = lib()
         self.src = _c(src).reshape(-1, 3)
         self.trg = _c(trg).reshape(-1, 3)
         self.pt = self.trg if pt is None else _c(pt).reshape(-1, 3)
+        self.bndry = BNDRY[bndry] if isinstance(bndry, str) else int(bndry)
         self.h = L.oracle_tree_create(_dp(self.src), len(self.src), _dp(self.trg), len(self.trg),
-                                      _dp(self.pt), len(self.pt), int(max_pts), int(refine))
+                                      _dp(self.pt), len(self.pt), int(max_pts), int(refine), self.bndry)
         self.n_nodes = L.oracle_tree_num_nodes(self.h)
         self.ops = None
 
@@ -275,6 +500,8 @@ class OracleTree:
         lib().oracle_set_operators(self.h, o["kernel"], o["m"], _dp(o["uc2ue0"]), _dp(o["uc2ue1"]),
                                    _dp(o["dc2de0"]), _dp(o["dc2de1"]), _dp(o["u2u"]), _dp(o["d2d"]),
                                    _dp(o["v1"]))
+        if "bc" in o:
+            lib().oracle_set_bc(self.h, _dp(o["bc"]))
 
     def evaluate(self, src_val, threads=None, return_phases=False):
         kid = self.ops["kernel"]

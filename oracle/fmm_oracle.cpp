@@ -19,6 +19,11 @@
 //   * EvalListPts semantics .......... include/fmm_pts.txx:3990-4441
 //   * FFT_UpEquiv / Hadamard / IFFT .. include/fmm_pts.txx:2689-2922,3249-3344,3597-3759
 //   * kernels ........................ include/kernel.txx:1137-1364
+//   * periodic trees and images ...... src/mortonid.cpp:15-45 (NbrList), include/mpi_tree.txx:
+//                                      1136-1170 (root colleagues), include/fmm_pts.txx:4646-4656,
+//                                      4934-4940,5228-5340 (coord_shift)
+//   * PeriodicBC / PostProcessing .... include/fmm_pts.txx:2641-2657,5740-5774 (the BC_Type
+//                                      operator itself is precomputed in __init__.py)
 // The reference's sctl::approx_rsqrt is replaced by an exact 1/sqrt (SCTL is not
 // vendored); OpenMP partitioning and summation order differ, so agreement with the
 // reference is at rounding level, not bitwise, for floating point results.  Tree and
@@ -94,8 +99,8 @@ struct MortonId {
   bool operator!=(const MortonId& m) const { return !(*this == m); }
   bool operator>(const MortonId& m) const { return m < *this; }
   bool isAncestor(const MortonId& o) const { return o.depth > depth && o.getAncestor(depth) == *this; }
-  // src/mortonid.cpp:15-45 (free space only)
-  void NbrList(std::vector<MortonId>& nbrs, int level) const {
+  // src/mortonid.cpp:15-45; periodic wraps every axis (balanceOctree passes bndry != FreeSpace)
+  void NbrList(std::vector<MortonId>& nbrs, int level, bool periodic = false) const {
     nbrs.clear();
     const int64_t maxCoord = ((int64_t)1) << MAX_DEPTH;
     uint32_t mask = (uint32_t)(maxCoord - (((int64_t)1) << (MAX_DEPTH - level)));
@@ -105,6 +110,14 @@ struct MortonId {
       int64_t nx = pX + ((i / 1) % 3 - 1) * step;
       int64_t ny = pY + ((i / 3) % 3 - 1) * step;
       int64_t nz = pZ + ((i / 9) % 3 - 1) * step;
+      if (periodic) {
+        if (nx < 0) nx += maxCoord;
+        if (nx >= maxCoord) nx -= maxCoord;
+        if (ny < 0) ny += maxCoord;
+        if (ny >= maxCoord) ny -= maxCoord;
+        if (nz < 0) nz += maxCoord;
+        if (nz >= maxCoord) nz -= maxCoord;
+      }
       if (nx >= 0 && nx < maxCoord && ny >= 0 && ny < maxCoord && nz >= 0 && nz < maxCoord) {
         MortonId t; t.x = (uint32_t)nx; t.y = (uint32_t)ny; t.z = (uint32_t)nz; t.depth = (uint8_t)level;
         nbrs.push_back(t);
@@ -199,6 +212,7 @@ struct Node {
 
 struct Tree {
   int max_pts = 0;
+  int bndry = 0;  // BoundaryType (include/mpi_tree.hpp:22-29): 0 FreeSpace, 1 PX, 2 PXY, 3 PXYZ
   const double* X[3] = {nullptr, nullptr, nullptr};  // coordinate arrays (AoS) of the three point sets
   int64_t N[3] = {0, 0, 0};
   Node* root = nullptr;
@@ -439,7 +453,7 @@ struct Tree {
   }
 
   // ---- balanceOctree at np=1, omp_p=1: include/mpi_tree.txx:745-943
-  static void balanceOctree(std::vector<MortonId>& in, std::vector<MortonId>& out, unsigned maxDepth) {
+  static void balanceOctree(std::vector<MortonId>& in, std::vector<MortonId>& out, unsigned maxDepth, bool periodic) {
     if (in.size() == 1) { out = in; return; }
     std::vector<std::set<MortonId>> nodes(maxDepth + 1);
     for (size_t i = 0; i < in.size();) {
@@ -453,7 +467,7 @@ struct Tree {
     for (unsigned l = maxDepth; l >= 1; l--) {
       std::set<MortonId> nbrs_parent;
       for (const MortonId& node : nodes[l]) {
-        node.NbrList(nbrs, (int)l);
+        node.NbrList(nbrs, (int)l, periodic);
         for (const MortonId& nb : nbrs) nbrs_parent.insert(nb.getAncestor((int)l - 1));
       }
       nodes[l - 1].insert(nbrs_parent.begin(), nbrs_parent.end());
@@ -496,7 +510,7 @@ struct Tree {
   void Balance21() {
     std::vector<MortonId> in, out;
     for (Node* n = root; n; n = PreorderNxt(n)) if (n->leaf && !n->ghost) in.push_back(n->mid);
-    balanceOctree(in, out, MAX_DEPTH);
+    balanceOctree(in, out, MAX_DEPTH, bndry != 0);
     FindNode(out[0], true);
     FindNode(out[0], true);
     size_t node_iter = 0;
@@ -517,9 +531,16 @@ struct Tree {
     while (n) { Truncate(n); n->ghost = true; n = PreorderNxt(n); }
   }
 
-  // ---- SetColleagues (free space): include/mpi_tree.txx:1136-1255
+  // ---- SetColleagues: include/mpi_tree.txx:1136-1255.  The root is its own colleague at
+  // offset 0 and, along each periodic axis (PX: x; PXY: x, y; PXYZ: all), at offsets -1 and +1.
   void SetColleagues() {
-    root->colleague[13] = root;
+    for (int i = 0; i < 27; i++) root->colleague[i] = nullptr;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          if ((dx && bndry < 1) || (dy && bndry < 2) || (dz && bndry < 3)) continue;
+          root->colleague[((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)] = root;
+        }
     std::vector<std::vector<Node*>> by_depth(MAX_DEPTH + 1);
     for (Node* n = PreorderNxt(root); n; n = PreorderNxt(n)) by_depth[n->depth].push_back(n);
     for (int d = 0; d <= MAX_DEPTH; d++)
@@ -806,6 +827,7 @@ struct Operators {
   std::vector<double> u2u[8], d2d[8];                  // by child position bits
   std::vector<double> v1[27];                          // by parent offset (dx+1)+3(dy+1)+9(dz+1); layout [(j+i*k1)*M_dim + f][s*8+t][2]
   double src_scal = 0, trg_scal = 1;                   // m2m/m2l/l2l exponents (kernels here are scale invariant)
+  std::vector<double> bc;                              // periodic: root upward equiv -> root downward check (BC_Type)
 };
 
 static void matvec_acc(const double* x, int rows, const double* M, int cols, double* y) {
@@ -827,6 +849,17 @@ struct FMM {
   double t_phase[12] = {0};
 
   int ns() const { return O->nsurf; }
+
+  // Periodic image shift of source node s in list `type`, slot `k`, of target node t: the
+  // source's expected centre (t's centre plus the slot's relative coordinate) minus its actual
+  // centre; zero in free space.  Units of the relative coordinate: U1 the target size, U0/X
+  // half of it, U2/W a quarter (include/fmm_pts.txx:4646-4656,4934-4940,5228-5340).
+  void shift_of(const Node* t, const Node* s, int type, int k, double* sh) const {
+    const double st = std::pow(0.5, t->depth), ss = std::pow(0.5, s->depth);
+    const double unit = (type == U1_T) ? st : (type == U0_T || type == X_T) ? 0.5 * st : 0.25 * st;
+    const int* rel = &T->il.rel[type][3 * k];
+    for (int d = 0; d < 3; d++) sh[d] = rel[d] * unit + (t->coord[d] + 0.5 * st) - (s->coord[d] + 0.5 * ss);
+  }
 
   void gather_pts(const Node* n, int set, std::vector<double>& xyz, const double* shift) const {
     const std::vector<int64_t>& v = n->pts[set];
@@ -898,6 +931,10 @@ struct FMM {
         }
       }
     }
+    // ---------------- periodic boundary: root upward equivalent -> root downward check
+    // (FMM_Pts::PeriodicBC, include/fmm_pts.txx:2641-2657, called from include/fmm_tree.txx:582-584)
+    if (T->bndry != 0 && !o.bc.empty())
+      matvec_acc(up[nodes[0]->id].data(), n * k0, o.bc.data(), n * k1m, dn[nodes[0]->id].data());
     t_phase[1] = omp_get_wtime() - t0;
 
     // ---------------- X list (S2L): include/fmm_pts.txx:4444-4743
@@ -909,9 +946,12 @@ struct FMM {
       if (t->leaf && t->pt_cnt[1] <= Nsrf) continue;
       double zero[3] = {0, 0, 0};
       std::vector<double> surf = surf_at(m, zero, t->depth, RAD0);
-      for (Node* s : t->lst[X_T]) {
+      for (size_t k = 0; k < t->lst[X_T].size(); k++) {
+        Node* s = t->lst[X_T][k];
         if (!s || !(s->leaf && s->pts[0].size())) continue;
-        double shift[3] = {-t->coord[0], -t->coord[1], -t->coord[2]};
+        double shift[3];
+        shift_of(t, s, X_T, (int)k, shift);
+        for (int d = 0; d < 3; d++) shift[d] -= t->coord[d];
         std::vector<double> sx, sv;
         gather_pts(s, 0, sx, shift);
         gather_src_val(s, k0, sv);
@@ -929,10 +969,14 @@ struct FMM {
       if (!(t->leaf && !t->ghost && t->pts[1].size())) continue;
       std::vector<double> tx, tv(K.k1 * t->pts[1].size(), 0.0);
       gather_pts(t, 1, tx, nullptr);
-      for (Node* s : t->lst[W_T]) {
+      for (size_t k = 0; k < t->lst[W_T].size(); k++) {
+        Node* s = t->lst[W_T][k];
         if (!s || !s->pt_cnt[0]) continue;
         if (s->leaf && s->pt_cnt[0] <= Nsrf) continue;
-        std::vector<double> surf = surf_at(m, s->coord, s->depth, RAD0);
+        double sh[3];
+        shift_of(t, s, W_T, (int)k, sh);
+        double sc[3] = {s->coord[0] + sh[0], s->coord[1] + sh[1], s->coord[2] + sh[2]};
+        std::vector<double> surf = surf_at(m, sc, s->depth, RAD0);
         p2p(K, surf.data(), n, up[s->id].data(), tx.data(), (int64_t)t->pts[1].size(), tv.data());
       }
       for (size_t j = 0; j < t->pts[1].size(); j++) for (int d = 0; d < K.k1; d++) trg_val[K.k1 * t->pts[1][j] + d] += tv[K.k1 * j + d];
@@ -947,16 +991,27 @@ struct FMM {
       if (!(t->leaf && !t->ghost && t->pts[1].size())) continue;
       std::vector<double> tx, tv(K.k1 * t->pts[1].size(), 0.0), sx, sv;
       gather_pts(t, 1, tx, nullptr);
-      auto add = [&](Node* s) {
-        gather_pts(s, 0, sx, nullptr);
+      auto add = [&](Node* s, int ty, size_t k) {
+        double sh[3];
+        shift_of(t, s, ty, (int)k, sh);
+        gather_pts(s, 0, sx, sh);
         gather_src_val(s, K.k0, sv);
         p2p(K, sx.data(), (int64_t)s->pts[0].size(), sv.data(), tx.data(), (int64_t)t->pts[1].size(), tv.data());
       };
       for (int ty : {U0_T, U1_T, U2_T})
-        for (Node* s : t->lst[ty]) if (s && s->leaf && s->pts[0].size()) add(s);
+        for (size_t k = 0; k < t->lst[ty].size(); k++) {
+          Node* s = t->lst[ty][k];
+          if (s && s->leaf && s->pts[0].size()) add(s, ty, k);
+        }
       if (t->pt_cnt[1] <= Nsrf)
-        for (Node* s : t->lst[X_T]) if (s && s->leaf && s->pts[0].size()) add(s);
-      for (Node* s : t->lst[W_T]) if (s && s->leaf && s->pts[0].size() && s->pt_cnt[0] <= Nsrf) add(s);
+        for (size_t k = 0; k < t->lst[X_T].size(); k++) {
+          Node* s = t->lst[X_T][k];
+          if (s && s->leaf && s->pts[0].size()) add(s, X_T, k);
+        }
+      for (size_t k = 0; k < t->lst[W_T].size(); k++) {
+        Node* s = t->lst[W_T][k];
+        if (s && s->leaf && s->pts[0].size() && s->pt_cnt[0] <= Nsrf) add(s, W_T, k);
+      }
       for (size_t j = 0; j < t->pts[1].size(); j++) for (int d = 0; d < K.k1; d++) trg_val[K.k1 * t->pts[1][j] + d] += tv[K.k1 * j + d];
     }
     t_phase[4] = omp_get_wtime() - t0;
@@ -999,6 +1054,23 @@ struct FMM {
       for (size_t j = 0; j < t->pts[1].size(); j++) for (int d = 0; d < K.k1; d++) trg_val[K.k1 * t->pts[1][j] + d] += tv[K.k1 * j + d];
     }
     t_phase[7] = omp_get_wtime() - t0;
+
+    // ---------------- FMM_Pts::PostProcessing (include/fmm_pts.txx:5740-5774): with PXYZ and
+    // a kernel whose M2L carries the volume potential of a uniform density (laplace_poten:
+    // -|x|^2/6; stokes_vel: -(|x|^2 - x_j^2)/4 on the diagonal, include/kernel.txx:1289-1295,
+    // 1456-1467), subtract the root's total equivalent density times that potential.
+    if (T->bndry == 3 && !o.bc.empty() && K.id != 1) {
+      double avg[3] = {0, 0, 0};
+      const std::vector<double>& ue = up[nodes[0]->id];
+      for (size_t i = 0; i < ue.size(); i += k0) for (int c = 0; c < k0; c++) avg[c] += ue[i + c];
+      #pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < T->N[1]; i++) {
+        const double* x = &T->X[1][3 * i];
+        const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+        if (K.id == 0) trg_val[i] -= avg[0] * (-r2 / 6);
+        else for (int j = 0; j < 3; j++) trg_val[3 * i + j] -= avg[j] * (-(r2 - x[j] * x[j]) / 4);
+      }
+    }
   }
 
   void VList(int max_depth) {
@@ -1118,7 +1190,7 @@ extern "C" {
 
 // sets: src (n_src), trg (n_trg), pt (tree-defining points; pass trg again for PtFMM_CreateTree)
 void* oracle_tree_create(const double* src, int64_t n_src, const double* trg, int64_t n_trg,
-                         const double* pt, int64_t n_pt, int max_pts, int refine) {
+                         const double* pt, int64_t n_pt, int max_pts, int refine, int bndry) {
   OracleHandle* h = new OracleHandle();
   h->coord_copy[0].assign(src, src + 3 * n_src);
   h->coord_copy[1].assign(trg, trg + 3 * n_trg);
@@ -1126,6 +1198,7 @@ void* oracle_tree_create(const double* src, int64_t n_src, const double* trg, in
   for (int j = 0; j < 3; j++) { h->tree.X[j] = h->coord_copy[j].data(); }
   h->tree.N[0] = n_src; h->tree.N[1] = n_trg; h->tree.N[2] = n_pt;
   h->tree.max_pts = max_pts;
+  h->tree.bndry = bndry;
   h->tree.il.Initialize();
   h->tree.Initialize();          // include/fmm_tree.txx:23-41
   if (refine) h->tree.RefineTree();  // include/fmm_tree.txx:44-69
@@ -1191,6 +1264,15 @@ void oracle_set_operators(void* p, int kernel, int m, const double* uc2ue0, cons
   size_t vsz = (size_t)k0 * k1 * M_dim * 128;
   for (int r = 0; r < 27; r++) { if (r == 13) continue; o.v1[r].assign(v1 + r * vsz, v1 + (r + 1) * vsz); }
   o.src_scal = 0.0; o.trg_scal = 1.0;  // potential-type expansion kernels (include/kernel.txx:72-199)
+}
+
+// Periodic boundary operator (n_surf*k0 x n_surf*k1 of the expansion kernel), from the
+// Python precompute; without it a periodic tree evaluates only the wrapped lists.
+void oracle_set_bc(void* p, const double* bc) {
+  OracleHandle* h = (OracleHandle*)p;
+  const Operators& o = h->ops;
+  size_t sz = (size_t)o.nsurf * o.Km.k0 * o.nsurf * o.Km.k1;
+  h->ops.bc.assign(bc, bc + sz);
 }
 
 // Evaluate: src_val (input order, k0 per source), trg_val (input order, k1 per target, overwritten)

@@ -85,7 +85,7 @@ struct TreeHandle {
 };
 
 TreeHandle* make_tree(const double* src, int64_t n_src, const double* trg, int64_t n_trg, const double* pt, int64_t n_pt,
-                      int max_pts) {
+                      int max_pts, int bndry = kFreeSpace) {
   if (n_src < 0 || n_trg < 0 || n_pt < 0) throw Error(kErrInvalidArgument, "negative point count");
   if ((n_src && !src) || (n_trg && !trg) || (n_pt && !pt)) throw Error(kErrInvalidArgument, "null coordinate array");
   auto h = std::make_unique<TreeHandle>();
@@ -93,10 +93,11 @@ TreeHandle* make_tree(const double* src, int64_t n_src, const double* trg, int64
   h->trg.assign(trg, trg + 3 * n_trg);
   // the tree-defining set of PtFMM_CreateTree is the target set itself (include/pvfmm.txx:239)
   const double* pts = (pt == trg && n_pt == n_trg) ? h->trg.data() : pt;
-  if (device_tree_enabled())
+  // periodic trees are built on the host (the device builder covers free space)
+  if (bndry == kFreeSpace && device_tree_enabled())
     build_octree_device(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pts, n_pt, max_pts);
   else
-    build_octree(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pts, n_pt, max_pts);
+    build_octree(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pts, n_pt, max_pts, bndry);
   return h.release();
 }
 
@@ -105,6 +106,7 @@ void setup_tree(TreeHandle* t, FmmHandle* f, int n_vec_max) {
   t->ensure_stream();
   t->plan.reset();
   t->fmm = f;
+  if (t->T.bndry != kFreeSpace) build_bc_operator(f->O, t->T.bndry);  // once per operator set and boundary type
   t->plan = make_plan(t->T, f->O, t->src.data(), t->trg.data(), f->precision, n_vec_max, t->stream);
 }
 
@@ -261,7 +263,6 @@ struct Context {
 
 template <typename Real>
 Context* create_context(Real box_size, int n, int m, int kernel, bool single) {
-  if (box_size > 0) throw Error(kErrUnsupported, "periodic boxes (box_size > 0) are not supported; pass box_size <= 0");
   if (n <= 0) throw Error(kErrInvalidArgument, "max points per box must be positive");
   kernel_info(kernel);  // validates the kernel
   auto c = std::make_unique<Context>();
@@ -315,15 +316,20 @@ void context_eval(Context* c, const Real* src_pos, const Real* sl_den, const Rea
   const KernelInfo ki = kernel_info(c->kernel);
   if (setup || !c->tree) {
     Real scale_x, shift_x[3];
-    Real s0, x0[3], s1, x1[3];
-    bounding_box<Real>(n_src, src_pos, &s0, x0);
-    bounding_box<Real>(n_trg, trg_pos, &s1, x1);
-    Real c0[3] = {(Real)(0.5 - x0[0]) / s0, (Real)(0.5 - x0[1]) / s0, (Real)(0.5 - x0[2]) / s0};
-    Real c1[3] = {(Real)(0.5 - x1[0]) / s1, (Real)(0.5 - x1[1]) / s1, (Real)(0.5 - x1[2]) / s1};
-    scale_x = 0;
-    for (int k = 0; k < 3; k++) scale_x = std::max<Real>(scale_x, std::fabs(c0[k] - c1[k]));
-    scale_x = 1 / (scale_x + 1 / s0 + 1 / s1);
-    for (int k = 0; k < 3; k++) shift_x[k] = (Real)0.5 - (c0[k] + c1[k]) * scale_x / 2;
+    if (c->box_size <= 0) {
+      Real s0, x0[3], s1, x1[3];
+      bounding_box<Real>(n_src, src_pos, &s0, x0);
+      bounding_box<Real>(n_trg, trg_pos, &s1, x1);
+      Real c0[3] = {(Real)(0.5 - x0[0]) / s0, (Real)(0.5 - x0[1]) / s0, (Real)(0.5 - x0[2]) / s0};
+      Real c1[3] = {(Real)(0.5 - x1[0]) / s1, (Real)(0.5 - x1[1]) / s1, (Real)(0.5 - x1[2]) / s1};
+      scale_x = 0;
+      for (int k = 0; k < 3; k++) scale_x = std::max<Real>(scale_x, std::fabs(c0[k] - c1[k]));
+      scale_x = 1 / (scale_x + 1 / s0 + 1 / s1);
+      for (int k = 0; k < 3; k++) shift_x[k] = (Real)0.5 - (c0[k] + c1[k]) * scale_x / 2;
+    } else {  // periodic box [0, box_size)^3 (src/pvfmm-wrapper.cpp:514-519): coordinates wrap into it
+      scale_x = 1 / (Real)c->box_size;
+      shift_x[0] = shift_x[1] = shift_x[2] = 0;
+    }
     auto map = [&](const Real* p, long n, std::vector<double>& out) {
       out.resize(3 * (size_t)n);
       for (long i = 0; i < n; i++)
@@ -338,7 +344,8 @@ void context_eval(Context* c, const Real* src_pos, const Real* sl_den, const Rea
     map(src_pos, n_src, sp);
     map(trg_pos, n_trg, tp);
     c->tree.reset();
-    c->tree.reset(make_tree(sp.data(), n_src, tp.data(), n_trg, c->grid.data(), (int64_t)(c->grid.size() / 3), c->max_pts));
+    c->tree.reset(make_tree(sp.data(), n_src, tp.data(), n_trg, c->grid.data(), (int64_t)(c->grid.size() / 3), c->max_pts,
+                            c->box_size > 0 ? kPXYZ : kFreeSpace));
     if (c->fmm->precision != c->precision) {
       // the pseudo-inverse truncation depends on the evaluation precision
       build_operators_cached(c->fmm->O, c->kernel, c->m, 0.0, c->precision, std::getenv("PVFMM_DIR"));
@@ -467,6 +474,12 @@ void pvfmm_amd_fmm_destroy(void* fmm) { delete (FmmHandle*)fmm; }
 void* pvfmm_amd_tree_create(const double* src, int64_t n_src, const double* trg, int64_t n_trg, int max_pts) {
   TreeHandle* t = nullptr;
   guarded([&] { t = make_tree(src, n_src, trg, n_trg, trg, n_trg, max_pts); });
+  return t;
+}
+void* pvfmm_amd_tree_create_bndry(const double* src, int64_t n_src, const double* trg, int64_t n_trg, const double* pt,
+                                  int64_t n_pt, int max_pts, int bndry) {
+  TreeHandle* t = nullptr;
+  guarded([&] { t = make_tree(src, n_src, trg, n_trg, pt ? pt : trg, pt ? n_pt : n_trg, max_pts, bndry); });
   return t;
 }
 void* pvfmm_amd_tree_create_pts(const double* src, int64_t n_src, const double* trg, int64_t n_trg, const double* pt,
@@ -624,6 +637,12 @@ int pvfmm_amd_tree_stats(void* tree, int64_t* o, int cap) {
 int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out) {
   FmmHandle* f = (FmmHandle*)fmm;
   if (!f) return -1;
+  if (which >= 7 && which <= 9) {  // periodic boundary operators, built on first request
+    int64_t n = -1;
+    if (guarded([&] { build_bc_operator(f->O, which - 6); n = (int64_t)f->O.bc[which - 6].size(); }) != 0) return -1;
+    if (out) std::copy(f->O.bc[which - 6].begin(), f->O.bc[which - 6].end(), out);
+    return n;
+  }
   const Operators& O = f->O;
   const std::vector<double>* v = nullptr;
   switch (which) {

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 
 #include "kernels.hpp"
 
@@ -254,8 +255,13 @@ class DevicePlan final : public DevicePlanBase {
   int64_t n_src_ = 0, n_trg_ = 0;
   int k0_ = 1, k1_ = 1, e0_ = 1, e1_ = 1, K0_ = 0, K1_ = 0, ns_ = 0, m_ = 0, M_dim_ = 0;
   int n_nodes_ = 0, max_depth_ = 0;
+  int bndry_ = 0;     // boundary type of the tree (0 free space, 1-3 periodic)
+  bool vol_ = false;  // PXYZ with a kernel carrying the analytic volume potential
+  DBuf<Real> bc_;     // periodic boundary operator (K0_ x K1_)
+  DBuf<double> txa_;  // target coordinates in sorted order (SoA), for the volume-potential correction
 
-  // tree arrays
+  // tree arrays (periodic trees append virtual image nodes: a source node's point range and size
+  // with its centre moved by whole periods)
   DBuf<double> center_;
   DBuf<float> node_size_;
   DBuf<int32_t> src_begin_, src_end_, trg_begin_, trg_end_;
@@ -466,11 +472,43 @@ class DevicePlan final : public DevicePlanBase {
     max_depth_ = T.max_depth;
     if (n_src_ >= (1ll << 31) || n_trg_ >= (1ll << 31)) throw Error(kErrUnsupported, "more than 2^31 points");
     const int N = n_nodes_;
+    bndry_ = T.bndry;
+
+    // ---- periodic images: every (source node, shift) that a U, W or X list of a target node
+    // reaches across the boundary becomes a virtual node N + j
+    std::unordered_map<int64_t, int32_t> img_of;
+    std::vector<std::pair<int32_t, int>> imgs;  // (source node, shift code (sx+1) + 3(sy+1) + 9(sz+1))
+    auto img = [&](int32_t t, int type, int k, int32_t s) -> int32_t {
+      if (bndry_ == kFreeSpace) return s;
+      int sh[3];
+      image_shift(T, t, s, type, k, sh);
+      const int code = (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1);
+      if (code == 13) return s;
+      const int64_t key = (int64_t)s * 27 + code;
+      auto it = img_of.find(key);
+      if (it != img_of.end()) return it->second;
+      const int32_t v = N + (int32_t)imgs.size();
+      img_of.emplace(key, v);
+      imgs.emplace_back(s, code);
+      return v;
+    };
+    if (bndry_ != kFreeSpace)
+      for (int t = 0; t < N; t++) {
+        if (T.pt_cnt1(t) == 0) continue;
+        for (int type : {kU0, kU1, kU2, kW, kX}) {
+          const int nl = T.list_count(type);
+          for (int k = 0; k < nl; k++) {
+            const int32_t s = T.lists[type][(size_t)t * nl + k];
+            if (s >= 0) (void)img(t, type, k, s);
+          }
+        }
+      }
+    const int NV = N + (int)imgs.size();
 
     // ---- node arrays
-    std::vector<double> center(3 * (size_t)N);
-    std::vector<float> nsz(N);
-    std::vector<int32_t> sb(N), se(N), tb(N), te(N);
+    std::vector<double> center(3 * (size_t)NV);
+    std::vector<float> nsz(NV);
+    std::vector<int32_t> sb(NV), se(NV), tb(NV, 0), te(NV, 0);
     for (int i = 0; i < N; i++) {
       double c[3];
       T.corner(i, c);
@@ -479,6 +517,13 @@ class DevicePlan final : public DevicePlanBase {
       nsz[i] = (float)s;
       sb[i] = (int32_t)T.src_begin[i]; se[i] = (int32_t)T.src_end[i];
       tb[i] = (int32_t)T.trg_begin[i]; te[i] = (int32_t)T.trg_end[i];
+    }
+    for (size_t j = 0; j < imgs.size(); j++) {
+      const int32_t s = imgs[j].first, v = N + (int32_t)j, code = imgs[j].second;
+      const int sh[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+      for (int d = 0; d < 3; d++) center[3 * (size_t)v + d] = center[3 * (size_t)s + d] + sh[d];
+      nsz[v] = nsz[s];
+      sb[v] = sb[s]; se[v] = se[s];
     }
     center_.upload(center, &total);
     node_size_.upload(nsz, &total);
@@ -733,7 +778,7 @@ class DevicePlan final : public DevicePlanBase {
       int cnt = 0;
       for (int k = 0; k < nl_x; k++) {
         int32_t s = T.lists[kX][(size_t)i * nl_x + k];
-        if (s >= 0 && T.is_leaf(s) && T.pt_cnt0(s) > 0) { x_items.push_back(s); cnt++; }
+        if (s >= 0 && T.is_leaf(s) && T.pt_cnt0(s) > 0) { x_items.push_back(img(i, kX, k, s)); cnt++; }
       }
       if (!cnt) continue;
       x_node.push_back(i); x_orow.push_back(i);
@@ -819,31 +864,43 @@ class DevicePlan final : public DevicePlanBase {
         int32_t s = T.lists[kW][(size_t)t * nl_w + k];
         if (s < 0 || T.pt_cnt0(s) == 0) continue;
         if (T.is_leaf(s) && T.pt_cnt0(s) <= ns_) continue;
-        far_node.push_back(s); far_den.push_back((int64_t)s * K0_); far_rad.push_back(kRad0);
+        far_node.push_back(img(t, kW, k, s)); far_den.push_back((int64_t)s * K0_); far_rad.push_back(kRad0);
       }
       far_pairs += (int64_t)(far_node.size() - far_ptr.back()) * ns_ * T.pt_cnt1(t);
       far_ptr.push_back((int32_t)far_node.size());
-      auto add = [&](int32_t s) {
+      auto add = [&](int32_t s, int type, int k) {
         if (s >= 0 && T.is_leaf(s) && T.pt_cnt0(s) > 0) {
-          u_items.push_back(s);
+          u_items.push_back(img(t, type, k, s));
           u_pairs += T.pt_cnt0(s) * T.pt_cnt1(t);
           u_stream += T.pt_cnt0(s);
         }
       };
-      for (int k = 0; k < nl_u0; k++) add(T.lists[kU0][(size_t)t * nl_u0 + k]);  // U_ListSetup 5197-5321
-      for (int k = 0; k < nl_u1; k++) add(T.lists[kU1][(size_t)t * nl_u1 + k]);
-      for (int k = 0; k < nl_u2; k++) add(T.lists[kU2][(size_t)t * nl_u2 + k]);
+      for (int k = 0; k < nl_u0; k++) add(T.lists[kU0][(size_t)t * nl_u0 + k], kU0, k);  // U_ListSetup 5197-5321
+      for (int k = 0; k < nl_u1; k++) add(T.lists[kU1][(size_t)t * nl_u1 + k], kU1, k);
+      for (int k = 0; k < nl_u2; k++) add(T.lists[kU2][(size_t)t * nl_u2 + k], kU2, k);
       if (T.pt_cnt1(t) <= ns_)
-        for (int k = 0; k < nl_x; k++) add(T.lists[kX][(size_t)t * nl_x + k]);
+        for (int k = 0; k < nl_x; k++) add(T.lists[kX][(size_t)t * nl_x + k], kX, k);
       for (int k = 0; k < nl_w; k++) {
         int32_t s = T.lists[kW][(size_t)t * nl_w + k];
-        if (s >= 0 && T.is_leaf(s) && T.pt_cnt0(s) <= ns_) add(s);
+        if (s >= 0 && T.is_leaf(s) && T.pt_cnt0(s) <= ns_) add(s, kW, k);
       }
       u_ptr.push_back((int32_t)u_items.size());
     }
     far_ptr_.upload(far_ptr, &total); far_node_.upload(far_node, &total); far_den_.upload(far_den, &total);
     far_rad_.upload(far_rad, &total);
     u_ptr_.upload(u_ptr, &total); u_items_.upload(u_items, &total);
+    if (bndry_ != kFreeSpace) {  // FMM_Pts::PeriodicBC and PostProcessing (include/fmm_pts.txx:2641-2657,5740-5774)
+      const std::vector<double>& bc = O_.bc[bndry_];
+      if (bc.size() != (size_t)K0_ * K1_) throw Error(kErrInternal, "periodic tree without its boundary operator");
+      bc_.upload(conv(bc), &total);
+      vol_ = bndry_ == kPXYZ && ki.id != 1;
+      if (vol_) {
+        std::vector<double> xa(3 * (size_t)n_trg_);
+        for (int64_t k = 0; k < n_trg_; k++)
+          for (int d = 0; d < 3; d++) xa[d * n_trg_ + k] = trg_xyz[3 * T.trg.perm[k] + d];
+        txa_.upload(xa, &total);
+      }
+    }
     h_.tleaf = tleaf; h_.u_ptr = u_ptr; h_.u_items = u_items;
     h_.far_ptr = far_ptr; h_.far_node = far_node; h_.far_rad = far_rad;
 
@@ -1117,11 +1174,12 @@ class DevicePlan final : public DevicePlanBase {
   }
 
  public:
-  bool adjoint_supported() const override { return O_.ker.pt_kind == O_.ker.exp_kind; }
+  bool adjoint_supported() const override { return O_.ker.pt_kind == O_.ker.exp_kind && bndry_ == kFreeSpace; }
   void evaluate_adjoint(const double* d_g, double* d_out, int n_vec, double den_scale, double out_scale) override {
     if (!adjoint_supported())
-      throw Error(kErrUnsupported, std::string("the adjoint evaluation needs a symmetric kernel (laplace_poten, stokes_vel); ") +
-                                       O_.ker.name + " is not");
+      throw Error(kErrUnsupported, bndry_ != kFreeSpace ? std::string("the adjoint evaluation needs a free-space tree")
+                                                        : std::string("the adjoint evaluation needs a symmetric kernel (laplace_poten, stokes_vel); ") +
+                                                              O_.ker.name + " is not");
     if (n_vec < 1) throw Error(kErrInvalidArgument, "n_vec must be >= 1");
     if (!adj_) build_adjoint();
     for (int v = 0; v < n_vec; v += nvb_)
@@ -1161,6 +1219,10 @@ class DevicePlan final : public DevicePlanBase {
                      gemm_work_elems_, sp, nv, eq_vs_, eq_vs_);
       }
     }
+    // periodic images beyond the root's colleagues: root check potential += root equivalent x BC
+    if (bndry_ != kFreeSpace)
+      K::gemm_rows(st_, 1, K1_, K0_, eq_.p, lvl_seq_.p, K0_, bc_.p, dn_.p, lvl_seq_.p, K1_, nullptr, true, gemm_work_.p,
+                   gemm_work_elems_, sp, nv, eq_vs_, dn_vs_);
     mark(4);
     // X list: source points -> downward check surface (RAD0)
     K::p2p_surface(st_, ek, n_x_, x_node_.p, x_ptr_.p, x_items_.p, src_begin_.p, src_end_.p, center_.p, node_size_.p, kRad0,
@@ -1227,6 +1289,8 @@ class DevicePlan final : public DevicePlanBase {
     // U list: near-field P2P
     K::p2p_points(st_, pk, n_tleaf_, t_leaf_.p, u_ptr_.p, u_items_.p, trg_begin_.p, trg_end_.p, src_begin_.p, src_end_.p,
                   center_.p, tx64_.p, n_trg_, sx64_.p, sden_.p, n_src_, tval_.p, true, nv, den_vs, val_vs);
+    if (vol_)  // minus the root's total equivalent density times the analytic volume potential
+      K::vol_correction(st_, pk, n_trg_, txa_.p, eq_.p, K0_, e0_, tval_.p, nv, eq_vs_, val_vs);
     mark(12);
     // targets not in any leaf with targets are impossible; every target lies in exactly one leaf
     K::scatter_output(st_, n_trg_, k1_, tval_.p, trg_perm_.p, out_scale, d_out, nv, val_vs);

@@ -124,6 +124,40 @@ __global__ void k_fill(T* p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = T(0);
 }
 
+// Periodic volume-potential correction: each block first sums the root's equivalent values per
+// component (K0 of them, at most a few thousand, read through L2), then corrects its targets.
+// Grid (blocks, nv); val SoA [j n + i] per vector, x SoA absolute coordinates.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_vol_correction(int kind, int64_t n, const double* __restrict__ x,
+                                                        const Real* __restrict__ eq_root, int K0, int e0,
+                                                        double* __restrict__ val, int64_t eq_vs, int64_t val_vs) {
+  __shared__ double part[3][256];
+  const int v = blockIdx.y;
+  const Real* eq = eq_root + v * eq_vs;
+  double acc[3] = {0, 0, 0};
+  for (int i = threadIdx.x; i < K0; i += 256) acc[i % e0] += (double)eq[i];
+  for (int c = 0; c < 3; c++) part[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 3; c++) part[c][threadIdx.x] += part[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  const double a0 = part[0][0], a1 = part[1][0], a2 = part[2][0];
+  double* out = val + v * val_vs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double x0 = x[i], x1 = x[n + i], x2 = x[2 * n + i];
+    const double r2 = x0 * x0 + x1 * x1 + x2 * x2;
+    if (kind == 0) {
+      out[i] += a0 * r2 * (1.0 / 6.0);
+    } else {
+      out[i] += a0 * (r2 - x0 * x0) * 0.25;
+      out[n + i] += a1 * (r2 - x1 * x1) * 0.25;
+      out[2 * n + i] += a2 * (r2 - x2 * x2) * 0.25;
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------- direct sum
 // FP64 all-pairs evaluation (the reference's Kernel<T>::ker_poten, include/kernel.hpp:84).
 // Block (b, c) takes targets [256 b, 256 b + 256), one per thread, and the sources of chunk c,
@@ -2000,6 +2034,15 @@ template <typename Real>
 void DeviceKernels<Real>::fill(hipStream_t s, Real* p, int64_t n) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_fill<Real>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
+  HIP_LAUNCH_CHECK();
+}
+template <typename Real>
+void DeviceKernels<Real>::vol_correction(hipStream_t s, int kind, int64_t n, const double* x, const Real* eq_root, int K0,
+                                         int e0, double* val, int nv, int64_t eq_vs, int64_t val_vs) {
+  if (n == 0 || nv <= 0) return;
+  if ((kind != 0 && kind != 2) || e0 < 1 || e0 > 3) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+  hipLaunchKernelGGL(k_vol_correction<Real>, dim3(std::min(grid_for(n, 256), 1024), nv), dim3(256), 0, s, kind, n, x, eq_root,
+                     K0, e0, val, eq_vs, val_vs);
   HIP_LAUNCH_CHECK();
 }
 template <typename Real>

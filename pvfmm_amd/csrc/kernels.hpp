@@ -26,6 +26,12 @@ struct DeviceKernels {
   static void scatter_output(hipStream_t s, int64_t n, int k1, const double* val, const int64_t* perm,
                              double scal, double* out, int n_vec, int64_t out_stride);
   static void fill(hipStream_t s, Real* p, int64_t n);
+  // Periodic (PXYZ) post-processing of FMM_Pts::PostProcessing (include/fmm_pts.txx:5740-5774):
+  // val[j*n + i] -= sum_c avg_c V_cj(x_i), avg_c the sum of component c over the root's K0
+  // equivalent values eq_root[p*e0 + c], V the kernel's analytic potential of a uniform unit
+  // density (kind 0: -|x|^2/6; kind 2: -(|x|^2 - x_j^2)/4 for c = j); x SoA absolute coordinates.
+  static void vol_correction(hipStream_t s, int kind, int64_t n, const double* x, const Real* eq_root, int K0, int e0,
+                             double* val, int nv, int64_t eq_vs, int64_t val_vs);
   static void fill64(hipStream_t s, double* p, int64_t n);
 
   // P2P into surface points (check surfaces): per row, targets are ns surface points of

@@ -139,11 +139,26 @@ const RelTable& rel_table(int type) {
 }
 
 // ----------------------------------------------------------------------------- tree build
+void image_shift(const Octree& T, int32_t t, int32_t s, int type, int k, int* shift) {
+  const std::array<int, 3>& rel = rel_table(type).rel[k];
+  // in units of 2^-32 of the period: node sizes are at least 2^-29, so a quarter of one is whole
+  const int64_t st = (int64_t)4 << (30 - T.depth[t]), ss = (int64_t)4 << (30 - T.depth[s]);
+  const int64_t unit = type == kU1 || type == kV1 ? st : (type == kU0 || type == kX) ? st / 2 : st / 4;
+  const uint32_t ct[3] = {T.x[t], T.y[t], T.z[t]}, cs[3] = {T.x[s], T.y[s], T.z[s]};
+  for (int d = 0; d < 3; d++) {
+    const int64_t diff = rel[d] * unit + (4 * (int64_t)ct[d] + st / 2) - (4 * (int64_t)cs[d] + ss / 2);
+    shift[d] = (int)(diff / ((int64_t)4 << 30));
+    if (diff != (int64_t)shift[d] * ((int64_t)4 << 30)) throw Error(kErrInternal, "list entry is not a periodic image");
+  }
+}
+
 void build_octree(Octree& T, const double* src, int64_t n_src, const double* trg, int64_t n_trg,
-                  const double* pt, int64_t n_pt, int max_pts) {
+                  const double* pt, int64_t n_pt, int max_pts, int bndry) {
   if (max_pts <= 0) throw Error(kErrInvalidArgument, "max_pts must be positive");
+  if (bndry < kFreeSpace || bndry > kPXYZ) throw Error(kErrInvalidArgument, "boundary type must be 0 (free), 1 (PX), 2 (PXY) or 3 (PXYZ)");
   T = Octree();
   T.max_pts = max_pts;
+  T.bndry = bndry;
   sort_points(T.src, src, n_src);
   sort_points(T.trg, trg, n_trg);
   PointSet P;
@@ -207,6 +222,9 @@ void build_octree(Octree& T, const double* src, int64_t n_src, const double* trg
         for (int dy = -1; dy <= 1; dy++)
           for (int dx = -1; dx <= 1; dx++) {
             int64_t nx = x + dx * step, ny = y + dy * step, nz = z + dz * step;
+            if (bndry != kFreeSpace) {  // the periodic balance wraps every axis
+              nx = (nx + maxc) % maxc; ny = (ny + maxc) % maxc; nz = (nz + maxc) % maxc;
+            }
             if (nx < 0 || ny < 0 || nz < 0 || nx >= maxc || ny >= maxc || nz >= maxc) continue;
             add.push_back(node_id(l - 1, (uint32_t)nx & pmask, (uint32_t)ny & pmask, (uint32_t)nz & pmask));
           }
@@ -284,6 +302,9 @@ void build_octree(Octree& T, const double* src, int64_t n_src, const double* trg
     for (int c = 0; c < 27; c++) {
       int dx = c % 3 - 1, dy = (c / 3) % 3 - 1, dz = c / 9 - 1;
       int64_t nx = (int64_t)T.x[i] + dx * step, ny = (int64_t)T.y[i] + dy * step, nz = (int64_t)T.z[i] + dz * step;
+      if (bndry >= kPX) nx = (nx + maxc) % maxc;
+      if (bndry >= kPXY) ny = (ny + maxc) % maxc;
+      if (bndry >= kPXYZ) nz = (nz + maxc) % maxc;
       if (nx < 0 || ny < 0 || nz < 0 || nx >= maxc || ny >= maxc || nz >= maxc) continue;
       auto it = where.find(node_id(dd, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz));
       if (it != where.end()) T.colleague[(size_t)i * 27 + c] = it->second;

@@ -34,8 +34,12 @@ struct PointSet {
   std::vector<uint32_t> ix, iy, iz;  // 30-bit integer coordinates, sorted
 };
 
+// Boundary types of the reference (BoundaryType, include/mpi_tree.hpp:22-29).
+enum Bndry : int { kFreeSpace = 0, kPX = 1, kPXY = 2, kPXYZ = 3 };
+
 struct Octree {
   int max_pts = 0;
+  int bndry = kFreeSpace;            // periodic axes: PX x; PXY x, y; PXYZ all (unit period)
   int max_depth = 0;                 // deepest level present
   // nodes in level-major order; siblings are contiguous (first_child .. first_child+7)
   std::vector<uint32_t> x, y, z;     // integer corner coordinates (units of 2^-30)
@@ -61,8 +65,17 @@ struct Octree {
 
 // Build the tree of PtFMM_CreateTree + SetupFMM for the given point sets.  `pt` is the set the
 // reference stores in MPI_Node::pt_coord (the targets for PtFMM_CreateTree, include/pvfmm.txx:239).
-// Coordinates must lie in [0,1)^3.
+// Coordinates must lie in [0,1)^3.  bndry (a Bndry value) selects periodic neighbours: the 2:1
+// balance then wraps every axis (balanceOctree's periodic flag, include/mpi_tree.txx:745-967)
+// and the colleagues wrap along the periodic axes (SetColleagues, include/mpi_tree.txx:1136-1170).
 void build_octree(Octree& T, const double* src, int64_t n_src, const double* trg, int64_t n_trg,
-                  const double* pt, int64_t n_pt, int max_pts);
+                  const double* pt, int64_t n_pt, int max_pts, int bndry = kFreeSpace);
+
+// Periodic image shift (whole periods per axis, in {-1, 0, 1}) of the source node s that sits in
+// slot k of the list of type `type` of target node t: the slot's relative coordinate places the
+// source's centre at t's centre plus rel * unit (unit: t's size for U1 and V1, half of it for U0
+// and X, a quarter for U2 and W), and the shift is that place minus s's centre
+// (coord_shift, include/fmm_pts.txx:4646-4656,4934-4940,5228-5340).  Zero in free space.
+void image_shift(const Octree& T, int32_t t, int32_t s, int type, int k, int* shift);
 
 }  // namespace pvfmm_amd

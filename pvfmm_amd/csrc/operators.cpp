@@ -19,6 +19,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include "device_fmm.hpp"
+#include "octree.hpp"
 #include "kernels.hpp"
 #include "precompute.hpp"
 
@@ -439,6 +440,378 @@ void build_operators(Operators& O, int kernel_id, int m, double cutoff_rel, int 
   compute_factors(O);
   compute_translations(O);
   compute_v1(O);
+}
+
+// ----------------------------------------------------------------------------- periodic boundary
+namespace {
+
+// Gauss-Legendre rule on [0, 1] (Newton iteration on P_n from the Chebyshev-like first guess)
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w) {
+  const double pi = 3.14159265358979323846;
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; i++) {
+    double z = std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 1;
+    for (int it = 0; it < 100; it++) {
+      double p1 = 1, p0 = 0;
+      for (int j = 1; j <= n; j++) {
+        const double pm = p0;
+        p0 = p1;
+        p1 = ((2 * j - 1) * z * p0 - (j - 1) * pm) / j;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-17) break;
+    }
+    x[i] = 0.5 * (1 - z);
+    w[i] = 1.0 / ((1 - z * z) * dp * dp);
+  }
+}
+
+// One Duffy pyramid of the signed box [0, E]^3 (E relative to the point, singular corner at the
+// point): for a kernel homogeneous of degree -1 the radial integral is 1/2 and what is left is
+// the smooth integral over (s, t) in [0, 1]^2 of K at E[ax] along axis ax and E[p] s, E[q] t
+// along the others, graded geometrically toward s = t = 0 when E[ax] is small.
+void duffy_pyramid(int kind, int ax, const double* E, int ne, const std::vector<double>& gx, const std::vector<double>& gw,
+                   double* acc) {
+  const int p = (ax + 1) % 3, q = (ax + 2) % 3;
+  const double lead = std::fabs(E[ax]), rho = std::max(std::fabs(E[p]), std::fabs(E[q])) / lead;
+  const int L = rho > 1 ? std::min(30, (int)std::ceil(std::log(rho) / std::log(5.0))) : 0;
+  std::vector<double> sx, sw;  // graded nodes on [0, 1]
+  for (int l = L; l >= 0; l--) {
+    const double a = l == L ? 0.0 : std::pow(5.0, -l - 1), b = std::pow(5.0, -l);
+    for (size_t i = 0; i < gx.size(); i++) {
+      sx.push_back(a + (b - a) * gx[i]);
+      sw.push_back((b - a) * gw[i]);
+    }
+  }
+  double blk[9], r[3];
+  r[ax] = E[ax];
+  for (size_t i = 0; i < sx.size(); i++)
+    for (size_t j = 0; j < sx.size(); j++) {
+      r[p] = E[p] * sx[i];
+      r[q] = E[q] * sx[j];
+      eval_pair(kind, r, blk);
+      const double wt = sw[i] * sw[j];
+      for (int c = 0; c < ne; c++) acc[c] += wt * blk[c];
+    }
+}
+
+// row-major 8x8 inverse by Gauss-Jordan with partial pivoting
+void invert8(const double* A, double* Ainv) {
+  double a[8][16];
+  for (int i = 0; i < 8; i++)
+    for (int j = 0; j < 16; j++) a[i][j] = j < 8 ? A[i * 8 + j] : (j - 8 == i);
+  for (int c = 0; c < 8; c++) {
+    int piv = c;
+    for (int i = c + 1; i < 8; i++)
+      if (std::fabs(a[i][c]) > std::fabs(a[piv][c])) piv = i;
+    for (int j = 0; j < 16; j++) std::swap(a[c][j], a[piv][j]);
+    const double d = a[c][c];
+    for (int j = 0; j < 16; j++) a[c][j] /= d;
+    for (int i = 0; i < 8; i++)
+      if (i != c) {
+        const double f = a[i][c];
+        for (int j = 0; j < 16; j++) a[i][j] -= f * a[c][j];
+      }
+  }
+  for (int i = 0; i < 8; i++)
+    for (int j = 0; j < 8; j++) Ainv[i * 8 + j] = a[i][8 + j];
+}
+
+// the analytic potential of a uniform unit density (laplace_vol_poten, stokes_vol_poten,
+// include/kernel.txx:1289-1295,1456-1467): e0 x (n*e1)
+std::vector<double> vol_poten(int kind, const double* x, int n) {
+  const int e0 = kind == 2 ? 3 : 1, e1 = e0;
+  std::vector<double> out((size_t)e0 * n * e1, 0.0);
+  for (int i = 0; i < n; i++) {
+    const double r2 = x[3 * i] * x[3 * i] + x[3 * i + 1] * x[3 * i + 1] + x[3 * i + 2] * x[3 * i + 2];
+    if (kind == 0) out[i] = -r2 / 6;
+    else
+      for (int k = 0; k < 3; k++) out[(size_t)k * n * 3 + (size_t)i * 3 + k] = -(r2 - x[3 * i + k] * x[3 * i + k]) / 4;
+  }
+  return out;
+}
+
+// C = A (n x k) B (k x m), on the FP64 matrix cores when dev
+std::vector<double> mul(const std::vector<double>& A, const std::vector<double>& B, int n, int k, int m, bool dev) {
+  std::vector<double> C((size_t)n * m);
+  if (!dev) {
+    matmul(A.data(), B.data(), C.data(), n, k, m);
+    return C;
+  }
+  DevStream st;
+  DevArray a, b, c, io;
+  std::vector<int32_t> iota(n);
+  for (int i = 0; i < n; i++) iota[i] = i;
+  double* dc = c.alloc<double>((size_t)n * m);
+  gemm_device(st.s, n, m, k, a.upload(A), b.upload(B), dc, m, io.upload(iota));
+  HIP_CHECK(hipMemcpyAsync(C.data(), dc, sizeof(double) * C.size(), hipMemcpyDeviceToHost, st.s));
+  HIP_CHECK(hipStreamSynchronize(st.s));
+  return C;
+}
+
+}  // namespace
+
+void cube_integral(int kind, const double* pts, int n, double lo, double size, double* out) {
+  const int e0 = kind == 2 ? 3 : 1, e1 = kind == 0 ? 1 : 3, ne = e0 * e1;
+  if (kind == 1) throw Error(kErrInvalidArgument, "cube_integral: expansion kernels only (laplace, stokes)");
+  std::vector<double> gx, gw;
+  gauss_legendre01(20, gx, gw);
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < n; i++) {
+    // inclusion-exclusion over the 8 corners: signed boxes from the point to each corner
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 8; c++) {
+      double E[3], sign = 1;
+      for (int d = 0; d < 3; d++) {
+        const bool hi = (c >> d) & 1;
+        E[d] = (hi ? lo + size : lo) - pts[3 * i + d];
+        if (!hi) sign = -sign;
+      }
+      const double V = E[0] * E[1] * E[2];
+      if (V == 0) continue;
+      double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int ax = 0; ax < 3; ax++) duffy_pyramid(kind, ax, E, ne, gx, gw, J);
+      for (int q = 0; q < ne; q++) acc[q] += sign * 0.5 * V * J[q];
+    }
+    for (int q = 0; q < ne; q++) out[(size_t)i * ne + q] = acc[q];
+  }
+}
+
+void build_bc_operator(Operators& O, int bndry) {
+  if (bndry < 1 || bndry > 3) throw Error(kErrInvalidArgument, "periodic boundary type must be 1 (PX), 2 (PXY) or 3 (PXYZ)");
+  if (!O.bc[bndry].empty()) return;
+  const int m = O.m, mo = m + 2, kind = O.ker.exp_kind, e0 = O.ker.e0, e1 = O.ker.e1;
+  const bool dev = O.on_device;
+  Operators G;  // order m + 2: surfaces and check -> equivalent factors of the unit box
+  operators_geometry(G, O.ker.id, mo, O.pinv_rel_cutoff, 0);
+  G.on_device = dev;
+  compute_factors(G);
+  const int n = G.ns, K0 = n * e0, K1 = n * e1, ru = G.rank_u, rd = G.rank_d;
+  // the factors of a box of size 2 are those of the unit box times 2^(src_scal + trg_scal)
+  const double s2 = std::pow(2.0, O.ker.src_scal + O.ker.trg_scal);
+  auto surf = [](const Operators& Q, double cx, double cy, double cz, int depth, double rad) {  // corner-anchored
+    const double size = std::ldexp(1.0, -depth), c[3] = {cx + 0.5 * size, cy + 0.5 * size, cz + 0.5 * size};
+    return surf_points(Q, c, size, rad);
+  };
+  auto K = [&](const std::vector<double>& a, const std::vector<double>& b) {
+    const int na = (int)a.size() / 3, nb = (int)b.size() / 3;
+    std::vector<double> M((size_t)na * e0 * nb * e1);
+    kernel_matrix(kind, a.data(), na, b.data(), nb, M.data());
+    return M;
+  };
+  auto keep = [&](int a, int b, int c) { return !((bndry == kPX && (b != 0 || c != 0)) || (bndry == kPXY && c != 0)); };
+  auto axpy = [](std::vector<double>& y, const std::vector<double>& x, double a) {
+    for (size_t i = 0; i < y.size(); i++) y[i] += a * x[i];
+  };
+  const bool vol = bndry == kPXYZ && O.ker.id != 1;  // laplace_grad's expansion kernel has no volume potential
+
+  // zero-average projections (include/fmm_pts.txx:862-913): Z removes the total equivalent
+  // charge spread as the equivalent density of a uniformly charged unit cube, C the mean check
+  // potential of each component
+  std::vector<double> S2E;  // e0 x K0, rows summing to 1
+  {
+    const std::vector<double> uc = surf(G, 0, 0, 0, 0, kRad1);
+    std::vector<double> gi((size_t)n * e0 * e1), s2c((size_t)e0 * K1);
+    cube_integral(kind, uc.data(), n, 0.0, 1.0, gi.data());
+    for (int j = 0; j < e0; j++)
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < e1; k++) s2c[(size_t)j * K1 + (size_t)i * e1 + k] = gi[((size_t)i * e0 + k) * e1 + j];
+    S2E = mul(mul(s2c, G.uc2ue0, e0, K1, ru, false), G.uc2ue1, e0, ru, K0, false);
+    for (int j = 0; j < e0; j++) {
+      double t = 0;
+      for (int i = 0; i < K0; i++) t += S2E[(size_t)j * K0 + i];
+      for (int i = 0; i < K0; i++) S2E[(size_t)j * K0 + i] /= t;
+    }
+  }
+  auto z_left = [&](std::vector<double>& X, int cols) {  // X <- Z X, X: K0 x cols
+    std::vector<double> R = mul(S2E, X, e0, K0, cols, false);
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < e0; k++) {
+        double* x = &X[((size_t)i * e0 + k) * cols];
+        for (int c = 0; c < cols; c++) x[c] -= R[(size_t)k * cols + c];
+      }
+  };
+  auto z_right = [&](std::vector<double>& X, int rows) {  // X <- X Z, X: rows x K0
+    #pragma omp parallel for schedule(static)
+    for (int r = 0; r < rows; r++) {
+      double* x = &X[(size_t)r * K0];
+      double t[3] = {0, 0, 0};
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < e0; k++) t[k] += x[i * e0 + k];
+      for (int j = 0; j < K0; j++)
+        for (int k = 0; k < e0; k++) x[j] -= t[k] * S2E[(size_t)k * K0 + j];
+    }
+  };
+  auto c_left = [&](std::vector<double>& X, int cols) {  // X <- C X, X: K1 x cols
+    for (int k = 0; k < e1; k++) {
+      std::vector<double> mean(cols, 0.0);
+      for (int i = 0; i < n; i++)
+        for (int c = 0; c < cols; c++) mean[c] += X[((size_t)i * e1 + k) * cols + c];
+      for (int i = 0; i < n; i++)
+        for (int c = 0; c < cols; c++) X[((size_t)i * e1 + k) * cols + c] -= mean[c] / n;
+    }
+  };
+  auto c_right = [&](std::vector<double>& X, int rows) {  // X <- X C, X: rows x K1
+    #pragma omp parallel for schedule(static)
+    for (int r = 0; r < rows; r++) {
+      double* x = &X[(size_t)r * K1];
+      double t[3] = {0, 0, 0};
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < e1; k++) t[k] += x[i * e1 + k];
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < e1; k++) x[i * e1 + k] -= t[k] / n;
+    }
+  };
+
+  // M2M: the 8 unit boxes of [-1, 1]^3 -> that box (include/fmm_pts.txx:917-941)
+  std::vector<double> M2M;
+  {
+    const std::vector<double> chk = surf(G, -1, -1, -1, -1, kRad1);
+    std::vector<double> S((size_t)K0 * K1, 0.0);
+    for (int c = 0; c < 8; c++)
+      if (keep(c & 1, (c >> 1) & 1, (c >> 2) & 1))
+        axpy(S, K(surf(G, -1 + (c & 1), -1 + ((c >> 1) & 1), -1 + ((c >> 2) & 1), 0, kRad0), chk), 1.0);
+    M2M = mul(mul(S, G.uc2ue0, K0, K1, ru, dev), G.uc2ue1, K0, ru, K0, dev);
+    for (double& v : M2M) v *= s2;
+  }
+  // L2L: [-1, 1]^3 -> its child [-1, 0]^3 (include/fmm_pts.txx:942-951)
+  std::vector<double> L2L;
+  {
+    const std::vector<double> MM = K(surf(G, -1, -1, -1, -1, kRad1), surf(G, -1, -1, -1, 0, kRad0));
+    L2L = mul(G.dc2de0, mul(G.dc2de1, MM, rd, K0, K1, dev), K1, rd, K1, dev);
+    for (double& v : L2L) v *= s2;
+  }
+  // M2L: unit boxes at offsets in [-2, 3]^3 not adjacent to [0, 1]^3 (include/fmm_pts.txx:952-975)
+  std::vector<double> M2L((size_t)K0 * K1, 0.0);
+  {
+    const std::vector<double> dnc = surf(G, 0, 0, 0, 0, kRad0);
+    for (int c = -2; c < 4; c++)
+      for (int b = -2; b < 4; b++)
+        for (int a = -2; a < 4; a++)
+          if (keep(a, b, c) && std::max(std::abs(a), std::max(std::abs(b), std::abs(c))) > 1)
+            axpy(M2L, K(surf(G, a, b, c, 0, kRad0), dnc), 1.0);
+  }
+  z_left(M2M, K0); z_right(M2M, K0);
+  c_left(L2L, K1); c_right(L2L, K1);
+  z_left(M2L, K1); c_right(M2L, K0);
+
+  // the levels: M <- M2L + M2M (2^-(src_scal + trg_scal) M) L2L, PVFMM_BC_LEVELS = 45 in all
+  const double down = 1.0 / s2;
+  for (double& v : M2M) v *= down;
+  std::vector<double> M = M2L;
+  const int levels = 45;
+  if (dev) {
+    DevStream st;
+    DevArray a, b, c, dm, dt, io;
+    std::vector<int32_t> iota(std::max(K0, K1));
+    for (size_t i = 0; i < iota.size(); i++) iota[i] = (int32_t)i;
+    const int32_t* I = io.upload(iota);
+    const double* d_m2m = a.upload(M2M);
+    const double* d_l2l = b.upload(L2L);
+    const double* d_m2l = c.upload(M2L);
+    double* d_M = dm.upload(M);
+    double* d_T = dt.alloc<double>((size_t)K0 * K1);
+    for (int l = 1; l < levels; l++) {
+      gemm_device(st.s, K0, K1, K0, d_m2m, d_M, d_T, K1, I);
+      HIP_CHECK(hipMemcpyAsync(d_M, d_m2l, sizeof(double) * M.size(), hipMemcpyDeviceToDevice, st.s));
+      DeviceKernels<double>::gemm_rows(st.s, K0, K1, K1, d_T, I, K1, d_l2l, d_M, I, K1, nullptr, true);
+    }
+    HIP_CHECK(hipMemcpyAsync(M.data(), d_M, sizeof(double) * M.size(), hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+  } else {
+    for (int l = 1; l < levels; l++) {
+      std::vector<double> T = mul(mul(M2M, M, K0, K0, K1, false), L2L, K0, K1, K1, false);
+      for (size_t i = 0; i < M.size(); i++) M[i] = M2L[i] + T[i];
+    }
+  }
+
+  // far field of the analytic volume potential (include/fmm_pts.txx:1001-1033): the potential of
+  // a uniform density everywhere minus that of the 27 boxes around [1, 2]^3
+  if (vol) {
+    const std::vector<double> dc = surf(G, 1, 1, 1, 0, kRad0);
+    std::vector<double> gi((size_t)n * e0 * e1);
+    cube_integral(kind, dc.data(), n, 0.0, 3.0, gi.data());
+    std::vector<double> far = vol_poten(kind, dc.data(), n);  // e0 x K1
+    for (int j = 0; j < e0; j++)
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < e1; k++) far[(size_t)j * K1 + (size_t)i * e1 + k] -= gi[((size_t)i * e0 + k) * e1 + j];
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < e0; k++)
+        for (int c = 0; c < K1; c++) M[((size_t)i * e0 + k) * K1 + c] += far[(size_t)k * K1 + c];
+  }
+
+  // corner correction (include/fmm_pts.txx:1036-1138): fit a + bx + cy + dz + exy + fxz + gyz +
+  // hxyz to the potential at the 8 corners of [0, 1]^3 (local expansion plus the root's images
+  // next to each corner), and subtract the enabled terms on the check surface
+  {
+    const double corners[24] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 1, 1, 0, 0, 0, 1, 1, 0, 1, 0, 1, 1, 1, 1, 1};
+    const std::vector<double> cpts(corners, corners + 24);
+    const std::vector<double> ue = surf(G, 0, 0, 0, 0, kRad0), de = surf(G, 0, 0, 0, 0, kRad1);
+    const std::vector<double> dnc = surf(G, 0, 0, 0, 0, kRad0);
+    std::vector<double> cv = mul(mul(M, G.dc2de0, K0, K1, rd, dev), mul(G.dc2de1, K(de, cpts), rd, K0, 8 * e1, dev), K0, rd,
+                                 8 * e1, dev);  // K0 x 8 e1
+    for (int kk = 0; kk < 8; kk++)
+      for (int j0 = -1; j0 <= 1; j0++)
+        for (int j1 = -1; j1 <= 1; j1++)
+          for (int j2 = -1; j2 <= 1; j2++) {
+            if ((bndry == kPXY && j2 != 0) || (bndry == kPX && (j1 != 0 || j2 != 0))) continue;
+            const double pt[3] = {corners[3 * kk] - j0, corners[3 * kk + 1] - j1, corners[3 * kk + 2] - j2};
+            if (std::fabs(pt[0] - 0.5) < 1 && std::fabs(pt[1] - 0.5) < 1 && std::fabs(pt[2] - 0.5) < 1) continue;
+            const std::vector<double> E = K(ue, std::vector<double>(pt, pt + 3));  // K0 x e1
+            for (int r = 0; r < K0; r++)
+              for (int j = 0; j < e1; j++) cv[(size_t)r * 8 * e1 + kk * e1 + j] += E[(size_t)r * e1 + j];
+          }
+    if (vol) {
+      const std::vector<double> va = vol_poten(kind, corners, 8);  // e0 x 8 e1
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < e0; k++)
+          for (int c = 0; c < 8 * e1; c++) cv[((size_t)i * e0 + k) * 8 * e1 + c] -= va[(size_t)k * 8 * e1 + c];
+    }
+    auto basis = [](const double* p, double* b) {
+      b[0] = 1; b[1] = p[0]; b[2] = p[1]; b[3] = p[2];
+      b[4] = p[0] * p[1]; b[5] = p[0] * p[2]; b[6] = p[1] * p[2]; b[7] = p[0] * p[1] * p[2];
+    };
+    double V[64], Vinv[64];  // V[coeff][corner]
+    for (int kk = 0; kk < 8; kk++) {
+      double b[8];
+      basis(&corners[3 * kk], b);
+      for (int c = 0; c < 8; c++) V[c * 8 + kk] = b[c];
+    }
+    invert8(V, Vinv);  // Vinv[corner][coeff]
+    const int enable = bndry == kPXYZ ? 4 : bndry == kPXY ? 3 : 2;  // terms 1, x, y, z in use
+    std::vector<double> cvec((size_t)8 * n, 0.0);  // cvec[coeff][s]
+    for (int s = 0; s < n; s++) {
+      double b[8];
+      basis(&dnc[3 * s], b);
+      for (int c = 0; c < enable; c++) cvec[(size_t)c * n + s] = b[c];
+    }
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < K0; i++) {
+      double coeff[8][3] = {};
+      for (int c = 0; c < 8; c++)
+        for (int kk = 0; kk < 8; kk++)
+          for (int j = 0; j < e1; j++) coeff[c][j] += cv[(size_t)i * 8 * e1 + kk * e1 + j] * Vinv[kk * 8 + c];
+      for (int s = 0; s < n; s++)
+        for (int j = 0; j < e1; j++) {
+          double t = 0;
+          for (int c = 0; c < enable; c++) t += coeff[c][j] * cvec[(size_t)c * n + s];
+          M[(size_t)i * K1 + (size_t)s * e1 + j] -= t;
+        }
+    }
+  }
+
+  // order m + 2 -> m (include/fmm_pts.txx:1141-1160)
+  const int K0m = O.K0(), K1m = O.K1();
+  const std::vector<double> M2Md =
+      mul(mul(K(surf(O, -0.5, -0.5, -0.5, 0, kRad0), surf(G, -0.5, -0.5, -0.5, 0, kRad1)), G.uc2ue0, K0m, K1, ru, dev),
+          G.uc2ue1, K0m, ru, K0, dev);
+  const std::vector<double> L2Ld =
+      mul(G.dc2de0, mul(G.dc2de1, K(surf(G, -0.5, -0.5, -0.5, 0, kRad1), surf(O, -0.5, -0.5, -0.5, 0, kRad0)), rd, K0, K1m, dev),
+          K1, rd, K1m, dev);
+  O.bc[bndry] = mul(mul(M2Md, M, K0m, K0, K1, dev), L2Ld, K0m, K1, K1m, dev);
 }
 
 // ----------------------------------------------------------------------------- cache file

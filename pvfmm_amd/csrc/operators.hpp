@@ -36,6 +36,11 @@ struct Operators {
   std::vector<std::complex<double>> v1;
   std::vector<int32_t> v1_offsets;  // the 26 offsets R in use
 
+  // Periodic boundary operators by boundary type (1 PX, 2 PXY, 3 PXYZ; empty until
+  // build_bc_operator): root upward equivalent density -> root downward check potential of all
+  // periodic images beyond the root's colleagues, (ns*e0) x (ns*e1).
+  std::vector<double> bc[4];
+
   bool on_device = false;   // the factors and U2U/D2D came from the device precompute
   bool from_cache = false;  // every operator was read from a precompute file
 
@@ -54,6 +59,21 @@ void build_operators(Operators& O, int kernel_id, int m, double cutoff_rel = 0.0
 // written (atomically, through a temporary name), an existing one is never replaced.
 void build_operators_cached(Operators& O, int kernel_id, int m, double cutoff_rel, int precision, const char* dir);
 std::string precomp_file_name(const Operators& O, int precision);
+
+// The periodic boundary operator of boundary type bndry (1-3) into O.bc[bndry], unless it is
+// there already: FMM_Pts::Precomp(BC_Type) of the reference's default build (include/fmm_pts.txx:
+// 790-1163) -- the lattice sum over PVFMM_BC_LEVELS = 45 levels of images, each level the same
+// operator for a box twice as large, with the zero-charge and zero-mean projections, the far
+// field of the kernel's analytic volume potential (PXYZ; laplace potential and stokes velocity),
+// and the subtraction of the constant and linear terms fitted at the cube corners; computed at
+// order m + 2 and mapped to order m.  FP64; the products run on the GPU when the precompute does.
+void build_bc_operator(Operators& O, int bndry);
+
+// Integral over the cube [lo, lo + size]^3 of the expansion kernel (kind 0 laplace, 2 stokes)
+// from a uniform unit density to each of n points (the reference's cheb_integ of degree 0,
+// include/cheb_utils.txx:1037-1081): out[(p*e0 + a)*e1 + b] for source component a, target
+// component b.  Points may lie inside the cube.
+void cube_integral(int kind, const double* pts, int n, double lo, double size, double* out);
 
 // Expansion-kernel matrix K[src*e0+i][trg*e1+j] (Kernel::BuildMatrix layout,
 // include/kernel.txx:1059-1135), FP64.

@@ -64,6 +64,7 @@ def _declare(L):
         "pvfmm_amd_fmm_destroy": (None, [vp]),
         "pvfmm_amd_tree_create": (vp, [c_double_p, i64, c_double_p, i64, i]),
         "pvfmm_amd_tree_create_pts": (vp, [c_double_p, i64, c_double_p, i64, c_double_p, i64, i]),
+        "pvfmm_amd_tree_create_bndry": (vp, [c_double_p, i64, c_double_p, i64, c_double_p, i64, i, i]),
         "pvfmm_amd_tree_destroy": (None, [vp]),
         "pvfmm_amd_tree_setup": (i, [vp, vp, i]),
         "pvfmm_amd_tree_evaluate": (i, [vp, c_double_p, c_double_p, i]),

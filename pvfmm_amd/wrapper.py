@@ -176,26 +176,41 @@ class PtFMM:
             pass
 
     def operator(self, which: str) -> np.ndarray:
-        idx = {"uc2ue0": 0, "uc2ue1": 1, "dc2de0": 2, "dc2de1": 3, "u2u": 4, "d2d": 5, "v1": 6}[which]
+        idx = {"uc2ue0": 0, "uc2ue1": 1, "dc2de0": 2, "dc2de1": 3, "u2u": 4, "d2d": 5, "v1": 6,
+               "bc_px": 7, "bc_pxy": 8, "bc_pxyz": 9}[which]
         n = ffi.lib().pvfmm_amd_fmm_operator(self._h, idx, None)
+        if n < 0:
+            ffi.check()
         out = np.zeros(n)
         ffi.lib().pvfmm_amd_fmm_operator(self._h, idx, ffi.dptr(out))
         return out
+
+
+BOUNDARY = {"free": 0, "px": 1, "pxy": 2, "pxyz": 3, "periodic": 3}  # BoundaryType, include/mpi_tree.hpp:22-29
 
 
 class PtFMMTree:
     """PtFMM_CreateTree + SetupFMM + PtFMM_Evaluate (include/pvfmm.txx:215-328).
 
     Coordinates are (n,3) arrays in [0,1)^3; densities and values are in input order with k0
-    (k1) components per point.
+    (k1) components per point.  bndry: "free" or a periodic boundary type ("px", "pxy",
+    "pxyz" = "periodic"; unit period along the periodic axes).
     """
 
-    def __init__(self, src, trg=None, max_pts: int = 100, pt=None):
+    def __init__(self, src, trg=None, max_pts: int = 100, pt=None, bndry: str = "free"):
         src = np.ascontiguousarray(src, dtype=np.float64).reshape(-1, 3)
         trg = src if trg is None else np.ascontiguousarray(trg, dtype=np.float64).reshape(-1, 3)
         self.n_src, self.n_trg = len(src), len(trg)
+        if bndry not in BOUNDARY:
+            raise ValueError(f"bndry must be one of {sorted(BOUNDARY)}")
+        self.bndry = BOUNDARY[bndry]
         L = ffi.lib()
-        if pt is None:
+        if self.bndry:
+            ptv = None if pt is None else np.ascontiguousarray(pt, dtype=np.float64).reshape(-1, 3)
+            self._h = L.pvfmm_amd_tree_create_bndry(ffi.dptr(src), len(src), ffi.dptr(trg), len(trg),
+                                                    ffi.dptr(ptv) if ptv is not None else None,
+                                                    0 if ptv is None else len(ptv), int(max_pts), self.bndry)
+        elif pt is None:
             self._h = L.pvfmm_amd_tree_create(ffi.dptr(src), len(src), ffi.dptr(trg), len(trg), int(max_pts))
         else:
             pt = np.ascontiguousarray(pt, dtype=np.float64).reshape(-1, 3)

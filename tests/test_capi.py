@@ -35,9 +35,10 @@ def test_error_codes_without_abort():
     assert not L.PVFMMCreateContextD(-1.0, 100, 6, 2, 0)
     assert L.PVFMMGetLastError() == 2
     assert L.PVFMMErrorString(2) == b"unsupported feature"
-    # periodic boxes are not supported
-    assert not L.PVFMMCreateContextD(1.0, 100, 6, 0, 0)
-    assert L.PVFMMGetLastError() == 2
+    # a periodic box (box_size > 0) is a valid context
+    c = L.PVFMMCreateContextD(1.0, 100, 6, 0, 0)
+    assert c and L.PVFMMGetLastError() == 0
+    L.PVFMMDestroyContextD(ctypes.byref(ctypes.c_void_p(c)))
     # odd multipole order
     assert not L.pvfmm_amd_fmm_create(0, 5, 0, 0.0)
     assert L.PVFMMGetLastError() == 1
