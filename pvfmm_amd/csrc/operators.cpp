@@ -580,9 +580,12 @@ void cube_integral(int kind, const double* pts, int n, double lo, double size, d
   }
 }
 
-void build_bc_operator(Operators& O, int bndry) {
-  if (bndry < 1 || bndry > 3) throw Error(kErrInvalidArgument, "periodic boundary type must be 1 (PX), 2 (PXY) or 3 (PXYZ)");
-  if (!O.bc[bndry].empty()) return;
+// The boundary operators of every type in `types` (bits 1 << bndry), sharing the order m + 2
+// factors and the products common to all types.
+static void build_bc_operators(Operators& O, int types) {
+  for (int b = 1; b <= 3; b++)
+    if (!O.bc[b].empty()) types &= ~(1 << b);
+  if (!types) return;
   const int m = O.m, mo = m + 2, kind = O.ker.exp_kind, e0 = O.ker.e0, e1 = O.ker.e1;
   const bool dev = O.on_device;
   Operators G;  // order m + 2: surfaces and check -> equivalent factors of the unit box
@@ -602,11 +605,9 @@ void build_bc_operator(Operators& O, int bndry) {
     kernel_matrix(kind, a.data(), na, b.data(), nb, M.data());
     return M;
   };
-  auto keep = [&](int a, int b, int c) { return !((bndry == kPX && (b != 0 || c != 0)) || (bndry == kPXY && c != 0)); };
   auto axpy = [](std::vector<double>& y, const std::vector<double>& x, double a) {
     for (size_t i = 0; i < y.size(); i++) y[i] += a * x[i];
   };
-  const bool vol = bndry == kPXYZ && O.ker.id != 1;  // laplace_grad's expansion kernel has no volume potential
 
   // zero-average projections (include/fmm_pts.txx:862-913): Z removes the total equivalent
   // charge spread as the equivalent density of a uniformly charged unit cube, C the mean check
@@ -666,17 +667,6 @@ void build_bc_operator(Operators& O, int bndry) {
     }
   };
 
-  // M2M: the 8 unit boxes of [-1, 1]^3 -> that box (include/fmm_pts.txx:917-941)
-  std::vector<double> M2M;
-  {
-    const std::vector<double> chk = surf(G, -1, -1, -1, -1, kRad1);
-    std::vector<double> S((size_t)K0 * K1, 0.0);
-    for (int c = 0; c < 8; c++)
-      if (keep(c & 1, (c >> 1) & 1, (c >> 2) & 1))
-        axpy(S, K(surf(G, -1 + (c & 1), -1 + ((c >> 1) & 1), -1 + ((c >> 2) & 1), 0, kRad0), chk), 1.0);
-    M2M = mul(mul(S, G.uc2ue0, K0, K1, ru, dev), G.uc2ue1, K0, ru, K0, dev);
-    for (double& v : M2M) v *= s2;
-  }
   // L2L: [-1, 1]^3 -> its child [-1, 0]^3 (include/fmm_pts.txx:942-951)
   std::vector<double> L2L;
   {
@@ -684,125 +674,7 @@ void build_bc_operator(Operators& O, int bndry) {
     L2L = mul(G.dc2de0, mul(G.dc2de1, MM, rd, K0, K1, dev), K1, rd, K1, dev);
     for (double& v : L2L) v *= s2;
   }
-  // M2L: unit boxes at offsets in [-2, 3]^3 not adjacent to [0, 1]^3 (include/fmm_pts.txx:952-975)
-  std::vector<double> M2L((size_t)K0 * K1, 0.0);
-  {
-    const std::vector<double> dnc = surf(G, 0, 0, 0, 0, kRad0);
-    for (int c = -2; c < 4; c++)
-      for (int b = -2; b < 4; b++)
-        for (int a = -2; a < 4; a++)
-          if (keep(a, b, c) && std::max(std::abs(a), std::max(std::abs(b), std::abs(c))) > 1)
-            axpy(M2L, K(surf(G, a, b, c, 0, kRad0), dnc), 1.0);
-  }
-  z_left(M2M, K0); z_right(M2M, K0);
   c_left(L2L, K1); c_right(L2L, K1);
-  z_left(M2L, K1); c_right(M2L, K0);
-
-  // the levels: M <- M2L + M2M (2^-(src_scal + trg_scal) M) L2L, PVFMM_BC_LEVELS = 45 in all
-  const double down = 1.0 / s2;
-  for (double& v : M2M) v *= down;
-  std::vector<double> M = M2L;
-  const int levels = 45;
-  if (dev) {
-    DevStream st;
-    DevArray a, b, c, dm, dt, io;
-    std::vector<int32_t> iota(std::max(K0, K1));
-    for (size_t i = 0; i < iota.size(); i++) iota[i] = (int32_t)i;
-    const int32_t* I = io.upload(iota);
-    const double* d_m2m = a.upload(M2M);
-    const double* d_l2l = b.upload(L2L);
-    const double* d_m2l = c.upload(M2L);
-    double* d_M = dm.upload(M);
-    double* d_T = dt.alloc<double>((size_t)K0 * K1);
-    for (int l = 1; l < levels; l++) {
-      gemm_device(st.s, K0, K1, K0, d_m2m, d_M, d_T, K1, I);
-      HIP_CHECK(hipMemcpyAsync(d_M, d_m2l, sizeof(double) * M.size(), hipMemcpyDeviceToDevice, st.s));
-      DeviceKernels<double>::gemm_rows(st.s, K0, K1, K1, d_T, I, K1, d_l2l, d_M, I, K1, nullptr, true);
-    }
-    HIP_CHECK(hipMemcpyAsync(M.data(), d_M, sizeof(double) * M.size(), hipMemcpyDeviceToHost, st.s));
-    HIP_CHECK(hipStreamSynchronize(st.s));
-  } else {
-    for (int l = 1; l < levels; l++) {
-      std::vector<double> T = mul(mul(M2M, M, K0, K0, K1, false), L2L, K0, K1, K1, false);
-      for (size_t i = 0; i < M.size(); i++) M[i] = M2L[i] + T[i];
-    }
-  }
-
-  // far field of the analytic volume potential (include/fmm_pts.txx:1001-1033): the potential of
-  // a uniform density everywhere minus that of the 27 boxes around [1, 2]^3
-  if (vol) {
-    const std::vector<double> dc = surf(G, 1, 1, 1, 0, kRad0);
-    std::vector<double> gi((size_t)n * e0 * e1);
-    cube_integral(kind, dc.data(), n, 0.0, 3.0, gi.data());
-    std::vector<double> far = vol_poten(kind, dc.data(), n);  // e0 x K1
-    for (int j = 0; j < e0; j++)
-      for (int i = 0; i < n; i++)
-        for (int k = 0; k < e1; k++) far[(size_t)j * K1 + (size_t)i * e1 + k] -= gi[((size_t)i * e0 + k) * e1 + j];
-    for (int i = 0; i < n; i++)
-      for (int k = 0; k < e0; k++)
-        for (int c = 0; c < K1; c++) M[((size_t)i * e0 + k) * K1 + c] += far[(size_t)k * K1 + c];
-  }
-
-  // corner correction (include/fmm_pts.txx:1036-1138): fit a + bx + cy + dz + exy + fxz + gyz +
-  // hxyz to the potential at the 8 corners of [0, 1]^3 (local expansion plus the root's images
-  // next to each corner), and subtract the enabled terms on the check surface
-  {
-    const double corners[24] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 1, 1, 0, 0, 0, 1, 1, 0, 1, 0, 1, 1, 1, 1, 1};
-    const std::vector<double> cpts(corners, corners + 24);
-    const std::vector<double> ue = surf(G, 0, 0, 0, 0, kRad0), de = surf(G, 0, 0, 0, 0, kRad1);
-    const std::vector<double> dnc = surf(G, 0, 0, 0, 0, kRad0);
-    std::vector<double> cv = mul(mul(M, G.dc2de0, K0, K1, rd, dev), mul(G.dc2de1, K(de, cpts), rd, K0, 8 * e1, dev), K0, rd,
-                                 8 * e1, dev);  // K0 x 8 e1
-    for (int kk = 0; kk < 8; kk++)
-      for (int j0 = -1; j0 <= 1; j0++)
-        for (int j1 = -1; j1 <= 1; j1++)
-          for (int j2 = -1; j2 <= 1; j2++) {
-            if ((bndry == kPXY && j2 != 0) || (bndry == kPX && (j1 != 0 || j2 != 0))) continue;
-            const double pt[3] = {corners[3 * kk] - j0, corners[3 * kk + 1] - j1, corners[3 * kk + 2] - j2};
-            if (std::fabs(pt[0] - 0.5) < 1 && std::fabs(pt[1] - 0.5) < 1 && std::fabs(pt[2] - 0.5) < 1) continue;
-            const std::vector<double> E = K(ue, std::vector<double>(pt, pt + 3));  // K0 x e1
-            for (int r = 0; r < K0; r++)
-              for (int j = 0; j < e1; j++) cv[(size_t)r * 8 * e1 + kk * e1 + j] += E[(size_t)r * e1 + j];
-          }
-    if (vol) {
-      const std::vector<double> va = vol_poten(kind, corners, 8);  // e0 x 8 e1
-      for (int i = 0; i < n; i++)
-        for (int k = 0; k < e0; k++)
-          for (int c = 0; c < 8 * e1; c++) cv[((size_t)i * e0 + k) * 8 * e1 + c] -= va[(size_t)k * 8 * e1 + c];
-    }
-    auto basis = [](const double* p, double* b) {
-      b[0] = 1; b[1] = p[0]; b[2] = p[1]; b[3] = p[2];
-      b[4] = p[0] * p[1]; b[5] = p[0] * p[2]; b[6] = p[1] * p[2]; b[7] = p[0] * p[1] * p[2];
-    };
-    double V[64], Vinv[64];  // V[coeff][corner]
-    for (int kk = 0; kk < 8; kk++) {
-      double b[8];
-      basis(&corners[3 * kk], b);
-      for (int c = 0; c < 8; c++) V[c * 8 + kk] = b[c];
-    }
-    invert8(V, Vinv);  // Vinv[corner][coeff]
-    const int enable = bndry == kPXYZ ? 4 : bndry == kPXY ? 3 : 2;  // terms 1, x, y, z in use
-    std::vector<double> cvec((size_t)8 * n, 0.0);  // cvec[coeff][s]
-    for (int s = 0; s < n; s++) {
-      double b[8];
-      basis(&dnc[3 * s], b);
-      for (int c = 0; c < enable; c++) cvec[(size_t)c * n + s] = b[c];
-    }
-    #pragma omp parallel for schedule(static)
-    for (int i = 0; i < K0; i++) {
-      double coeff[8][3] = {};
-      for (int c = 0; c < 8; c++)
-        for (int kk = 0; kk < 8; kk++)
-          for (int j = 0; j < e1; j++) coeff[c][j] += cv[(size_t)i * 8 * e1 + kk * e1 + j] * Vinv[kk * 8 + c];
-      for (int s = 0; s < n; s++)
-        for (int j = 0; j < e1; j++) {
-          double t = 0;
-          for (int c = 0; c < enable; c++) t += coeff[c][j] * cvec[(size_t)c * n + s];
-          M[(size_t)i * K1 + (size_t)s * e1 + j] -= t;
-        }
-    }
-  }
-
   // order m + 2 -> m (include/fmm_pts.txx:1141-1160)
   const int K0m = O.K0(), K1m = O.K1();
   const std::vector<double> M2Md =
@@ -811,7 +683,147 @@ void build_bc_operator(Operators& O, int bndry) {
   const std::vector<double> L2Ld =
       mul(G.dc2de0, mul(G.dc2de1, K(surf(G, -0.5, -0.5, -0.5, 0, kRad1), surf(O, -0.5, -0.5, -0.5, 0, kRad0)), rd, K0, K1m, dev),
           K1, rd, K1m, dev);
-  O.bc[bndry] = mul(mul(M2Md, M, K0m, K0, K1, dev), L2Ld, K0m, K1, K1m, dev);
+
+  for (int bndry = 1; bndry <= 3; bndry++) {
+    if (!(types & (1 << bndry))) continue;
+    auto keep = [&](int a, int b, int c) { return !((bndry == kPX && (b != 0 || c != 0)) || (bndry == kPXY && c != 0)); };
+    const bool vol = bndry == kPXYZ && O.ker.id != 1;  // laplace_grad's expansion kernel has no volume potential
+    // M2M: the 8 unit boxes of [-1, 1]^3 -> that box (include/fmm_pts.txx:917-941)
+    std::vector<double> M2M;
+    {
+      const std::vector<double> chk = surf(G, -1, -1, -1, -1, kRad1);
+      std::vector<double> S((size_t)K0 * K1, 0.0);
+      for (int c = 0; c < 8; c++)
+        if (keep(c & 1, (c >> 1) & 1, (c >> 2) & 1))
+          axpy(S, K(surf(G, -1 + (c & 1), -1 + ((c >> 1) & 1), -1 + ((c >> 2) & 1), 0, kRad0), chk), 1.0);
+      M2M = mul(mul(S, G.uc2ue0, K0, K1, ru, dev), G.uc2ue1, K0, ru, K0, dev);
+      for (double& v : M2M) v *= s2;
+    }
+    // M2L: unit boxes at offsets in [-2, 3]^3 not adjacent to [0, 1]^3 (include/fmm_pts.txx:952-975)
+    std::vector<double> M2L((size_t)K0 * K1, 0.0);
+    {
+      const std::vector<double> dnc = surf(G, 0, 0, 0, 0, kRad0);
+      for (int c = -2; c < 4; c++)
+        for (int b = -2; b < 4; b++)
+          for (int a = -2; a < 4; a++)
+            if (keep(a, b, c) && std::max(std::abs(a), std::max(std::abs(b), std::abs(c))) > 1)
+              axpy(M2L, K(surf(G, a, b, c, 0, kRad0), dnc), 1.0);
+    }
+    z_left(M2M, K0); z_right(M2M, K0);
+    z_left(M2L, K1); c_right(M2L, K0);
+
+    // the levels: M <- M2L + M2M (2^-(src_scal + trg_scal) M) L2L, PVFMM_BC_LEVELS = 45 in all
+    const double down = 1.0 / s2;
+    for (double& v : M2M) v *= down;
+    std::vector<double> M = M2L;
+    const int levels = 45;
+    if (dev) {
+      DevStream st;
+      DevArray a, b, c, dm, dt, io;
+      std::vector<int32_t> iota(std::max(K0, K1));
+      for (size_t i = 0; i < iota.size(); i++) iota[i] = (int32_t)i;
+      const int32_t* I = io.upload(iota);
+      const double* d_m2m = a.upload(M2M);
+      const double* d_l2l = b.upload(L2L);
+      const double* d_m2l = c.upload(M2L);
+      double* d_M = dm.upload(M);
+      double* d_T = dt.alloc<double>((size_t)K0 * K1);
+      for (int l = 1; l < levels; l++) {
+        gemm_device(st.s, K0, K1, K0, d_m2m, d_M, d_T, K1, I);
+        HIP_CHECK(hipMemcpyAsync(d_M, d_m2l, sizeof(double) * M.size(), hipMemcpyDeviceToDevice, st.s));
+        DeviceKernels<double>::gemm_rows(st.s, K0, K1, K1, d_T, I, K1, d_l2l, d_M, I, K1, nullptr, true);
+      }
+      HIP_CHECK(hipMemcpyAsync(M.data(), d_M, sizeof(double) * M.size(), hipMemcpyDeviceToHost, st.s));
+      HIP_CHECK(hipStreamSynchronize(st.s));
+    } else {
+      for (int l = 1; l < levels; l++) {
+        std::vector<double> T = mul(mul(M2M, M, K0, K0, K1, false), L2L, K0, K1, K1, false);
+        for (size_t i = 0; i < M.size(); i++) M[i] = M2L[i] + T[i];
+      }
+    }
+
+    // far field of the analytic volume potential (include/fmm_pts.txx:1001-1033): the potential of
+    // a uniform density everywhere minus that of the 27 boxes around [1, 2]^3
+    if (vol) {
+      const std::vector<double> dc = surf(G, 1, 1, 1, 0, kRad0);
+      std::vector<double> gi((size_t)n * e0 * e1);
+      cube_integral(kind, dc.data(), n, 0.0, 3.0, gi.data());
+      std::vector<double> far = vol_poten(kind, dc.data(), n);  // e0 x K1
+      for (int j = 0; j < e0; j++)
+        for (int i = 0; i < n; i++)
+          for (int k = 0; k < e1; k++) far[(size_t)j * K1 + (size_t)i * e1 + k] -= gi[((size_t)i * e0 + k) * e1 + j];
+      for (int i = 0; i < n; i++)
+        for (int k = 0; k < e0; k++)
+          for (int c = 0; c < K1; c++) M[((size_t)i * e0 + k) * K1 + c] += far[(size_t)k * K1 + c];
+    }
+
+    // corner correction (include/fmm_pts.txx:1036-1138): fit a + bx + cy + dz + exy + fxz + gyz +
+    // hxyz to the potential at the 8 corners of [0, 1]^3 (local expansion plus the root's images
+    // next to each corner), and subtract the enabled terms on the check surface
+    {
+      const double corners[24] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 1, 1, 0, 0, 0, 1, 1, 0, 1, 0, 1, 1, 1, 1, 1};
+      const std::vector<double> cpts(corners, corners + 24);
+      const std::vector<double> ue = surf(G, 0, 0, 0, 0, kRad0), de = surf(G, 0, 0, 0, 0, kRad1);
+      const std::vector<double> dnc = surf(G, 0, 0, 0, 0, kRad0);
+      std::vector<double> cv = mul(mul(M, G.dc2de0, K0, K1, rd, dev), mul(G.dc2de1, K(de, cpts), rd, K0, 8 * e1, dev), K0, rd,
+                                   8 * e1, dev);  // K0 x 8 e1
+      for (int kk = 0; kk < 8; kk++)
+        for (int j0 = -1; j0 <= 1; j0++)
+          for (int j1 = -1; j1 <= 1; j1++)
+            for (int j2 = -1; j2 <= 1; j2++) {
+              if ((bndry == kPXY && j2 != 0) || (bndry == kPX && (j1 != 0 || j2 != 0))) continue;
+              const double pt[3] = {corners[3 * kk] - j0, corners[3 * kk + 1] - j1, corners[3 * kk + 2] - j2};
+              if (std::fabs(pt[0] - 0.5) < 1 && std::fabs(pt[1] - 0.5) < 1 && std::fabs(pt[2] - 0.5) < 1) continue;
+              const std::vector<double> E = K(ue, std::vector<double>(pt, pt + 3));  // K0 x e1
+              for (int r = 0; r < K0; r++)
+                for (int j = 0; j < e1; j++) cv[(size_t)r * 8 * e1 + kk * e1 + j] += E[(size_t)r * e1 + j];
+            }
+      if (vol) {
+        const std::vector<double> va = vol_poten(kind, corners, 8);  // e0 x 8 e1
+        for (int i = 0; i < n; i++)
+          for (int k = 0; k < e0; k++)
+            for (int c = 0; c < 8 * e1; c++) cv[((size_t)i * e0 + k) * 8 * e1 + c] -= va[(size_t)k * 8 * e1 + c];
+      }
+      auto basis = [](const double* p, double* b) {
+        b[0] = 1; b[1] = p[0]; b[2] = p[1]; b[3] = p[2];
+        b[4] = p[0] * p[1]; b[5] = p[0] * p[2]; b[6] = p[1] * p[2]; b[7] = p[0] * p[1] * p[2];
+      };
+      double V[64], Vinv[64];  // V[coeff][corner]
+      for (int kk = 0; kk < 8; kk++) {
+        double b[8];
+        basis(&corners[3 * kk], b);
+        for (int c = 0; c < 8; c++) V[c * 8 + kk] = b[c];
+      }
+      invert8(V, Vinv);  // Vinv[corner][coeff]
+      const int enable = bndry == kPXYZ ? 4 : bndry == kPXY ? 3 : 2;  // terms 1, x, y, z in use
+      std::vector<double> cvec((size_t)8 * n, 0.0);  // cvec[coeff][s]
+      for (int s = 0; s < n; s++) {
+        double b[8];
+        basis(&dnc[3 * s], b);
+        for (int c = 0; c < enable; c++) cvec[(size_t)c * n + s] = b[c];
+      }
+      #pragma omp parallel for schedule(static)
+      for (int i = 0; i < K0; i++) {
+        double coeff[8][3] = {};
+        for (int c = 0; c < 8; c++)
+          for (int kk = 0; kk < 8; kk++)
+            for (int j = 0; j < e1; j++) coeff[c][j] += cv[(size_t)i * 8 * e1 + kk * e1 + j] * Vinv[kk * 8 + c];
+        for (int s = 0; s < n; s++)
+          for (int j = 0; j < e1; j++) {
+            double t = 0;
+            for (int c = 0; c < enable; c++) t += coeff[c][j] * cvec[(size_t)c * n + s];
+            M[(size_t)i * K1 + (size_t)s * e1 + j] -= t;
+          }
+      }
+    }
+
+    O.bc[bndry] = mul(mul(M2Md, M, K0m, K0, K1, dev), L2Ld, K0m, K1, K1m, dev);
+  }
+}
+
+void build_bc_operator(Operators& O, int bndry) {
+  if (bndry < 1 || bndry > 3) throw Error(kErrInvalidArgument, "periodic boundary type must be 1 (PX), 2 (PXY) or 3 (PXYZ)");
+  build_bc_operators(O, 1 << bndry);
 }
 
 // ----------------------------------------------------------------------------- cache file
@@ -819,7 +831,7 @@ void build_bc_operator(Operators& O, int bndry) {
 // sizeof(T), int scale_invar, then for each of the 16 Mat_Type values (precomp_mat.hpp:21-39)
 // int n and n matrices, each int rows, int cols and rows*cols values of T.
 namespace {
-enum : int { kUC2UE0 = 0, kUC2UE1 = 1, kDC2DE0 = 2, kDC2DE1 = 3, kU2UT = 5, kD2DT = 6, kV1T = 14, kMatTypes = 16 };
+enum : int { kUC2UE0 = 0, kUC2UE1 = 1, kDC2DE0 = 2, kDC2DE1 = 3, kU2UT = 5, kD2DT = 6, kV1T = 14, kBCT = 15, kMatTypes = 16 };
 struct Mat {
   int rows = 0, cols = 0;
   std::vector<double> v;
@@ -933,8 +945,18 @@ void build_operators_cached(Operators& O, int kernel_id, int m, double cutoff_re
   } else {
     compute_v1(O);
   }
+  // boundary operators, one per BoundaryType (FreeSpace stored as zeros, include/fmm_pts.txx:
+  // 288-300): read where present; a new file gets all of them, as the reference computes them
+  // in Initialize
+  bool have_bc = exists && have_f && mats[kBCT].size() == 4;
+  for (int b = 0; have_bc && b < 4; b++) have_bc = mats[kBCT][b].rows == K0 && mats[kBCT][b].cols == K1;
+  if (have_bc)
+    for (int b = 1; b <= 3; b++) O.bc[b] = mats[kBCT][b].v;
   O.from_cache = have_f && have_t && have_v;
   if (exists) return;  // an existing file is never replaced (PrecompMat::Save2File with replace = false)
+  build_bc_operators(O, (1 << kPX) | (1 << kPXY) | (1 << kPXYZ));
+  std::vector<Mat> bcm{Mat{K0, K1, std::vector<double>((size_t)K0 * K1, 0.0)}};
+  for (int b = 1; b <= 3; b++) bcm.push_back(Mat{K0, K1, O.bc[b]});
   std::vector<std::vector<Mat>> out(kMatTypes);
   out[kUC2UE0] = {Mat{K1, O.rank_u, O.uc2ue0}};
   out[kUC2UE1] = {Mat{O.rank_u, K0, O.uc2ue1}};
@@ -953,6 +975,7 @@ void build_operators_cached(Operators& O, int kernel_id, int m, double cutoff_re
     const double* b = reinterpret_cast<const double*>(&O.v1[(size_t)R * M_dim * bw * bh]);
     out[kV1T].push_back(Mat{M_dim, 2 * bw * bh, std::vector<double>(b, b + (size_t)2 * M_dim * bw * bh)});
   }
+  out[kBCT] = std::move(bcm);
   write_container(path, out);
 }
 

@@ -60,11 +60,15 @@ def test_precomp_file_round_trip(cache_dir, kernel, name, prec, suffix):
     assert len(mats[5]) == 8 and all(a.shape == (K, K) for a in mats[5])
     assert len(mats[6]) == 8 and all(a.shape == (K, K) for a in mats[6])
     assert len(mats[14]) == 27 and mats[14][13].size == 0
-    assert all(len(mats[t]) == 0 for t in (4, 7, 8, 9, 10, 11, 12, 13, 15))
+    assert all(len(mats[t]) == 0 for t in (4, 7, 8, 9, 10, 11, 12, 13))
+    # BC_Type: one periodic boundary operator per BoundaryType, FreeSpace as zeros
+    assert len(mats[15]) == 4 and all(a.shape == (K, K) for a in mats[15]) and not mats[15][0].any()
     np.testing.assert_array_equal(mats[0][0].ravel(), f1.operator("uc2ue0"))
+    for b, name_ in ((1, "bc_px"), (2, "bc_pxy"), (3, "bc_pxyz")):
+        np.testing.assert_array_equal(mats[15][b].ravel(), f1.operator(name_))
     mtime = path.stat().st_mtime_ns
     f2 = P.PtFMM(kernel, m, prec)  # read back
-    for op in OPS:
+    for op in OPS + ("bc_px", "bc_pxy", "bc_pxyz"):
         np.testing.assert_array_equal(f1.operator(op), f2.operator(op))
     assert path.stat().st_mtime_ns == mtime  # an existing file is not rewritten
     # the same evaluation from computed and from read operators
