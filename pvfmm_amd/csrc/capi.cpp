@@ -93,9 +93,8 @@ TreeHandle* make_tree(const double* src, int64_t n_src, const double* trg, int64
   h->trg.assign(trg, trg + 3 * n_trg);
   // the tree-defining set of PtFMM_CreateTree is the target set itself (include/pvfmm.txx:239)
   const double* pts = (pt == trg && n_pt == n_trg) ? h->trg.data() : pt;
-  // periodic trees are built on the host (the device builder covers free space)
-  if (bndry == kFreeSpace && device_tree_enabled())
-    build_octree_device(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pts, n_pt, max_pts);
+  if (device_tree_enabled())
+    build_octree_device(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pts, n_pt, max_pts, bndry);
   else
     build_octree(h->T, h->src.data(), n_src, h->trg.data(), n_trg, pts, n_pt, max_pts, bndry);
   return h.release();

@@ -187,16 +187,19 @@ __global__ void k_gather_flagged(int n, const u128* __restrict__ in, const int32
 }
 
 // the depth-(l-1) parents of the 27 same-level neighbours (self included) of each node of S_l;
-// out of the unit cube: kNoKey
-__global__ void k_neighbour_parents(int n, const u128* __restrict__ S, int l, u128* __restrict__ out) {
+// out of the unit cube: kNoKey (a periodic tree wraps every axis, as the reference's balance does)
+__global__ void k_neighbour_parents(int n, const u128* __restrict__ S, int l, int periodic, u128* __restrict__ out) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= (int64_t)n * 27) return;
   const int i = (int)(e / 27), c = (int)(e % 27);
   uint32_t x, y, z;
   unmorton(S[i], x, y, z);
   const int64_t step = (int64_t)1 << (kMaxDepth - l), maxc = (int64_t)1 << kMaxDepth;
-  const int64_t nx = (int64_t)x + (c % 3 - 1) * step, ny = (int64_t)y + ((c / 3) % 3 - 1) * step,
-                nz = (int64_t)z + (c / 9 - 1) * step;
+  int64_t nx = (int64_t)x + (c % 3 - 1) * step, ny = (int64_t)y + ((c / 3) % 3 - 1) * step,
+          nz = (int64_t)z + (c / 9 - 1) * step;
+  if (periodic) {
+    nx = (nx + maxc) % maxc; ny = (ny + maxc) % maxc; nz = (nz + maxc) % maxc;
+  }
   if (nx < 0 || ny < 0 || nz < 0 || nx >= maxc || ny >= maxc || nz >= maxc) {
     out[e] = kNoKey;
     return;
@@ -239,9 +242,10 @@ struct LevelTable {
   int n_levels;
 };
 
+// colleagues wrap along the periodic axes of bndry (1: x; 2: x, y; 3: all)
 __global__ void k_node_data(int N, LevelTable L, const uint8_t* __restrict__ depth, const u128* __restrict__ sk, int n_s,
-                            const u128* __restrict__ tk, int n_t, uint32_t* __restrict__ xyz, int32_t* __restrict__ rng,
-                            int32_t* __restrict__ coll) {
+                            const u128* __restrict__ tk, int n_t, int bndry, uint32_t* __restrict__ xyz,
+                            int32_t* __restrict__ rng, int32_t* __restrict__ coll) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int d = depth[i];
@@ -258,8 +262,11 @@ __global__ void k_node_data(int N, LevelTable L, const uint8_t* __restrict__ dep
   const int64_t step = (int64_t)1 << (kMaxDepth - d), maxc = (int64_t)1 << kMaxDepth;
   const int a = L.lb[d], nl = L.lb[d + 1] - a;
   for (int c = 0; c < 27; c++) {
-    const int64_t nx = (int64_t)x + (c % 3 - 1) * step, ny = (int64_t)y + ((c / 3) % 3 - 1) * step,
-                  nz = (int64_t)z + (c / 9 - 1) * step;
+    int64_t nx = (int64_t)x + (c % 3 - 1) * step, ny = (int64_t)y + ((c / 3) % 3 - 1) * step,
+            nz = (int64_t)z + (c / 9 - 1) * step;
+    if (bndry >= 1) nx = (nx + maxc) % maxc;
+    if (bndry >= 2) ny = (ny + maxc) % maxc;
+    if (bndry >= 3) nz = (nz + maxc) % maxc;
     int v = -1;
     if (nx >= 0 && ny >= 0 && nz >= 0 && nx < maxc && ny < maxc && nz < maxc) {
       const int f = find(L.keys + a, nl, morton((uint32_t)nx, (uint32_t)ny, (uint32_t)nz));
@@ -432,8 +439,9 @@ bool device_tree_enabled() {
 }
 
 void build_octree_device(Octree& T, const double* src, int64_t n_src, const double* trg, int64_t n_trg, const double* pt,
-                         int64_t n_pt, int max_pts) {
+                         int64_t n_pt, int max_pts, int bndry) {
   if (max_pts <= 0) throw Error(kErrInvalidArgument, "max_pts must be positive");
+  if (bndry < kFreeSpace || bndry > kPXYZ) throw Error(kErrInvalidArgument, "boundary type must be 0 (free), 1 (PX), 2 (PXY) or 3 (PXYZ)");
   if (n_src >= (1ll << 31) || n_trg >= (1ll << 31) || n_pt >= (1ll << 31))
     throw Error(kErrUnsupported, "more than 2^31 points");
   for (const auto& set : {std::make_pair(src, n_src), std::make_pair(trg, n_trg), std::make_pair(pt, n_pt)})
@@ -441,6 +449,7 @@ void build_octree_device(Octree& T, const double* src, int64_t n_src, const doub
       if (!(set.first[i] >= 0.0 && set.first[i] < 1.0)) throw Error(kErrInvalidArgument, "point coordinates must lie in [0,1)");
   T = Octree();
   T.max_pts = max_pts;
+  T.bndry = bndry;
   hipStream_t s = nullptr;
   HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   std::unique_ptr<std::remove_pointer<hipStream_t>::type, hipError_t (*)(hipStream_t)> guard(s, &hipStreamDestroy);
@@ -516,7 +525,8 @@ void build_octree_device(Octree& T, const double* src, int64_t n_src, const doub
     const int n = n_S[l], old = n_S[l - 1];
     DVec<u128> cand;
     cand.resize((size_t)27 * n + old);
-    hipLaunchKernelGGL(k_neighbour_parents, dim3(blocks_for((int64_t)27 * n)), dim3(256), 0, s, n, S[l].p, l, cand.p);
+    hipLaunchKernelGGL(k_neighbour_parents, dim3(blocks_for((int64_t)27 * n)), dim3(256), 0, s, n, S[l].p, l,
+                       bndry != kFreeSpace ? 1 : 0, cand.p);
     OCT_CHECK();
     if (old > 0) HIP_CHECK(hipMemcpyAsync(cand.p + (size_t)27 * n, S[l - 1].p, sizeof(u128) * old, hipMemcpyDeviceToDevice, s));
     n_S[l - 1] = sort_unique(s, tmp, cand, 27 * n + old, S[l - 1]);
@@ -585,7 +595,7 @@ void build_octree_device(Octree& T, const double* src, int64_t n_src, const doub
   dcoll.resize((size_t)27 * N);
   hipLaunchKernelGGL(k_node_data, dim3(blocks_for(N)), dim3(256), 0, s, N,
                      LevelTable{dkeys.p, dlb.p, (int)T.level_begin.size() - 1}, ddep.p, S0.keys.p, S0.n, S1.keys.p, S1.n,
-                     dxyz.p, drng.p, dcoll.p);
+                     bndry, dxyz.p, drng.p, dcoll.p);
   OCT_CHECK();
 
   // 6. interaction lists

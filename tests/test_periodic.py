@@ -174,6 +174,26 @@ def test_periodic_rejects_bad_types():
 
 
 # ----------------------------------------------------------------------------- product, GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize("bndry", BTYPES)
+@pytest.mark.parametrize("kind", ["uniform", "cubed", "dup"])
+def test_periodic_device_tree_matches_host(gpu, monkeypatch, bndry, kind):
+    """The GPU tree builder wraps the balance and the colleagues as the host builder does:
+    nodes, point orders and every list identical, for each periodic boundary type."""
+    rng = np.random.default_rng(hash((bndry, kind, 1)) % (1 << 32))
+    x = _points(kind, 4000, rng)
+    trg = _points("uniform", 1500, rng)
+    monkeypatch.delenv("PVFMM_TREE", raising=False)
+    td = P.PtFMMTree(x, trg, 16, bndry=bndry)
+    monkeypatch.setenv("PVFMM_TREE", "host")
+    th = P.PtFMMTree(x, trg, 16, bndry=bndry)
+    assert np.array_equal(td.nodes(), th.nodes())
+    for which in (0, 1):
+        assert np.array_equal(td.point_order(which), th.point_order(which))
+    for t in LIST_TYPES:
+        assert np.array_equal(td.lists(t), th.lists(t)), f"list type {t} differs"
+
+
 CASES = [  # kernel, precision, m, bndry, fp64 bar vs oracle, mixed bar
     ("laplace_poten", "fp64", 8, "pxyz"),
     ("laplace_poten", "mixed", 8, "pxyz"),
@@ -213,6 +233,29 @@ def test_periodic_gpu_matches_oracle(gpu, kernel, prec, m, bndry):
 def t_bc_shape(kernel, m):
     n = (3 if kernel == "stokes_vel" else 1) * (6 * (m - 1) ** 2 + 2)
     return (n, n)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["root-leaf", "separate-targets", "stokes-targets"])
+def test_periodic_gpu_edge_cases(gpu, case):
+    """A tree that is only its root (every U-list partner is the root or one of its 26
+    images), and target sets apart from the sources, against the oracle."""
+    rng = np.random.default_rng(12)
+    kernel, m = ("stokes_vel", 6) if case.startswith("stokes") else ("laplace_poten", 8)
+    if case == "root-leaf":
+        x, trg, max_pts = rng.random((40, 3)), None, 64
+    else:
+        x, trg, max_pts = rng.random((3000, 3)), rng.random((1200, 3)), 32
+    k0 = 3 if kernel == "stokes_vel" else 1
+    q = rng.standard_normal((len(x), k0))
+    t = P.PtFMMTree(x, trg, max_pts, bndry="pxyz")
+    f = P.PtFMM(kernel, m)
+    t.setup(f)
+    u = t.evaluate(q.ravel())
+    uo = _oracle_eval(kernel, x, q, m, max_pts, "pxyz", trg=trg, bc=f.operator("bc_pxyz").reshape(t_bc_shape(kernel, m)))
+    e = _rel(u, uo)
+    print(f"periodic {case}: rel-L2 vs oracle {e:.3e}", flush=True)
+    assert e < (1e-10 if kernel == "stokes_vel" else 1e-11)
 
 
 @pytest.mark.gpu
