@@ -215,7 +215,8 @@ def bench_workload(name: str, steps: int, warmup: int, check: bool):
     """Secondary workloads (one JSON line each): uniform-poten-64k (laplace_poten FP64, 65,536
     uniform points); ellipsoid-stokes-1m (stokes_vel on an ellipsoid, mixed and FP64);
     densities-32 (32 density vectors on one tree, mixed); clouds-64 (64 independent clouds as one
-    graph launch, mixed); energy-1m (energy forward/backward)."""
+    graph launch, mixed); periodic-poten-1m (laplace_poten in the periodic unit cube, mixed and
+    FP64); energy-1m (energy forward/backward)."""
     import torch
 
     import pvfmm_amd as P
@@ -324,6 +325,41 @@ def bench_workload(name: str, steps: int, warmup: int, check: bool):
                     autotune_s=t_tune, autotuned_shapes=shapes, graph_records=batch.records(),
                     config={"workload": f"{name}: 64 uniform clouds, {ntot} points, m={c['m']}, max_pts={c['max_pts']}, "
                                         "mixed, autotuned launch shapes, one HIP graph launch"})
+    elif name == "periodic-poten-1m":
+        # periodic boundaries: both evaluators at m = 8; the error is against the FP64 evaluator
+        # at m = 10 on the same tree, with the mean difference removed (a periodic potential's
+        # constant is convention); the Ewald-sum checks of the periodic path are in the tests
+        c = W.uniform_periodic_1m()
+        n = len(c["src"])
+        tree = P.PtFMMTree(c["src"], c["trg"], c["max_pts"], bndry=c["bndry"])
+        dq = torch.from_numpy(c["den"]).to("cuda:0")
+        t0 = time.perf_counter()
+        tree.setup(P.PtFMM(c["kernel"], 10, "fp64"))
+        t_setup10 = time.perf_counter() - t0
+        uref = tree.evaluate_device(dq).cpu().numpy()
+        res = {}
+        for prec in ("mixed", "fp64"):
+            t0 = time.perf_counter()
+            tree.setup(P.PtFMM(c["kernel"], c["m"], prec))
+            t_setup = time.perf_counter() - t0
+            shapes = tree.autotune(dq)
+            t = timed(lambda: tree.evaluate_device(dq))
+            u = tree.evaluate_device(dq).cpu().numpy()
+            d = u - uref
+            e = float(np.linalg.norm(d - d.mean()) / np.linalg.norm(uref - uref.mean()))
+            tree.set_timing(True)
+            tree.evaluate_device(dq)
+            phases = {k: round(v, 3) for k, v in tree.phase_times().items()}
+            tree.set_timing(False)
+            res[prec] = {"ms_per_step": 1e3 * t, "mtargets_s": n / t / 1e6, "rel_l2_vs_fp64_m10": e,
+                         "setup_s": t_setup, "launch_shapes": shapes, "phases_ms": phases}
+            log(f"{name} {prec}: {1e3 * t:.2f} ms ({n / t / 1e6:.1f} Mtargets/s), rel-L2 vs FP64 m=10 {e:.3e}, "
+                f"setup {t_setup:.2f} s")
+            log("   phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
+        line.update(metric="laplace_poten periodic FMM Mtargets/s", value=res["mixed"]["mtargets_s"], unit="Mtargets/s",
+                    ms_per_step=res["mixed"]["ms_per_step"], dtype="f32", results=res, setup_fp64_m10_s=t_setup10,
+                    config={"workload": f"{name}: laplace_poten, periodic (PXYZ), N={n} uniform, m={c['m']}, "
+                                        f"max_pts={c['max_pts']}"})
     elif name == "energy-1m":
         bf = bwd_fwd(steps, warmup, 1048576)
         line.update(metric="laplace energy t_bwd / t_fwd", value=bf["ratio"], unit="ratio", higher_is_better=False,
@@ -350,7 +386,8 @@ def main():
     ap.add_argument("--no-bwd", action="store_true", help="skip the energy forward/backward timing")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--workload", default="adaptive-grad-2m", choices=["adaptive-grad-2m"] + sorted(
-        k for k in ("uniform-poten-64k", "ellipsoid-stokes-1m", "densities-32", "clouds-64", "energy-1m")),
+        k for k in ("uniform-poten-64k", "ellipsoid-stokes-1m", "densities-32", "clouds-64", "periodic-poten-1m",
+                    "energy-1m")),
                     help="adaptive-grad-2m (headline, default) or a secondary workload")
     args = ap.parse_args()
     if args.gpus != 1:

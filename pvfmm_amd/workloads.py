@@ -104,11 +104,21 @@ def uniform_energy_1m(n=1048576):
     return dict(kernel="laplace_poten", m=10, max_pts=256, src=x, trg=x, den=q)
 
 
+def uniform_periodic_1m(n=1048576):
+    """laplace_poten in the periodic unit cube (PXYZ), N=2^20 uniform, q ~ N(0,1) (the net charge
+    is neutralised by a uniform background); m=8, max_pts=128."""
+    rng = rng_for(7)
+    x = uniform(rng, n)
+    q = rng.standard_normal(n)
+    return dict(kernel="laplace_poten", m=8, max_pts=128, src=x, trg=x, den=q, bndry="pxyz")
+
+
 WORKLOADS = {
     "uniform-poten-64k": uniform_poten_64k,
     "adaptive-grad-2m": adaptive_grad_2m,
     "ellipsoid-stokes-1m": ellipsoid_stokes_1m,
     "densities-32": uniform_poten_1m_32_densities,
     "clouds-64": uniform_clouds_64,
+    "periodic-poten-1m": uniform_periodic_1m,
     "energy-1m": uniform_energy_1m,
 }
