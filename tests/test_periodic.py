@@ -6,8 +6,9 @@ What pins what:
   * the oracle's periodic Laplace FMM against an Ewald sum (tests/ewald.py, tin-foil, uniform
     background for a net charge): potential up to its additive constant, and the field; the
     error falls geometrically with m;
-  * the oracle's periodic Stokes FMM (no Ewald sum here) by translation invariance of the
-    periodic velocity field up to a uniform flow;
+  * the oracle's periodic Stokes FMM against Hasimoto's Ewald sum (itself checked for
+    independence of its splitting parameter), and by translation invariance of the periodic
+    velocity field up to a uniform flow;
   * the oracle's closed-form cube integrals against brute-force quadrature and the identities
     tr S = 2 L, symmetry and div u = 0;
   * the oracle's periodic 2:1 balance by a brute-force check across the wrap;
@@ -93,6 +94,29 @@ def test_oracle_periodic_laplace_matches_ewald():
     qn = q - q.mean()
     g = _oracle_eval("laplace_grad", x, qn, 8, 20, "pxyz").reshape(-1, 3)
     assert _rel(g, ewald.laplace(x, qn, x, grad=True)) < 2e-6
+
+
+def test_stokes_ewald_sum_is_split_independent():
+    """The Hasimoto sum must not depend on its splitting parameter: a wrong real-space term or
+    self term would change with xi."""
+    rng = np.random.default_rng(1)
+    x = rng.random((200, 3))
+    f = rng.standard_normal((200, 3))
+    f -= f.mean(0)
+    u5, u6 = ewald.stokes(x, f, x, xi=5.0, kmax=14), ewald.stokes(x, f, x, xi=6.0, kmax=16)
+    assert _rel_mean_free(u5, u6) < 1e-11
+
+
+def test_oracle_periodic_stokes_matches_ewald():
+    """The oracle's periodic Stokes FMM against Hasimoto's Ewald sum (measured 1.8e-3 at m = 6,
+    1.8e-4 at m = 8, 1.6e-6 at m = 10: the periodic Stokes lattice operator converges slowly at
+    low orders)."""
+    rng = np.random.default_rng(1)
+    x = rng.random((300, 3))
+    f = rng.standard_normal((300, 3))
+    f -= f.mean(0)
+    u = _oracle_eval("stokes_vel", x, f, 8, 20, "pxyz").reshape(-1, 3)
+    assert _rel_mean_free(u, ewald.stokes(x, f, x)) < 5e-4
 
 
 def test_oracle_periodic_stokes_translation_invariant():
@@ -280,23 +304,43 @@ def test_periodic_gpu_matches_ewald(gpu, prec):
 
 
 @pytest.mark.gpu
-def test_periodic_context_c_abi(gpu):
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_periodic_stokes_gpu_matches_ewald(gpu, prec):
+    """Periodic Stokes on the GPU against Hasimoto's Ewald sum at m = 10 (velocities up to a
+    uniform flow), forces of zero sum."""
+    rng = np.random.default_rng(10)
+    n = 1500
+    x = rng.random((n, 3))
+    f = rng.standard_normal((n, 3))
+    f -= f.mean(0)
+    t = P.PtFMMTree(x, x, 32, bndry="pxyz")
+    t.setup(P.PtFMM("stokes_vel", 10, precision=prec))
+    e = _rel_mean_free(t.evaluate(f.ravel()).reshape(-1, 3), ewald.stokes(x, f, x))
+    print(f"periodic stokes {prec} vs Ewald: {e:.3e}", flush=True)
+    assert e < (1e-5 if prec == "fp64" else 1e-4)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,kid,out_exp", [("laplace_poten", 0, 1.0), ("laplace_grad", 1, 2.0), ("stokes_vel", 3, 1.0)])
+def test_periodic_context_c_abi(gpu, kernel, kid, out_exp):
     """PVFMMCreateContextD with box_size > 0: coordinates in [0, box_size) (and outside it,
     wrapped by whole boxes), results equal the oracle's periodic FMM of the scaled points with
-    the outputs scaled by box_size^-1 (laplace potential)."""
+    the outputs scaled by box_size^-e, e the kernel's output exponent (src/pvfmm-wrapper.cpp:
+    522-541: 1 for the potentials and velocities, 2 for the gradient)."""
     import ctypes
 
     from pvfmm_amd import ffi
 
     rng = np.random.default_rng(9)
     n, box = 5000, 2.0
+    k0, k1 = (3, 3) if kernel == "stokes_vel" else (1, 3 if kernel == "laplace_grad" else 1)
     x = rng.random((n, 3)) * box
     x[:50] += box  # one box beyond: wrapped into [0, box)
-    q = rng.standard_normal(n)
+    q = rng.standard_normal(k0 * n)
     L = ffi.lib()
-    ctx = L.PVFMMCreateContextD(box, 64, 8, 0, 0)
+    ctx = L.PVFMMCreateContextD(box, 64, 8, kid, 0)
     ffi.check()
-    out = np.zeros(n)
+    out = np.zeros(k1 * n)
     xf = x.ravel().copy()
     L.PVFMMEvalD(ffi.dptr(xf), ffi.dptr(q), None, n, ffi.dptr(xf), ffi.dptr(out), n, ctx, 1)
     ffi.check()
@@ -304,11 +348,13 @@ def test_periodic_context_c_abi(gpu):
     L.PVFMMDestroyContextD(ctypes.byref(p))
     xs = (x / box) % 1.0
     ot = O.OracleTree(xs, xs, 64, pt=_context_grid(64), bndry="pxyz")
-    ops = O.precompute("laplace_poten", 8)
-    ops["bc"] = P.PtFMM("laplace_poten", 8).operator("bc_pxyz").reshape(296, 296)  # the context's operators
+    ops = O.precompute(kernel, 8)
+    ops["bc"] = P.PtFMM(kernel, 8).operator("bc_pxyz").reshape(t_bc_shape(kernel, 8))  # the context's operators
     ot.set_operators(ops)
-    uo = ot.evaluate(q) / box
-    assert _rel(out, uo) < 1e-12
+    uo = ot.evaluate(q) * box ** -out_exp
+    # Stokes at m = 8: the product's Jacobi factors and the oracle's LAPACK ones differ at the
+    # FP64 cutoff (free-space Stokes at m = 8 agrees to 3e-9 as well); measured 7.2e-9
+    assert _rel(out, uo) < (5e-8 if kernel == "stokes_vel" else 1e-12)
 
 
 def _context_grid(n):
