@@ -39,13 +39,29 @@ def _rel_mean_free(a, b):
     return float(np.linalg.norm(d - d.mean(0)) / np.linalg.norm(b2 - b2.mean(0)))
 
 
+_OPS = {}
+
+
+def _oracle_ops(kernel, m, bndry, real):
+    """the oracle's operators and boundary operator, computed once per module run"""
+    key = (kernel, m, bndry, real)
+    if key not in _OPS:
+        ops = O.precompute(kernel, m, real)
+        ops["bc"] = O.bc_matrix(kernel, m, bndry, real)
+        _OPS[key] = ops
+    return _OPS[key]
+
+
 def _oracle_eval(kernel, x, q, m, max_pts, bndry, trg=None, bc=None, real="fp64"):
     """The oracle's periodic FMM; bc: a boundary operator to use instead of the oracle's own
     (the GPU parity tests pass the product's, whose own parity is tested separately)."""
     trg = x if trg is None else trg
     t = O.OracleTree(x, trg, max_pts, bndry=bndry)
-    ops = O.precompute(kernel, m, real)
-    ops["bc"] = O.bc_matrix(kernel, m, bndry, real) if bc is None else bc
+    if bc is None:
+        ops = _oracle_ops(kernel, m, bndry, real)
+    else:
+        ops = dict(O.precompute(kernel, m, real))
+        ops["bc"] = bc
     t.set_operators(ops)
     return t.evaluate(np.ravel(q))
 
