@@ -299,6 +299,40 @@ def test_periodic_gpu_edge_cases(gpu, case):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kernel,prec", [("laplace_poten", "mixed"), ("stokes_vel", "fp64")])
+def test_periodic_density_batches_and_graph(gpu, kernel, prec):
+    """Periodic plans evaluate several density vectors in one pass (the boundary product and the
+    volume-potential correction take every vector) equal to one at a time, and periodic trees
+    replay through a recorded HIP graph (PtFMMBatch) as they evaluate eagerly."""
+    import torch
+
+    rng = np.random.default_rng(13)
+    n = 4000
+    x = rng.random((n, 3))
+    k0 = 3 if kernel == "stokes_vel" else 1
+    q = rng.standard_normal((10, k0 * n))
+    fmm = P.PtFMM(kernel, 6, prec)
+    tb = P.PtFMMTree(x, x, 48, bndry="pxyz").setup(fmm, n_vec_max=8)
+    t1 = P.PtFMMTree(x, x, 48, bndry="pxyz").setup(fmm, n_vec_max=1)
+    ub = tb.evaluate(q)
+    for i in range(10):
+        r = _rel(ub[i], t1.evaluate(q[i]))
+        assert r < (1e-12 if prec == "fp64" else 1e-6), (i, r)
+    x2 = rng.random((2500, 3))
+    t2 = P.PtFMMTree(x2, x2, 48, bndry="pxy").setup(fmm)
+    dens = [torch.from_numpy(q[0]).to("cuda:0"), torch.from_numpy(rng.standard_normal(k0 * 2500)).to("cuda:0")]
+    eager = [t1.evaluate_device(dens[0]).clone(), t2.evaluate_device(dens[1]).clone()]
+    batch = P.PtFMMBatch([t1, t2])
+    outs = [torch.empty_like(e) for e in eager]
+    for rep in range(3):
+        batch.evaluate_device(dens, outs)
+        torch.cuda.synchronize()
+        for o, e in zip(outs, eager):
+            assert torch.equal(o, e), rep
+    assert batch.records() == 1
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_periodic_gpu_matches_ewald(gpu, prec):
     """End to end against the Ewald sum at m = 10: potential (net charge, so the volume-potential
