@@ -22,7 +22,9 @@ positions enter through the tree: the coordinate gradients come from the gradien
 
 The tree is built once for the point set given to LaplaceFMM; x passed to potential()/energy()
 must be those positions and only receives the gradient.  All FMM work runs in libpvfmm.so on
-the GPU.
+the GPU.  With a periodic boundary (bndry "px", "pxy" or "pxyz") the adjoint runs through the
+periodic images and the lattice operator as well; the position gradients are those of a
+neutral charge set there (the gradient kernel carries no uniform-background term).
 """
 from __future__ import annotations
 
@@ -34,14 +36,15 @@ from .wrapper import PtFMM, PtFMMTree
 class LaplaceFMM:
     """Potential and energy of a fixed point set with FMM forward and adjoint-FMM backward.
 
-    precision: "fp64" or "mixed" (FP32 bulk, FP64 operators and merge)."""
+    precision: "fp64" or "mixed" (FP32 bulk, FP64 operators and merge); bndry: "free" or a
+    periodic boundary type of PtFMMTree."""
 
-    def __init__(self, points, m: int = 10, max_pts: int = 256, precision: str = "mixed"):
+    def __init__(self, points, m: int = 10, max_pts: int = 256, precision: str = "mixed", bndry: str = "free"):
         pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
         self.n = len(pts)
         self.points = pts
-        self._tp = PtFMMTree(pts, pts, max_pts).setup(PtFMM("laplace_poten", m, precision), n_vec_max=2)
-        self._tg = PtFMMTree(pts, pts, max_pts).setup(PtFMM("laplace_grad", m, precision), n_vec_max=2)
+        self._tp = PtFMMTree(pts, pts, max_pts, bndry=bndry).setup(PtFMM("laplace_poten", m, precision), n_vec_max=2)
+        self._tg = PtFMMTree(pts, pts, max_pts, bndry=bndry).setup(PtFMM("laplace_grad", m, precision), n_vec_max=2)
 
     # -- raw evaluations on float64 GPU tensors
     def _phi(self, q):

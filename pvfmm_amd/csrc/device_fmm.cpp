@@ -529,6 +529,7 @@ class DevicePlan final : public DevicePlanBase {
     node_size_.upload(nsz, &total);
     src_begin_.upload(sb, &total); src_end_.upload(se, &total);
     trg_begin_.upload(tb, &total); trg_end_.upload(te, &total);
+    h_.center = center; h_.nsz = nsz; h_.sb = sb; h_.se = se; h_.tb = tb; h_.te = te; h_.imgs = imgs;
     src_perm_.upload(T.src.perm, &total);
     trg_perm_.upload(T.trg.perm, &total);
 
@@ -930,11 +931,23 @@ class DevicePlan final : public DevicePlanBase {
     std::vector<int32_t> tleaf, u_ptr, u_items, far_ptr, far_node, x_node, x_ptr, x_items, s2m_node;
     std::vector<double> far_rad;
     std::vector<int32_t> v_partner, v_src_fc, v_trg_fc, offs, vsrc_depth, vtrg_depth;
+    // the node table (real nodes, then the periodic images as (node, shift code))
+    std::vector<double> center;
+    std::vector<float> nsz;
+    std::vector<int32_t> sb, se, tb, te;
+    std::vector<std::pair<int32_t, int>> imgs;
   } h_;
   int64_t device_bytes_ = 0;
 
   struct AdjointPlan {
     DBuf<Real> uc2ue0T, uc2ue1T, dc2de0T, dc2de1T, u2uT, d2dT, u2urT, d2drT;
+    // node table of the transposed lists: the forward one plus images of target-side nodes (a
+    // forward item at shift s becomes, transposed, its target seen from the source at -s)
+    DBuf<double> center;
+    DBuf<float> nsz;
+    DBuf<int32_t> sb, se, tb, te;
+    DBuf<Real> bcT;        // periodic boundary operator transposed (K1_ x K0_)
+    DBuf<double> volpart;  // per-block partial sums of the volume-potential correction's transpose
     DBuf<Real> tden;     // adjoint input gathered into target order (k1 x n_trg per vector)
     DBuf<double> sval;   // adjoint output accumulated in source order (k0 x n_src per vector)
     DBuf<int32_t> iota;  // 0, 1, 2, ...: one item per row
@@ -964,22 +977,26 @@ class DevicePlan final : public DevicePlanBase {
   }
 
   // CSR of the transposed relation: for rows r with items it, the rows of item -> list of r
-  static void transpose_csr(const std::vector<int32_t>& row_key, const std::vector<int32_t>& ptr,
-                            const std::vector<int32_t>& items, int n_keys, std::vector<int32_t>& out_row,
-                            std::vector<int32_t>& out_ptr, std::vector<int32_t>& out_items) {
-    std::vector<int32_t> cnt(n_keys + 1, 0);
-    for (int32_t s : items) cnt[s + 1]++;
+  // Groups (key, value, payload) triples by key (stable): the rows out_row (keys in increasing
+  // order), their item ranges out_ptr and the values and payloads in forward order.
+  static void group_by_key(const std::vector<int32_t>& keys, const std::vector<int32_t>& vals,
+                           const std::vector<int32_t>& pay, int n_keys, std::vector<int32_t>& out_row,
+                           std::vector<int32_t>& out_ptr, std::vector<int32_t>& out_items, std::vector<int32_t>& out_pay) {
     std::vector<int32_t> start(n_keys + 1, 0);
-    for (int k = 0; k < n_keys; k++) start[k + 1] = start[k] + cnt[k + 1];
-    std::vector<int32_t> flat(items.size());
+    for (int32_t k : keys) start[k + 1]++;
+    for (int k = 0; k < n_keys; k++) start[k + 1] += start[k];
+    std::vector<int32_t> flat(keys.size()), flatp(keys.size());
     std::vector<int32_t> fill = start;
-    for (size_t r = 0; r + 1 < ptr.size(); r++)
-      for (int32_t i = ptr[r]; i < ptr[r + 1]; i++) flat[fill[items[i]]++] = row_key[r];
-    out_row.clear(); out_ptr.assign(1, 0); out_items.clear();
+    for (size_t i = 0; i < keys.size(); i++) {
+      flat[fill[keys[i]]] = vals[i];
+      flatp[fill[keys[i]]++] = pay[i];
+    }
+    out_row.clear(); out_ptr.assign(1, 0); out_items.clear(); out_pay.clear();
     for (int k = 0; k < n_keys; k++) {
       if (start[k + 1] == start[k]) continue;
       out_row.push_back(k);
       out_items.insert(out_items.end(), flat.begin() + start[k], flat.begin() + start[k + 1]);
+      out_pay.insert(out_pay.end(), flatp.begin() + start[k], flatp.begin() + start[k + 1]);
       out_ptr.push_back((int32_t)out_items.size());
     }
   }
@@ -1012,6 +1029,45 @@ class DevicePlan final : public DevicePlanBase {
     a->d2dT.upload(transposed(conv(O_.d2d), K1_, 8 * K1_), &total);                                     // 8K1 x K1
     a->tden.alloc((int64_t)k1_ * n_trg_ * nvb_, &total);
     a->sval.alloc((int64_t)k0_ * n_src_ * nvb_, &total);
+    // the adjoint's node table and the transposition of a list with images: a forward item v of
+    // row node r (v a source node, or an image (s, shift)) becomes item adj(r, -shift) of row s
+    std::vector<double> ac = h_.center;
+    std::vector<float> ansz = h_.nsz;
+    std::vector<int32_t> asb = h_.sb, ase = h_.se, atb = h_.tb, ate = h_.te;
+    std::unordered_map<int64_t, int32_t> aimg;
+    auto real_of = [&](int32_t v, int& code) -> int32_t {
+      if (v < N) { code = 13; return v; }
+      code = h_.imgs[v - N].second;
+      return h_.imgs[v - N].first;
+    };
+    auto adj = [&](int32_t node, int code) -> int32_t {
+      if (code == 13) return node;
+      const int neg = 26 - code;  // the negated shift
+      const int64_t key = (int64_t)node * 27 + neg;
+      auto it = aimg.find(key);
+      if (it != aimg.end()) return it->second;
+      const int32_t v = (int32_t)ansz.size();
+      const int sh[3] = {neg % 3 - 1, (neg / 3) % 3 - 1, neg / 9 - 1};
+      for (int d = 0; d < 3; d++) ac.push_back(h_.center[3 * (size_t)node + d] + sh[d]);
+      ansz.push_back(h_.nsz[node]);
+      asb.push_back(h_.sb[node]); ase.push_back(h_.se[node]);
+      atb.push_back(h_.tb[node]); ate.push_back(h_.te[node]);
+      aimg.emplace(key, v);
+      return v;
+    };
+    auto transpose = [&](const std::vector<int32_t>& row_key, const std::vector<int32_t>& ptr,
+                         const std::vector<int32_t>& items, std::vector<int32_t>& out_row, std::vector<int32_t>& out_ptr,
+                         std::vector<int32_t>& out_items, std::vector<int32_t>& out_real) {
+      std::vector<int32_t> keys(items.size()), vals(items.size()), real(items.size());
+      for (size_t r = 0; r + 1 < ptr.size(); r++)
+        for (int32_t i = ptr[r]; i < ptr[r + 1]; i++) {
+          int code = 13;
+          keys[i] = real_of(items[i], code);
+          vals[i] = adj(row_key[r], code);
+          real[i] = row_key[r];
+        }
+      group_by_key(keys, vals, real, N, out_row, out_ptr, out_items, out_real);
+    };
     {
       std::vector<int32_t> iota(std::max<int>(n_tleaf_, n_s2m_) + 1);
       for (size_t i = 0; i < iota.size(); i++) iota[i] = (int32_t)i;
@@ -1019,8 +1075,8 @@ class DevicePlan final : public DevicePlanBase {
     }
     // U list
     {
-      std::vector<int32_t> row, ptr, items;
-      transpose_csr(h_.tleaf, h_.u_ptr, h_.u_items, N, row, ptr, items);
+      std::vector<int32_t> row, ptr, items, real;
+      transpose(h_.tleaf, h_.u_ptr, h_.u_items, row, ptr, items, real);
       a->n_u = (int)row.size();
       a->u_leaf.upload(row, &total); a->u_ptr.upload(ptr, &total); a->u_items.upload(items, &total);
     }
@@ -1033,18 +1089,18 @@ class DevicePlan final : public DevicePlanBase {
         key.push_back(h_.tleaf[r]);
         ptr.push_back((int32_t)items.size());
       }
-      std::vector<int32_t> row, tptr, titems;
-      transpose_csr(key, ptr, items, N, row, tptr, titems);
+      std::vector<int32_t> row, tptr, titems, treal;
+      transpose(key, ptr, items, row, tptr, titems, treal);
       a->n_w = (int)row.size();
       a->w_node.upload(row, &total); a->w_ptr.upload(tptr, &total); a->w_items.upload(titems, &total);
     }
     // X list
     {
-      std::vector<int32_t> row, ptr, items;
-      transpose_csr(h_.x_node, h_.x_ptr, h_.x_items, N, row, ptr, items);
+      std::vector<int32_t> row, ptr, items, real;
+      transpose(h_.x_node, h_.x_ptr, h_.x_items, row, ptr, items, real);
       a->n_x = (int)row.size();
-      std::vector<int64_t> den(items.size());
-      for (size_t i = 0; i < items.size(); i++) den[i] = (int64_t)items[i] * K1_;
+      std::vector<int64_t> den(items.size());  // the check potentials of the real nodes
+      for (size_t i = 0; i < items.size(); i++) den[i] = (int64_t)real[i] * K1_;
       a->x_leaf.upload(row, &total); a->x_ptr.upload(ptr, &total); a->x_node.upload(items, &total);
       a->x_den.upload(den, &total);
       a->x_rad.upload(std::vector<double>(items.size(), kRad0), &total);
@@ -1092,6 +1148,14 @@ class DevicePlan final : public DevicePlanBase {
       a->fft_scal.upload(fscal, &total);
       a->ifft_scal.upload(iscal, &total);
     }
+    a->center.upload(ac, &total);
+    a->nsz.upload(ansz, &total);
+    a->sb.upload(asb, &total); a->se.upload(ase, &total);
+    a->tb.upload(atb, &total); a->te.upload(ate, &total);
+    if (bndry_ != kFreeSpace) {
+      a->bcT.upload(transposed(conv(O_.bc[bndry_]), K0_, K1_), &total);
+      if (vol_) a->volpart.alloc((int64_t)K::vol_adjoint_blocks() * 3 * nvb_, &total);
+    }
     device_bytes_ += total;
     stats.device_bytes = device_bytes_;
     adj_ = std::move(a);
@@ -1107,14 +1171,17 @@ class DevicePlan final : public DevicePlanBase {
     K::fill(st_, eq_.p, eq_vs_ * nv);
     K::fill(st_, dn_.p, dn_vs_ * nv);
     K::fill64(st_, a.sval.p, o_vs * nv);
+    // (volume-potential correction)^T: the targets' weighted sums onto the root's equivalent density
+    if (vol_)
+      K::vol_correction_adjoint(st_, O_.ker.pt_kind, n_trg_, txa_.p, a.tden.p, K0_, e0_, eq_.p, a.volpart.p, nv, g_vs, eq_vs_);
     // U list^T: target points -> source points of the leaves listing them
-    K::p2p_points(st_, ek, a.n_u, a.u_leaf.p, a.u_ptr.p, a.u_items.p, src_begin_.p, src_end_.p, trg_begin_.p, trg_end_.p,
-                  center_.p, sx64_.p, n_src_, tx64_.p, a.tden.p, n_trg_, a.sval.p, true, nv, g_vs, o_vs);
+    K::p2p_points(st_, ek, a.n_u, a.u_leaf.p, a.u_ptr.p, a.u_items.p, a.sb.p, a.se.p, a.tb.p, a.te.p,
+                  a.center.p, sx64_.p, n_src_, tx64_.p, a.tden.p, n_trg_, a.sval.p, true, nv, g_vs, o_vs);
     // (L2T P2P)^T: target points -> their leaf's downward equivalent surface (RAD1)
-    K::p2p_surface(st_, ek, n_tleaf_, t_leaf_.p, a.iota.p, t_leaf_.p, trg_begin_.p, trg_end_.p, center_.p, node_size_.p, kRad1,
+    K::p2p_surface(st_, ek, n_tleaf_, t_leaf_.p, a.iota.p, t_leaf_.p, a.tb.p, a.te.p, a.center.p, a.nsz.p, kRad1,
                    unit_surf_.p, ns_, tx_.p, a.tden.p, n_trg_, eq_.p, l2t_rc_.p, K0_, true, nv, g_vs, eq_vs_);
     // (W list M2T)^T: target points -> upward equivalent surfaces (RAD0) of their W sources
-    K::p2p_surface(st_, ek, a.n_w, a.w_node.p, a.w_ptr.p, a.w_items.p, trg_begin_.p, trg_end_.p, center_.p, node_size_.p, kRad0,
+    K::p2p_surface(st_, ek, a.n_w, a.w_node.p, a.w_ptr.p, a.w_items.p, a.tb.p, a.te.p, a.center.p, a.nsz.p, kRad0,
                    unit_surf_.p, ns_, tx_.p, a.tden.p, n_trg_, eq_.p, a.w_node.p, K0_, true, nv, g_vs, eq_vs_);
     // D2T GEMMs^T: equivalent -> check potential of each target leaf
     K::gemm_rows(st_, n_l2t_, rd_, K0_, eq_.p, l2t_rc_.p, K0_, a.dc2de1T.p, ctmp_.p, l2t_seq_.p, rd_, nullptr, false,
@@ -1134,6 +1201,10 @@ class DevicePlan final : public DevicePlanBase {
                      gemm_work_elems_, sp, nv, dn_vs_, dn_vs_);
       }
     }
+    // (periodic boundary)^T: root equivalent density += root check potential x BC^T
+    if (bndry_ != kFreeSpace)
+      K::gemm_rows(st_, 1, K0_, K1_, dn_.p, lvl_seq_.p, K1_, a.bcT.p, eq_.p, lvl_seq_.p, K0_, nullptr, true, gemm_work_.p,
+                   gemm_work_elems_, sp, nv, dn_vs_, eq_vs_);
     // V list^T: the M2L of offset -R from the target parents' children to the source parents'
     // (the spectra buffers swap roles: Y_ holds the new sources, X_ the new targets)
     for (int v = 0; v < nv; v++) {
@@ -1146,8 +1217,8 @@ class DevicePlan final : public DevicePlanBase {
                    a.ifft_scal.p);
     }
     // (X list S2L)^T: downward check surfaces (RAD0) -> source points
-    K::far_points(st_, ek, a.n_x, a.x_leaf.p, a.x_ptr.p, a.x_node.p, a.x_den.p, a.x_rad.p, src_begin_.p, src_end_.p,
-                  center_.p, node_size_.p, unit_surf_.p, ns_, sx_.p, n_src_, dn_.p, a.sval.p, true, nv, dn_vs_, o_vs);
+    K::far_points(st_, ek, a.n_x, a.x_leaf.p, a.x_ptr.p, a.x_node.p, a.x_den.p, a.x_rad.p, a.sb.p, a.se.p,
+                  a.center.p, a.nsz.p, unit_surf_.p, ns_, sx_.p, n_src_, dn_.p, a.sval.p, true, nv, dn_vs_, o_vs);
     // U2U^T, top-down: [8 children] += parent x [U2U_0 .. U2U_7]^T
     for (int l = 0; l <= max_depth_ - 1; l++) {
       const LevelRows& L = *m2m_[l];
@@ -1167,19 +1238,18 @@ class DevicePlan final : public DevicePlanBase {
     K::gemm_rows(st_, n_s2m_, K1_, ru_, ctmp_.p, s2m_orow_.p, ru_, a.uc2ue0T.p, chk_.p, s2m_orow_.p, K1_, s2m_alpha_.p, false,
                  gemm_work_.p, gemm_work_elems_, sp, nv, ctmp_vs_, chk_vs_);
     // (S2U P2P)^T: upward check surface (RAD1) -> the leaf's source points
-    K::far_points(st_, ek, n_s2m_, s2m_node_.p, a.iota.p, s2m_node_.p, a.s2u_den.p, a.s2u_rad.p, src_begin_.p, src_end_.p,
-                  center_.p, node_size_.p, unit_surf_.p, ns_, sx_.p, n_src_, chk_.p, a.sval.p, true, nv, chk_vs_, o_vs);
+    K::far_points(st_, ek, n_s2m_, s2m_node_.p, a.iota.p, s2m_node_.p, a.s2u_den.p, a.s2u_rad.p, a.sb.p, a.se.p,
+                  a.center.p, a.nsz.p, unit_surf_.p, ns_, sx_.p, n_src_, chk_.p, a.sval.p, true, nv, chk_vs_, o_vs);
     // gather^T: back to the input order of the sources (times the density scale)
     K::scatter_output(st_, n_src_, k0_, a.sval.p, src_perm_.p, den_scale, d_out, nv, o_vs);
   }
 
  public:
-  bool adjoint_supported() const override { return O_.ker.pt_kind == O_.ker.exp_kind && bndry_ == kFreeSpace; }
+  bool adjoint_supported() const override { return O_.ker.pt_kind == O_.ker.exp_kind; }
   void evaluate_adjoint(const double* d_g, double* d_out, int n_vec, double den_scale, double out_scale) override {
     if (!adjoint_supported())
-      throw Error(kErrUnsupported, bndry_ != kFreeSpace ? std::string("the adjoint evaluation needs a free-space tree")
-                                                        : std::string("the adjoint evaluation needs a symmetric kernel (laplace_poten, stokes_vel); ") +
-                                                              O_.ker.name + " is not");
+      throw Error(kErrUnsupported, std::string("the adjoint evaluation needs a symmetric kernel (laplace_poten, stokes_vel); ") +
+                                       O_.ker.name + " is not");
     if (n_vec < 1) throw Error(kErrInvalidArgument, "n_vec must be >= 1");
     if (!adj_) build_adjoint();
     for (int v = 0; v < n_vec; v += nvb_)

@@ -158,6 +158,53 @@ __global__ __launch_bounds__(256) void k_vol_correction(int kind, int64_t n, con
   }
 }
 
+// Transpose of k_vol_correction, first step: block b of vector v sums g_j(x_i) W_cj(x_i) over
+// its targets (i = 256 b + t, stepping by the grid) into part[(v nb + b) 3 + c].
+constexpr int kVolAdjBlocks = 1024;
+template <typename Real>
+__global__ __launch_bounds__(256) void k_vol_adjoint_partial(int kind, int64_t n, const double* __restrict__ x,
+                                                             const Real* __restrict__ g, int64_t g_vs,
+                                                             double* __restrict__ part) {
+  __shared__ double red[3][256];
+  const int v = blockIdx.y, nb = gridDim.x;
+  const Real* gv = g + v * g_vs;
+  double acc[3] = {0, 0, 0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)nb * blockDim.x) {
+    const double x0 = x[i], x1 = x[n + i], x2 = x[2 * n + i];
+    const double r2 = x0 * x0 + x1 * x1 + x2 * x2;
+    if (kind == 0) {
+      acc[0] += (double)gv[i] * r2 * (1.0 / 6.0);
+    } else {
+      acc[0] += (double)gv[i] * (r2 - x0 * x0) * 0.25;
+      acc[1] += (double)gv[n + i] * (r2 - x1 * x1) * 0.25;
+      acc[2] += (double)gv[2 * n + i] * (r2 - x2 * x2) * 0.25;
+    }
+  }
+  for (int c = 0; c < 3; c++) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[((int64_t)v * nb + blockIdx.x) * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+// second step (one block per vector): the block sums in block order, added to every surface
+// point's component c of the root's equivalent density
+template <typename Real>
+__global__ __launch_bounds__(256) void k_vol_adjoint_apply(int nb, const double* __restrict__ part, int K0, int e0,
+                                                           Real* __restrict__ eq, int64_t eq_vs) {
+  __shared__ double tot[3];
+  const int v = blockIdx.x;
+  if (threadIdx.x < 3) {
+    double t = 0;
+    for (int b = 0; b < nb; b++) t += part[((int64_t)v * nb + b) * 3 + threadIdx.x];
+    tot[threadIdx.x] = t;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < K0; i += 256) eq[v * eq_vs + i] += (Real)tot[i % e0];
+}
+
 // ----------------------------------------------------------------------------- direct sum
 // FP64 all-pairs evaluation (the reference's Kernel<T>::ker_poten, include/kernel.hpp:84).
 // Block (b, c) takes targets [256 b, 256 b + 256), one per thread, and the sources of chunk c,
@@ -2043,6 +2090,21 @@ void DeviceKernels<Real>::vol_correction(hipStream_t s, int kind, int64_t n, con
   if ((kind != 0 && kind != 2) || e0 < 1 || e0 > 3) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   hipLaunchKernelGGL(k_vol_correction<Real>, dim3(std::min(grid_for(n, 256), 1024), nv), dim3(256), 0, s, kind, n, x, eq_root,
                      K0, e0, val, eq_vs, val_vs);
+  HIP_LAUNCH_CHECK();
+}
+template <typename Real>
+int DeviceKernels<Real>::vol_adjoint_blocks() {
+  return kVolAdjBlocks;
+}
+template <typename Real>
+void DeviceKernels<Real>::vol_correction_adjoint(hipStream_t s, int kind, int64_t n, const double* x, const Real* g, int K0,
+                                                 int e0, Real* eq_root, double* part, int nv, int64_t g_vs, int64_t eq_vs) {
+  if (n == 0 || nv <= 0) return;
+  if ((kind != 0 && kind != 2) || e0 < 1 || e0 > 3 || !part) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+  hipLaunchKernelGGL(k_vol_adjoint_partial<Real>, dim3(kVolAdjBlocks, nv), dim3(256), 0, s, kind, n, x, g, g_vs, part);
+  HIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_vol_adjoint_apply<Real>, dim3(nv), dim3(256), 0, s, kVolAdjBlocks, (const double*)part, K0, e0, eq_root,
+                     eq_vs);
   HIP_LAUNCH_CHECK();
 }
 template <typename Real>

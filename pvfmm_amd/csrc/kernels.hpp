@@ -32,6 +32,12 @@ struct DeviceKernels {
   // density (kind 0: -|x|^2/6; kind 2: -(|x|^2 - x_j^2)/4 for c = j); x SoA absolute coordinates.
   static void vol_correction(hipStream_t s, int kind, int64_t n, const double* x, const Real* eq_root, int K0, int e0,
                              double* val, int nv, int64_t eq_vs, int64_t val_vs);
+  // Its transpose: eq_root[p*e0 + c] += sum_i sum_j g[j*n + i] W_cj(x_i) for every p, W the
+  // correction's factor (-V); per-block partial sums in a fixed order (part: vol_adjoint_blocks()
+  // * 3 * nv doubles), then added in block order (deterministic).
+  static int vol_adjoint_blocks();
+  static void vol_correction_adjoint(hipStream_t s, int kind, int64_t n, const double* x, const Real* g, int K0, int e0,
+                                     Real* eq_root, double* part, int nv, int64_t g_vs, int64_t eq_vs);
   static void fill64(hipStream_t s, double* p, int64_t n);
 
   // P2P into surface points (check surfaces): per row, targets are ns surface points of
