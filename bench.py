@@ -66,13 +66,23 @@ def time_steps(tree, dq, steps, warmup, timing):
 
 
 def cpu_model() -> str:
+    """/proc/cpuinfo's model name with its family, model and stepping numbers (a virtual machine
+    may show only a generic name)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (t.strip() for t in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                    info[k] = v
+            elif info:
+                break
     except OSError:
         pass
-    return "unknown"
+    if "model name" not in info:
+        return "unknown"
+    return (f"{info['model name']} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')})")
 
 
 def cpu_baseline(n_sample: int):
