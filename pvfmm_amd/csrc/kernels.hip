@@ -123,6 +123,11 @@ template <typename T>
 __global__ void k_fill(T* p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = T(0);
 }
+// the same with 16-byte stores: p 16-byte aligned, n16 = the number of 16-byte words
+__global__ void k_fill16(uint4* p, int64_t n16) {
+  const uint4 z = {0u, 0u, 0u, 0u};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x) p[i] = z;
+}
 
 // Periodic volume-potential correction: each block first sums the root's equivalent values per
 // component (K0 of them, at most a few thousand, read through L2), then corrects its targets.
@@ -2077,11 +2082,27 @@ void DeviceKernels<Real>::scatter_output(hipStream_t s, int64_t n, int k1, const
   hipLaunchKernelGGL(k_scatter_output, dim3(grid_for(n, 256), n_vec), dim3(256), 0, s, n, k1, val, perm, scal, out, out_stride);
   HIP_LAUNCH_CHECK();
 }
+// zeroes n values of T: 16-byte stores over the aligned whole words, scalar stores for the rest
+template <typename T>
+static void fill_any(hipStream_t s, T* p, int64_t n) {
+  if (n == 0) return;
+  int64_t head = 0;
+  if ((reinterpret_cast<uintptr_t>(p) % 16) == 0) {
+    const int64_t n16 = n * (int64_t)sizeof(T) / 16;
+    if (n16 > 0) {
+      hipLaunchKernelGGL(k_fill16, dim3(grid_for(n16, 256)), dim3(256), 0, s, reinterpret_cast<uint4*>(p), n16);
+      HIP_LAUNCH_CHECK();
+    }
+    head = n16 * 16 / (int64_t)sizeof(T);
+  }
+  if (head < n) {
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n - head, 256)), dim3(256), 0, s, p + head, n - head);
+    HIP_LAUNCH_CHECK();
+  }
+}
 template <typename Real>
 void DeviceKernels<Real>::fill(hipStream_t s, Real* p, int64_t n) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_fill<Real>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
-  HIP_LAUNCH_CHECK();
+  fill_any<Real>(s, p, n);
 }
 template <typename Real>
 void DeviceKernels<Real>::vol_correction(hipStream_t s, int kind, int64_t n, const double* x, const Real* eq_root, int K0,
@@ -2109,9 +2130,7 @@ void DeviceKernels<Real>::vol_correction_adjoint(hipStream_t s, int kind, int64_
 }
 template <typename Real>
 void DeviceKernels<Real>::fill64(hipStream_t s, double* p, int64_t n) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
-  HIP_LAUNCH_CHECK();
+  fill_any<double>(s, p, n);
 }
 
 // Density vectors in chunks of 8, 4, 2 and 1 per launch (the chunk shares the geometry).
