@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,7 @@ int guarded(F&& f) {
 struct FmmHandle {
   Operators O;
   int precision = 0;
+  std::mutex bc_mutex;  // the boundary operators are built on first use, by any tree set up with this handle
 };
 
 struct DeviceArray {
@@ -105,7 +107,10 @@ void setup_tree(TreeHandle* t, FmmHandle* f, int n_vec_max) {
   t->ensure_stream();
   t->plan.reset();
   t->fmm = f;
-  if (t->T.bndry != kFreeSpace) build_bc_operator(f->O, t->T.bndry);  // once per operator set and boundary type
+  if (t->T.bndry != kFreeSpace) {  // once per operator set and boundary type
+    std::lock_guard<std::mutex> lock(f->bc_mutex);
+    build_bc_operator(f->O, t->T.bndry);
+  }
   t->plan = make_plan(t->T, f->O, t->src.data(), t->trg.data(), f->precision, n_vec_max, t->stream);
 }
 
@@ -638,7 +643,12 @@ int64_t pvfmm_amd_fmm_operator(void* fmm, int which, double* out) {
   if (!f) return -1;
   if (which >= 7 && which <= 9) {  // periodic boundary operators, built on first request
     int64_t n = -1;
-    if (guarded([&] { build_bc_operator(f->O, which - 6); n = (int64_t)f->O.bc[which - 6].size(); }) != 0) return -1;
+    if (guarded([&] {
+          std::lock_guard<std::mutex> lock(f->bc_mutex);
+          build_bc_operator(f->O, which - 6);
+          n = (int64_t)f->O.bc[which - 6].size();
+        }) != 0)
+      return -1;
     if (out) std::copy(f->O.bc[which - 6].begin(), f->O.bc[which - 6].end(), out);
     return n;
   }
