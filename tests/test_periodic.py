@@ -148,6 +148,23 @@ def test_oracle_periodic_stokes_translation_invariant():
     assert _rel_mean_free(u2, u1) < 1e-3
 
 
+@pytest.mark.parametrize("bndry,shift", [("px", (0.37, 0.0, 0.0)), ("pxy", (0.37, 0.71, 0.0))])
+def test_oracle_partly_periodic_translation_invariant(bndry, shift):
+    """PX and PXY (no Ewald sum here): moving every point by a vector along the periodic axes
+    (mod 1) moves the potential with it, up to a constant (the corner fit removes the constant
+    and the linear terms along those axes)."""
+    rng = np.random.default_rng(5)
+    x = 0.1 + 0.8 * rng.random((500, 3))  # away from the free faces, whose neighbourhood changes
+    q = rng.standard_normal(500)
+    q -= q.mean()
+    u1 = _oracle_eval("laplace_poten", x, q, 8, 20, bndry)
+    xs = x.copy()
+    xs[:, :2] = (x[:, :2] + np.array(shift[:2])) % 1.0
+    u2 = _oracle_eval("laplace_poten", xs, q, 8, 20, bndry)
+    e = _rel_mean_free(u2, u1)
+    assert e < 1e-6, e
+
+
 @pytest.mark.parametrize("bndry", BTYPES)
 def test_oracle_periodic_balance_across_the_wrap(bndry):
     """Leaves that touch across a periodic face (every axis wraps in the balance) differ by at
