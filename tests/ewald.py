@@ -165,11 +165,14 @@ def laplace_slab(src, q, trg, alpha=6.0, kmax=12):
     assert abs(q.sum()) < 1e-12 * np.abs(q).sum(), "laplace_slab needs zero net charge"
     out = np.zeros(len(trg))
     imgs = np.array([[i, j, 0] for i in (-1, 0, 1) for j in (-1, 0, 1)], dtype=np.float64)
+    # k and -k give the same term: one half plane of the disc |m| <= kmax, counted twice; the
+    # z factor depends on |k| only, so it is evaluated once per distinct |k|
     r_ = np.arange(-kmax, kmax + 1)
     m = np.stack(np.meshgrid(r_, r_, indexing="ij"), -1).reshape(-1, 2)
-    m = m[np.any(m != 0, axis=1)]
+    m = m[((m[:, 0] > 0) | ((m[:, 0] == 0) & (m[:, 1] > 0))) & ((m * m).sum(1) <= kmax * kmax)]
     kv = 2 * np.pi * m.astype(np.float64)
-    kn = np.sqrt((kv * kv).sum(1))
+    m2, inv = np.unique((m * m).sum(1), return_inverse=True)
+    kn = 2 * np.pi * np.sqrt(m2.astype(np.float64))
     kk = kn / (2 * alpha)
     for t0 in range(0, len(trg), 32):
         T = trg[t0:t0 + 32]
@@ -182,8 +185,8 @@ def laplace_slab(src, q, trg, alpha=6.0, kmax=12):
         rxy = T[:, None, :2] - src[None, :, :2]  # (t, s, 2)
         z = T[:, None, 2] - src[None, :, 2]  # (t, s)
         az = alpha * z[..., None]
-        g = np.exp(-az * az - kk * kk) * (erfcx(az + kk) + erfcx(-az + kk)) / kn
-        out[t0:t0 + 32] += np.pi * ((np.cos(rxy @ kv.T) * g).sum(-1) * q).sum(1)
+        g = np.exp(-az * az - kk * kk) * (erfcx(az + kk) + erfcx(-az + kk)) / kn  # (t, s, |k|)
+        out[t0:t0 + 32] += 2 * np.pi * ((np.cos(rxy @ kv.T) * g[..., inv]).sum(-1) * q).sum(1)
         out[t0:t0 + 32] -= 2 * math.sqrt(math.pi) * ((np.exp(-(alpha * z) ** 2) / alpha
                                                       + math.sqrt(math.pi) * z * erf(alpha * z)) * q).sum(1)
     on = {tuple(x): i for i, x in enumerate(src)}

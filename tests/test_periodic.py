@@ -412,15 +412,21 @@ def test_partly_periodic_gpu_matches_lattice_sums(gpu, bndry, prec):
     """PX and PXY on the GPU at m = 10 against the wire and slab lattice sums, up to a constant
     (the oracle, on the CPU, reaches 1e-8 at this order)."""
     rng = np.random.default_rng(14)
-    n = 1500
+    n = 1000
     x = rng.random((n, 3))
     q = rng.standard_normal(n)
     q -= q.mean()
+    if bndry not in _LATTICE_REF:
+        _LATTICE_REF[bndry] = LATTICE[bndry](x, q, x)
     t = P.PtFMMTree(x, x, 32, bndry=bndry)
     t.setup(P.PtFMM("laplace_poten", 10, precision=prec))
-    e = _rel_mean_free(t.evaluate(q), LATTICE[bndry](x, q, x))
+    e = _rel_mean_free(t.evaluate(q), _LATTICE_REF[bndry])
     print(f"{bndry} {prec} vs lattice sum: {e:.3e}", flush=True)
-    assert e < (5e-8 if prec == "fp64" else 1e-5)
+    # measured with 1500 points: FP64 4.9e-9 (PX), 4.6e-9 (PXY); mixed 4.1e-7, 6.9e-7
+    assert e < (2e-8 if prec == "fp64" else 3e-6)
+
+
+_LATTICE_REF = {}
 
 
 @pytest.mark.gpu
