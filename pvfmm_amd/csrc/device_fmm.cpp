@@ -585,11 +585,13 @@ class DevicePlan final : public DevicePlanBase {
       //   B[2s+1][2t+1] = Re M[t][s],
       // stored per MFMA lane: [R][f][kb][nb][lane][q] = B[16kb + 4(lane/16) + q][16nb + lane%16].
       const int E0 = e0_, E1 = e1_, NS = 8 * E0, NR = 8 * E1;
-      // the FP32 scalar kernels use the bf16 parts below instead
+      // the FP32 kernels use the fp16 (and, scalar, bf16) parts below instead
       const bool bf16_parts = std::is_same<Real, float>::value && E0 == 1 && E1 == 1;
-      std::vector<Real> Vp(bf16_parts ? 0 : (size_t)27 * M_dim_ * E0 * E1 * 256, (Real)0);
+      const bool stokes_parts = std::is_same<Real, float>::value && E0 == 3 && E1 == 3;
+      const bool real_form = !bf16_parts && !stokes_parts;
+      std::vector<Real> Vp(real_form ? (size_t)27 * M_dim_ * E0 * E1 * 256 : 0, (Real)0);
       #pragma omp parallel for schedule(static)
-      for (int64_t rf = 0; rf < (bf16_parts ? 0 : (int64_t)27 * M_dim_); rf++) {
+      for (int64_t rf = 0; rf < (real_form ? (int64_t)27 * M_dim_ : 0); rf++) {
         const std::complex<double>* Mb = &O_.v1[(size_t)rf * NR * NS];  // [t][s]
         for (int kb = 0; kb < E0; kb++)
           for (int nb = 0; nb < E1; nb++)
@@ -670,6 +672,48 @@ class DevicePlan final : public DevicePlanBase {
                 std::memcpy(&Vh[((((size_t)j * 2 + 0) * M_dim_ + f) * 64 + l) * 8 + i], &h0, 2);
                 std::memcpy(&Vh[((((size_t)j * 2 + 1) * M_dim_ + f) * 64 + l) * 8 + i], &h1, 2);
               }
+        }
+        Vh_.upload(Vh, &total);
+        vtinv_.upload(tinv, &total);
+        split_ = true;
+      }
+      if (stokes_parts) {
+        // two fp16 parts of tau_f B for k_hadamard_split3, offsets in pairs (2j, 2j+1) of
+        // v1_offsets: [f][pair j][chunk c][tile nb][part p][lane][i] = part p of
+        // B_{2j + a}[8 b + i][16 nb + lane % 16], q = 4c + lane / 16, a = q / 6, b = q % 6;
+        // tau_f = 2^(14 - e) with every entry of frequency f's blocks below 2^e
+        const int n_pair = (int)O_.v1_offsets.size() / 2;
+        const size_t per_f = (size_t)n_pair * 3 * 3 * 2 * 64 * 8;
+        std::vector<uint16_t> Vh((size_t)M_dim_ * per_f);
+        std::vector<float> tinv(M_dim_);
+        auto entry = [&](int R, int f, int k, int n) {
+          const std::complex<double> v = O_.v1[((size_t)R * M_dim_ + f) * NR * NS + (size_t)(n >> 1) * NS + (k >> 1)];
+          return ((k & 1) == 0) ? ((n & 1) == 0 ? v.real() : v.imag()) : ((n & 1) == 0 ? -v.imag() : v.real());
+        };
+        #pragma omp parallel for schedule(static)
+        for (int f = 0; f < M_dim_; f++) {
+          double mx = 0;
+          for (int o = 0; o < 2 * n_pair; o++)
+            for (int k = 0; k < 2 * NS; k++)
+              for (int n = 0; n < 2 * NR; n++) mx = std::max(mx, std::fabs(entry(O_.v1_offsets[o], f, k, n)));
+          int e = 0;
+          if (mx > 0) (void)std::frexp(mx, &e);
+          const double tau = mx > 0 ? std::ldexp(1.0, std::min(100, std::max(-100, 14 - e))) : 1.0;
+          tinv[f] = (float)(1.0 / tau);
+          uint16_t* out = &Vh[(size_t)f * per_f];
+          for (int j = 0; j < n_pair; j++)
+            for (int c = 0; c < 3; c++)
+              for (int nb = 0; nb < 3; nb++)
+                for (int l = 0; l < 64; l++) {
+                  const int q = 4 * c + (l >> 4), a = q / 6, b = q % 6, R = O_.v1_offsets[2 * j + a];
+                  for (int i = 0; i < 8; i++) {
+                    const double x = entry(R, f, 8 * b + i, 16 * nb + (l & 15)) * tau;
+                    const _Float16 h0 = (_Float16)x, h1 = (_Float16)(x - (double)h0);
+                    const size_t at = (((((size_t)j * 3 + c) * 3 + nb) * 2) * 64 + l) * 8 + i;
+                    std::memcpy(&out[at], &h0, 2);
+                    std::memcpy(&out[at + 64 * 8], &h1, 2);
+                  }
+                }
         }
         Vh_.upload(Vh, &total);
         vtinv_.upload(tinv, &total);
@@ -832,7 +876,8 @@ class DevicePlan final : public DevicePlanBase {
     h_.v_partner = v_partner; h_.v_src_fc = v_src_fc; h_.v_trg_fc = v_trg_fc; h_.offs = offs;
     build_tiles(v_partner, n_vtrg_, n_vsrc_, vt_, &total);
     if (split_) {
-      vbound_.alloc(std::max(1, std::max(n_vsrc_, n_vtrg_)), &total);  // the adjoint's sources are the target slots
+      // per slot and component; the adjoint's sources are the target slots
+      vbound_.alloc(std::max(1, std::max(n_vsrc_ * e0_, n_vtrg_ * e1_)), &total);
       vrscale_.alloc(std::max(1, vt_.n), &total);
     }
     stats.v_tiles = vt_.n;

@@ -1158,25 +1158,28 @@ __device__ __forceinline__ void cmac_tw(Real gr, Real gi, Real& sr, Real& si) {
   }
 }
 
-// The spectrum bound of k_fft_up_m for the generic-m transform: bound[slot] = |scal[slot]| *
-// sum of |up| over the 8 children's ns values (component 0), in a fixed order.
+// The spectrum bound of k_fft_up_m for the generic-m transform: bound[slot * e0 + i] =
+// |scal[slot]| * sum of |up| over the 8 children's ns values of component i, in a fixed order.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_spectrum_bound(int n_slots, const int32_t* __restrict__ first_child,
-                                                        const Real* __restrict__ up, int K0, int ns,
+                                                        const Real* __restrict__ up, int K0, int e0, int ns,
                                                         const Real* __restrict__ scal, float* __restrict__ bound) {
   __shared__ Real red[256];
   const int slot = blockIdx.x, t = threadIdx.x;
   if (slot >= n_slots) return;  // whole block
   const int fc = first_child[slot];
-  Real a = 0;
-  for (int q = t; q < 8 * ns; q += 256) a += fabs(up[(int64_t)(fc + q / ns) * K0 + q % ns]);
-  red[t] = a;
-  __syncthreads();
-  if (t < 64) {
-    Real b = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
+  for (int i = 0; i < e0; i++) {
+    Real a = 0;
+    for (int q = t; q < 8 * ns; q += 256) a += fabs(up[(int64_t)(fc + q / ns) * K0 + (q % ns) * e0 + i]);
+    red[t] = a;
+    __syncthreads();
+    if (t < 64) {
+      Real b = red[t] + red[t + 64] + red[t + 128] + red[t + 192];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
-    if (t == 0) bound[slot] = (float)(b * fabs(scal[slot]));
+      for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
+      if (t == 0) bound[(int64_t)slot * e0 + i] = (float)(b * fabs(scal[slot]));
+    }
+    __syncthreads();
   }
 }
 
@@ -1272,8 +1275,9 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
     for (int q = 0; q < LPT; q++) v[q][x] = p >= 0 ? up[(int64_t)(fc + cg + CG * q) * K0 + p * e0 + i] * sc : Real(0);
   }
   if (bound) {
-    // sum of |v| over the slot's lattice values: no frequency of the spectrum exceeds it (unit
-    // twiddles); added in a fixed order (per thread, then the first wave, NT >= 64)
+    // sum of |v| over the slot's lattice values of component i: no frequency of that
+    // component's spectrum exceeds it (unit twiddles); added in a fixed order (per thread, then
+    // the first wave, NT >= 64)
     Real a = 0;
 #pragma unroll
     for (int q = 0; q < LPT; q++)
@@ -1286,7 +1290,7 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
       for (int u = t; u < NT; u += 64) b += s1r[u];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
-      if (t == 0) bound[slot] = (float)b;
+      if (t == 0) bound[(int64_t)slot * e0 + i] = (float)b;
     }
   }
   __syncthreads();
@@ -1994,6 +1998,166 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
   }
 }
 
+// FP32 frequency-domain M2L for Stokes (3x3 blocks, 48 reals per spectrum row) on the fp16
+// matrix cores, with the operand split of k_hadamard_split: the tile's partner spectra are
+// scaled by the power of two s with s times their largest bound below 2^14 (the bounds are per
+// source slot and component) and the transfer blocks of frequency f by tau_f, each split into
+// two fp16 parts, and the three largest products are summed in FP32; the result is divided by
+// s tau_f.  Offsets go in pairs (2j, 2j+1) of the offset list: the pair's K = 96 (48 reals of
+// each partner) is three K = 32 chunks, and lane (row, kg) of chunk c takes reals 8b..8b+7 of
+// the partner at offset 2j + a, with q = 4c + kg, a = q / 6, b = q % 6.  A pair of a 16-target
+// group then costs 27 MFMAs of K = 32 (3 chunks x 3 output tiles x 3 products) instead of 72
+// FP32 MFMAs of K = 4, at half the cycles each.
+// A block owns one tile (at most 64 targets in 4 groups of 16, at most UMAX partners) and one
+// frequency; the two frequencies of a pair take consecutive blocks of each tile on one XCD, so
+// the 384 bytes of a partner's two spectrum rows are read into that XCD's L2 once.  Wave w
+// computes the K chunks w, w + 4, ... of the 39 (13 pairs x 3) for all 4 groups, reading the
+// transfer-block parts of a chunk from global memory two chunks ahead; chunks at which none of
+// a group's targets has a partner are skipped.  The waves' partial sums are added through LDS
+// in wave order, wave w adding and writing group w.
+template <int UMAX>
+__global__ __launch_bounds__(256, 1) void k_hadamard_split3(int M_dim, int n_tiles, const int32_t* __restrict__ tile_t0,
+                                                           const int32_t* __restrict__ tile_u0,
+                                                           const int32_t* __restrict__ usrc, const uint16_t* __restrict__ uidx,
+                                                           const uint32_t* __restrict__ gmask, const uint16_t* __restrict__ Vh,
+                                                           const float* __restrict__ tinv, const float* __restrict__ bound,
+                                                           const float* __restrict__ X, float* __restrict__ Y) {
+  constexpr int NOFF = 26, NPAIR = NOFF / 2, RW = 48;
+  constexpr int NB = 3, NCH = 3;          // output tiles of 16 reals, K chunks of a pair
+  constexpr int EST = 52;                 // words per staged entry: 2 parts x 24 words, padded
+  constexpr int V4E = RW / 4;             // 4-vectors of a spectrum row
+  constexpr int NPASS = (UMAX * V4E + 255) / 256;
+  constexpr int XW = (UMAX + 1) * EST;    // staging (entry UMAX stays zero)
+  constexpr int RED = 4 * 4 * NB * 64 * 4;  // partial sums [wave][group][tile][lane][4]
+  constexpr int LW = XW > RED ? XW : RED;
+  constexpr int UW = 64 * NOFF / 4;       // partner table as 8-byte words
+  __shared__ __align__(16) uint32_t Xl[LW];
+  __shared__ __align__(8) uint16_t ul[64 * NOFF];
+  __shared__ float redm[4];
+  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
+  const int fp = (local / (2 * n_tiles)) * 8 + xcd, rem = local % (2 * n_tiles);
+  const int tile = rem >> 1, f = 2 * fp + (rem & 1);
+  if (f >= M_dim) return;  // whole block
+  const int t0 = tile_t0[tile], nt = tile_t0[tile + 1] - t0;
+  const int u0 = tile_u0[tile], nu = tile_u0[tile + 1] - u0;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, row = l & 15, kg = l >> 4;
+  // transfer-block parts of frequency f: [pair][chunk][tile][part] blocks of 64 lanes x 8, so
+  // global chunk ci = 3 j + c starts at block 6 ci.  Wave w takes chunks w, w + 4, ... (10, 10,
+  // 10 and 9 of the 39), their parts read PF chunks ahead into a ring.
+  constexpr int NCK = NPAIR * NCH, CPW = (NCK + 3) / 4, BPC = NB * 2, PF = 2;
+  const halfx8* vb = reinterpret_cast<const halfx8*>(Vh) + (int64_t)f * NCK * BPC * 64 + l;
+  halfx8 bq[PF + 1][BPC];
+#pragma unroll
+  for (int it = 0; it < PF; it++)
+#pragma unroll
+    for (int i = 0; i < BPC; i++) bq[it][i] = vb[((int64_t)(w + 4 * it) * BPC + i) * 64];  // w + 4 (PF - 1) < NCK
+  // partner spectra of the tile, and the largest bound of their components
+  floatx4 val[NPASS];
+  float bmx = 0.f;
+#pragma unroll
+  for (int i = 0; i < NPASS; i++) {
+    const int e = tid + 256 * i, u = e / V4E, r = e - u * V4E;
+    val[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (u < nu) {
+      const int s = usrc[u0 + u];
+      val[i] = *reinterpret_cast<const floatx4*>(X + ((int64_t)s * M_dim + f) * RW + 4 * r);
+      if (r < 3) bmx = fmaxf(bmx, bound[(int64_t)s * 3 + r]);
+    }
+  }
+  {
+    const uint2* ug = reinterpret_cast<const uint2*>(uidx + (int64_t)tile * 64 * NOFF);
+    for (int e = tid; e < UW; e += 256) reinterpret_cast<uint2*>(ul)[e] = ug[e];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) bmx = fmaxf(bmx, __shfl_xor(bmx, o));
+  if (l == 0) redm[w] = bmx;
+  for (int e = tid; e < EST; e += 256) Xl[UMAX * EST + e] = 0u;
+  __syncthreads();
+  const float bm = fmaxf(fmaxf(redm[0], redm[1]), fmaxf(redm[2], redm[3]));
+  float xsc = 1.f;
+  if (bm > 0.f && bm <= 3.4e38f) {
+    int ex = 0;
+    (void)frexpf(bm, &ex);  // bm < 2^ex
+    xsc = ldexpf(1.f, min(100, max(-100, 14 - ex)));
+  }
+  // entry layout (words): part p at 24p, reals 2q, 2q+1 in word 24p + q
+#pragma unroll
+  for (int i = 0; i < NPASS; i++) {
+    const int e = tid + 256 * i, u = e / V4E, r = e - u * V4E;
+    if (u < nu) {
+      halfx4 h0, h1;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float x = val[i][q] * xsc;  // exact (power of two)
+        h0[q] = (_Float16)x;
+        h1[q] = (_Float16)(x - (float)h0[q]);
+      }
+      uint32_t* d = &Xl[u * EST + 2 * r];
+      *reinterpret_cast<halfx4*>(d) = h0;
+      *reinterpret_cast<halfx4*>(d + 24) = h1;
+    }
+  }
+  uint32_t gm[4];
+#pragma unroll
+  for (int g = 0; g < 4; g++) gm[g] = gmask[4 * tile + g];
+  __syncthreads();
+  floatx4 acc[4][NB];
+#pragma unroll
+  for (int g = 0; g < 4; g++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[g][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int it = 0; it < CPW; it++) {
+    const int ci = w + 4 * it;
+    if (ci >= NCK) break;  // wave-uniform
+    if (ci + 4 * PF < NCK) {
+#pragma unroll
+      for (int i = 0; i < BPC; i++) bq[(it + PF) % (PF + 1)][i] = vb[((int64_t)(ci + 4 * PF) * BPC + i) * 64];
+    }
+    const halfx8* bc = bq[it % (PF + 1)];
+    const int j = ci / NCH, c = ci - NCH * j;
+    const int q = 4 * c + kg, a = q / 6, bo = q - 6 * a;
+    const int oi = 2 * j + a;
+    halfx8 A[4][2];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const int pos = ul[(oi * 16 + row) * 4 + g];  // a missing partner reads the zero entry
+#pragma unroll
+      for (int p = 0; p < 2; p++) A[g][p] = *reinterpret_cast<const halfx8*>(&Xl[pos * EST + 24 * p + 4 * bo]);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const uint32_t pm = (gm[g] >> (2 * j)) & 3u;  // offsets 2j, 2j+1 present in group g
+      if (!(c == 0 ? (pm & 1u) : c == 2 ? (pm & 2u) : pm)) continue;  // block-uniform
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        acc[g][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[g][0], bc[2 * nb], acc[g][nb], 0, 0, 0);
+        acc[g][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[g][1], bc[2 * nb], acc[g][nb], 0, 0, 0);
+        acc[g][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[g][0], bc[2 * nb + 1], acc[g][nb], 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();  // every wave is done reading the staged spectra
+  float* red = reinterpret_cast<float*>(Xl);
+#pragma unroll
+  for (int g = 0; g < 4; g++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) *reinterpret_cast<floatx4*>(&red[(((w * 4 + g) * NB + nb) * 64 + l) * 4]) = acc[g][nb];
+  __syncthreads();
+  const float ysc = tinv[f] / xsc;
+#pragma unroll
+  for (int nb = 0; nb < NB; nb++) {
+    floatx4 s = *reinterpret_cast<const floatx4*>(&red[((w * NB + nb) * 64 + l) * 4]);
+#pragma unroll
+    for (int v = 1; v < 4; v++) s += *reinterpret_cast<const floatx4*>(&red[(((v * 4 + w) * NB + nb) * 64 + l) * 4]);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int tg = 16 * w + 4 * kg + r;
+      if (tg < nt) Y[((int64_t)(t0 + tg) * M_dim + f) * RW + 16 * nb + row] = s[r] * ysc;
+    }
+  }
+}
+
 // The same products for 16 density vectors of one tree at once: the MFMA rows are the 16
 // vectors of one target (instead of 16 targets of one vector), so the 16 rows of an operand
 // read one partner's spectra, 16 consecutive 64-byte rows of X in the vector-group layout
@@ -2295,7 +2459,7 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
                                  const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X, int vg, int vi,
                                  float* bound) {
   if (n_slots == 0) return;
-  if (bound && (e0 != 1 || vg != 1)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+  if (bound && vg != 1) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   const dim3 g((unsigned)((n_slots + 7) / 8 * 8 * e0));  // whole XCD rounds of slots
 #define PVFMM_FFT_UP_M(MM)                                                                                        \
   case MM:                                                                                                        \
@@ -2324,7 +2488,7 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
                      scal, X, vg, vi);
   HIP_LAUNCH_CHECK();
   if (bound) {
-    hipLaunchKernelGGL(k_spectrum_bound<Real>, dim3(n_slots), dim3(256), 0, s, n_slots, first_child, up, K0,
+    hipLaunchKernelGGL(k_spectrum_bound<Real>, dim3(n_slots), dim3(256), 0, s, n_slots, first_child, up, K0, e0,
                        6 * (m - 1) * (m - 1) + 2, scal, bound);
     HIP_LAUNCH_CHECK();
   }
@@ -2378,13 +2542,20 @@ void DeviceKernels<Real>::hadamard_tiled(hipStream_t s, int n_tiles, int M_dim, 
                          tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
     }
   } else if (e0 == 3 && e1 == 3) {
-    // 2 frequencies per block, tiles of up to 160 partners
+    // tiles of up to 160 partners; FP32: one frequency per block on the fp16 matrix cores,
+    // FP64: 2 frequencies per block
     constexpr int UMS = 160;
     const int n_fg2 = (M_dim + 1) / 2;
     const int64_t nb2 = (int64_t)n_tiles * ((n_fg2 + 7) / 8) * 8;
-    if (nb2 >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
-    hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, 2, UMS, 1>), dim3((unsigned)nb2), dim3(256), 0, s, M_dim, n_tiles,
-                       tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
+    if (2 * nb2 >= (1ll << 31)) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+    if constexpr (std::is_same<Real, float>::value) {
+      if (!hp || !hp->Vh || !hp->tinv || !hp->bound) throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
+      hipLaunchKernelGGL((k_hadamard_split3<UMS>), dim3((unsigned)(2 * nb2)), dim3(256), 0, s, M_dim, n_tiles, tile_t0,
+                         tile_u0, usrc, uidx, gmask, hp->Vh, hp->tinv, hp->bound, X, Y);
+    } else {
+      hipLaunchKernelGGL((k_hadamard_tiled<Real, 3, 3, 2, UMS, 1>), dim3((unsigned)nb2), dim3(256), 0, s, M_dim, n_tiles,
+                         tile_t0, tile_u0, usrc, uidx, gmask, offsets, Vp, X, Y);
+    }
   } else {
     throw_device_error(hipErrorInvalidValue, __FILE__, __LINE__);
   }

@@ -6,12 +6,12 @@
 
 namespace pvfmm_amd {
 
-// Operands of the FP32 scalar Hadamard (fp16 matrix cores; see hadamard_tiled).
+// Operands of the FP32 Hadamard kernels on the fp16 matrix cores (see hadamard_tiled).
 struct HadamardParts {
   const uint16_t* Vh = nullptr;  // transfer blocks as two fp16 parts, scaled by tau_f
   const float* tinv = nullptr;   // 1 / tau_f per frequency (powers of two)
-  const float* bound = nullptr;  // spectrum bound per source slot (fft_up's `bound`)
-  float* run_scale = nullptr;    // scratch: one scale per run of tiles
+  const float* bound = nullptr;  // spectrum bound per source slot and component (fft_up's `bound`)
+  float* run_scale = nullptr;    // scratch: one scale per run of tiles (scalar kernels)
   int run_scale_cap = 0;         // its length
 };
 
@@ -82,9 +82,9 @@ struct DeviceKernels {
   // X layout [slot][f][8*e0] complex.  lattice[3p..3p+2] is the m^3 lattice cell of surface
   // point p; lat_inv[(z*m + y)*m + x] is its inverse (-1 inside the cube).
   // vg, vi: X holds vector groups of vg density vectors, [slot][f][vector vg][8*e0 complex];
-  // this call writes vector vi of the group (vg = 1, vi = 0: one vector).  bound (e0 = 1,
-  // vg = 1 only; may be null): bound[slot] = |scal[slot]| * sum of |up| over the slot's children,
-  // which no frequency of its spectrum exceeds.
+  // this call writes vector vi of the group (vg = 1, vi = 0: one vector).  bound (vg = 1 only;
+  // may be null): bound[slot * e0 + i] = |scal[slot]| * sum of |up| of component i over the
+  // slot's children, which no frequency of that component's spectrum exceeds.
   static void fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0,
                      int m, const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X, int vg = 1,
                      int vi = 0, float* bound = nullptr);
@@ -97,12 +97,15 @@ struct DeviceKernels {
   // entry) for none, at [k][o][r][g] ([k][n_off][16][4]), or for the FP32 scalar kernel at
   // [k][g][r][o % 2][o / 2] ([k][4][16][2][16]); bit o of gmask[k][g] is set when a
   // target of group g of tile k has a partner at offset o.  tiles_per_block: length of the
-  // run of tiles one block of the FP32 scalar kernel handles.  hp (FP32 scalar kernels only,
-  // else null): hp->Vh the transfer blocks as two fp16 parts of tau_f B, layout [pair j][part
-  // p][f][lane 64][8], lane l holding part p of B_{2j + l/32}[8 ((l/16) % 2) + i][l % 16], i < 8,
-  // for the offsets in the order of uidx; hp->tinv 1 / tau_f; hp->bound the spectrum bounds of
-  // X's slots (fft_up); hp->run_scale room for ceil(n_tiles / tiles_per_block) scales.  The FP32
-  // scalar kernels run on the fp16 matrix cores and need it.
+  // run of tiles one block of the FP32 scalar kernel handles.  hp (FP32 only, else null): the
+  // FP32 kernels run on the fp16 matrix cores and need it.  hp->Vh the transfer blocks as two
+  // fp16 parts of tau_f B for the offsets in the order of uidx, taken in pairs (2j, 2j+1):
+  // scalar kernels [pair j][part p][f][lane 64][8], lane l holding part p of
+  // B_{2j + l/32}[8 ((l/16) % 2) + i][l % 16], i < 8; Stokes [f][pair j][chunk c][tile nb]
+  // [part p][lane 64][8], lane l holding part p of B_{2j + a}[8 b + i][16 nb + l % 16] with
+  // q = 4c + l/16, a = q / 6, b = q % 6; hp->tinv 1 / tau_f; hp->bound the spectrum bounds of
+  // X's slots and components (fft_up); hp->run_scale (scalar kernels) room for
+  // ceil(n_tiles / tiles_per_block) scales.
   // They stage into an LDS pool of hadamard_pool_max() entries kept across a block's run of
   // tiles: usrc entries are (source slot << 9 | pool slot), uidx holds pool slots (the pool
   // size for none), and tile k's partners new since tile k-1 are dent[tile_d0[k] ..
