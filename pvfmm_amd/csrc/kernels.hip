@@ -2496,9 +2496,9 @@ void DeviceKernels<Real>::fft_up(hipStream_t s, int n_slots, const int32_t* firs
 
 template <typename Real>
 int DeviceKernels<Real>::hadamard_union_max(int e0) {
-  // scalar kernels: FP32 250, FP64 256 partners per tile; Stokes (2 frequencies per block): 160,
-  // which fits two FP32 blocks or one FP64 block in a CU's 160 KiB of LDS (fuller tiles pay for
-  // the per-tile transfer-block loads over more targets)
+  // scalar kernels: FP32 250, FP64 256 partners per tile; Stokes: 160, which fits three FP32
+  // blocks (one frequency each, fp16 parts) or one FP64 block (2 frequencies) in a CU's 160 KiB
+  // of LDS (fuller tiles pay for the per-tile transfer-block loads over more targets)
   if (e0 != 1) return 160;
   return sizeof(Real) == 4 ? 250 : 256;
 }
