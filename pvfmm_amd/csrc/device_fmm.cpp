@@ -386,8 +386,9 @@ class DevicePlan final : public DevicePlanBase {
     if (n_t > t0.back()) close_tile(n_t);
     const int n_tiles = (int)t0.size() - 1;
     // a missing partner points at the staging area's zero entry (index umax, or the pool size
-    // for the FP32 scalar kernel).  The FP32 scalar kernel reads its table as [k][group][row]
-    // [o % 2][o / 2 (padded to 16)], the others as [k][o][row][group].
+    // for the FP32 scalar kernel).  The FP32 scalar kernel reads its table as [k][group]
+    // [o / 16][o % 2][row][(o / 2) % 8] (pairs o / 2 padded to 16), the others as
+    // [k][o][row][group].
     const size_t per_tile = pair_layout ? (size_t)4 * 16 * 2 * 16 : (size_t)64 * n_off_;
     if (pair_layout && n_off_ > 32) throw Error(kErrInternal, "V-list offsets exceed the pair table");
     if (pair_layout && (pool + 1 > 511 || (int64_t)n_s >= (int64_t(1) << 22)))
@@ -403,7 +404,9 @@ class DevicePlan final : public DevicePlanBase {
           const int32_t sl = v_partner[(size_t)t * n_off_ + o];
           const int tt = t - t0[k];  // target tt = 16 * group + row of the MFMA tiles
           if (sl >= 0) {
-            const size_t at = pair_layout ? (size_t)k * per_tile + ((size_t)tt * 2 + (o & 1)) * 16 + (o >> 1)
+            const size_t at = pair_layout ? (size_t)k * per_tile +
+                                                (((((size_t)(tt >> 4) * 2 + (o >> 4)) * 2 + (o & 1)) * 16 + (tt & 15)) * 8 +
+                                                 ((o >> 1) & 7))
                                           : (((size_t)k * n_off_ + o) * 16 + (tt & 15)) * 4 + (tt >> 4);
             uidx[at] = (uint16_t)pos[sl];
             gmask[(size_t)k * 4 + (tt >> 4)] |= 1u << o;

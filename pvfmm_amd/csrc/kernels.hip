@@ -1824,7 +1824,7 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
   constexpr int RW = 16;                  // reals per staged entry and frequency
   constexpr int EST = 2 * RW / 2 + 4;     // words per staged entry: 2 parts x 16 fp16, padded
   constexpr int XBUF = (POOL + 2) * EST;  // entry POOL stays zero, POOL + 1 takes unused lanes
-  constexpr int UT = 4 * 16 * 2 * 16;     // partner table of a tile: [group][row][a][pair (padded to 16)]
+  constexpr int UT = 4 * 16 * 2 * 16;     // partner table of a tile: [group][pair / 8][a][row][pair % 8] (pairs padded to 16)
   constexpr int UW = UT / 4;              // as 8-byte words
   constexpr int UWP = UW / 256;
   constexpr int EPP = 256 / (RW / 4);     // staged entries per pass (a thread stages 4 reals)
@@ -1945,10 +1945,11 @@ __global__ __launch_bounds__(256, 2) void k_hadamard_split(int M_dim, int n_tile
     const uint32_t wm = __builtin_amdgcn_readfirstlane(m_g[4 * (k - k0) + w]);
     const uint32_t* xb = Xl;
     // staged positions of the partners of target (w, row) at offsets 2j + pa, j < 13, read at
-    // once so that the operand reads of a pair wait for no other LDS read
+    // once so that the operand reads of a pair wait for no other LDS read; the 16 rows of a
+    // read are consecutive 16-byte words (one per LDS bank group)
     u32x4 pw[2];
 #pragma unroll
-    for (int h = 0; h < 2; h++) pw[h] = *reinterpret_cast<const u32x4*>(&ul[buf][((w * 16 + row) * 2 + pa) * 16 + 8 * h]);
+    for (int h = 0; h < 2; h++) pw[h] = *reinterpret_cast<const u32x4*>(&ul[buf][(((w * 2 + h) * 2 + pa) * 16 + row) * 8]);
     auto pidx = [&](int j) __attribute__((always_inline)) { return (int)((pw[j >> 3][(j >> 1) & 3] >> (16 * (j & 1))) & 0xFFFFu); };
     auto aop = [&](int pos, int p) __attribute__((always_inline)) {
       return *reinterpret_cast<const halfx8*>(&xb[pos * EST + 8 * p + lane_off]);

@@ -95,7 +95,7 @@ struct DeviceKernels {
   // hadamard_union_max(e0)) are staged in LDS; uidx holds, for offset o and tile target
   // 16 * g + r, the partner's position in that list, or hadamard_union_max(e0) (the zero
   // entry) for none, at [k][o][r][g] ([k][n_off][16][4]), or for the FP32 scalar kernel at
-  // [k][g][r][o % 2][o / 2] ([k][4][16][2][16]); bit o of gmask[k][g] is set when a
+  // [k][g][o / 16][o % 2][r][(o / 2) % 8] ([k][4][2][2][16][8]); bit o of gmask[k][g] is set when a
   // target of group g of tile k has a partner at offset o.  tiles_per_block: length of the
   // run of tiles one block of the FP32 scalar kernel handles.  hp (FP32 only, else null): the
   // FP32 kernels run on the fp16 matrix cores and need it.  hp->Vh the transfer blocks as two
