@@ -587,7 +587,13 @@ class DevicePlan final : public DevicePlanBase {
       //   B[2s][2t] = Re M[t][s], B[2s+1][2t] = -Im M[t][s], B[2s][2t+1] = Im M[t][s],
       //   B[2s+1][2t+1] = Re M[t][s],
       // stored per MFMA lane: [R][f][kb][nb][lane][q] = B[16kb + 4(lane/16) + q][16nb + lane%16].
+      // Spectrum rows are component-major (the DFTs write and read them so): row position k of
+      // an input spectrum holds component k / 16 of child (k % 16) / 2, which is complex source
+      // index s_of(k) of M (child-major: s = child * E0 + component), and output position n is
+      // complex target index t_of(n).
       const int E0 = e0_, E1 = e1_, NS = 8 * E0, NR = 8 * E1;
+      auto s_of = [E0](int k) { return ((k & 15) >> 1) * E0 + (k >> 4); };
+      auto t_of = [E1](int n) { return ((n & 15) >> 1) * E1 + (n >> 4); };
       // the FP32 kernels use the fp16 (and, scalar, bf16) parts below instead
       const bool bf16_parts = std::is_same<Real, float>::value && E0 == 1 && E1 == 1;
       const bool stokes_parts = std::is_same<Real, float>::value && E0 == 3 && E1 == 3;
@@ -601,7 +607,7 @@ class DevicePlan final : public DevicePlanBase {
             for (int l = 0; l < 64; l++)
               for (int q = 0; q < 4; q++) {
                 const int k = 16 * kb + 4 * (l >> 4) + q, n = 16 * nb + (l & 15);
-                const std::complex<double> v = Mb[(size_t)(n >> 1) * NS + (k >> 1)];
+                const std::complex<double> v = Mb[(size_t)t_of(n) * NS + s_of(k)];
                 double val = ((k & 1) == 0) ? ((n & 1) == 0 ? v.real() : v.imag()) : ((n & 1) == 0 ? -v.imag() : v.real());
                 Vp[((((size_t)rf * E0 + kb) * E1 + nb) * 64 + l) * 4 + q] = (Real)val;
               }
@@ -690,7 +696,7 @@ class DevicePlan final : public DevicePlanBase {
         std::vector<uint16_t> Vh((size_t)M_dim_ * per_f);
         std::vector<float> tinv(M_dim_);
         auto entry = [&](int R, int f, int k, int n) {
-          const std::complex<double> v = O_.v1[((size_t)R * M_dim_ + f) * NR * NS + (size_t)(n >> 1) * NS + (k >> 1)];
+          const std::complex<double> v = O_.v1[((size_t)R * M_dim_ + f) * NR * NS + (size_t)t_of(n) * NS + s_of(k)];
           return ((k & 1) == 0) ? ((n & 1) == 0 ? v.real() : v.imag()) : ((n & 1) == 0 ? -v.imag() : v.real());
         };
         #pragma omp parallel for schedule(static)

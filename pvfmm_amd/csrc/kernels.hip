@@ -918,8 +918,8 @@ __device__ void init_twiddles(Real* twr, Real* twi, int n1, double sign) {
 // Forward DFT of CB children of one source parent per block (grid: n_slots * e0 * (8/CB)
 // blocks of 512 threads).  The m^3 lattice cube of each child (surface values only) sits in
 // LDS; the transform runs one x-frequency slice at a time: x (real->complex, m taps), y and z
-// (m taps each, the pruned inputs).  Spectra are written as X[slot][f][c*e0+i] complex so
-// that the children of one frequency form one contiguous row.
+// (m taps each, the pruned inputs).  Spectra are written as X[slot][f][8i+c] complex so
+// that the children of one frequency form one contiguous row, component-major.
 template <typename Real>
 __global__ __launch_bounds__(512) void k_fft_up(const int32_t* __restrict__ first_child, const Real* __restrict__ up, int K0, int e0,
                                                 int m, int CB, const int32_t* __restrict__ lattice, const Real* __restrict__ scal,
@@ -949,7 +949,7 @@ __global__ __launch_bounds__(512) void k_fft_up(const int32_t* __restrict__ firs
     cube[c * m3 + (z * m + y) * m + x] = up[(int64_t)(fc + c) * K0 + p * e0 + i] * sc;
   }
   __syncthreads();
-  Real* Xs = X + (((int64_t)slot * M_dim * vg + vi) * 8 * e0 + c0 * e0 + i) * 2;
+  Real* Xs = X + (((int64_t)slot * M_dim * vg + vi) * 8 * e0 + 8 * i + c0) * 2;
   for (int kx = 0; kx < nh; kx++) {
     for (int o = tid; o < CB * m2; o += nt) {  // x: real -> complex
       const Real* line = &cube[(o / m2) * m3 + (o % m2) * m];
@@ -985,7 +985,7 @@ __global__ __launch_bounds__(512) void k_fft_up(const int32_t* __restrict__ firs
         ai += vr[z * n1] * twi[e] + vi[z * n1] * twr[e];
       }
       const int64_t f = kx + nh * (ky + n1 * kz);
-      Real* d = Xs + (f * vg * 8 * e0 + c * e0) * 2;
+      Real* d = Xs + (f * vg * 8 * e0 + c) * 2;
       d[0] = ar;
       d[1] = ai;
     }
@@ -994,7 +994,8 @@ __global__ __launch_bounds__(512) void k_fft_up(const int32_t* __restrict__ firs
 }
 
 // Inverse DFT of CB children of one target parent per block (grid: n_slots * e1 * (8/CB)
-// blocks of 512 threads), evaluated only at the check-surface lattice points; per x-frequency
+// blocks of 512 threads; Y rows component-major, Y[slot][f][8j+c] complex), evaluated only at
+// the check-surface lattice points; per x-frequency
 // slice the n1 x n1 spectrum plane of each child is read once (contiguous child rows), then z,
 // y (n1 taps, m outputs) and the c2r x-sum is accumulated in registers.
 template <typename Real>
@@ -1023,13 +1024,13 @@ __global__ __launch_bounds__(512) void k_ifft_down(const int32_t* __restrict__ f
   Real acc[MAXQ];
 #pragma unroll
   for (int q = 0; q < MAXQ; q++) acc[q] = 0;
-  const Real* Ys = Y + (((int64_t)slot * M_dim * vg + vi) * 8 * e1 + c0 * e1 + j) * 2;
+  const Real* Ys = Y + (((int64_t)slot * M_dim * vg + vi) * 8 * e1 + 8 * j + c0) * 2;
   __syncthreads();
   for (int kx = 0; kx < nh; kx++) {
     for (int o = tid; o < CB * nn; o += nt) {  // children fastest: contiguous rows of Y
       const int c = o % CB, q = o / CB;         // q = ky + n1*kz
       const int64_t f = kx + (int64_t)nh * q;
-      const Real* src = Ys + (f * vg * 8 * e1 + c * e1) * 2;
+      const Real* src = Ys + (f * vg * 8 * e1 + c) * 2;
       yr[c * nn + q] = src[0];
       yi[c * nn + q] = src[1];
     }
@@ -1241,7 +1242,8 @@ struct DftShape {
 // x stage (real -> complex, kx < m+1) on thread-owned lattice lines into LDS; then one task
 // per (c, kx, ky < m) runs the y stage for ky and ky+m (w^{-(ky+m)y} = (-1)^y w^{-ky y}) and the
 // z stage of both (X[kz] = E + O, X[kz+m] = E - O over even/odd z), writing the spectra as
-// X[slot][f][c*e0+i] complex, f = kx + (m+1)(ky + 2m kz).
+// X[slot][f][8i+c] complex (component-major, so each component's block writes its own 64-byte
+// segment of a row), f = kx + (m+1)(ky + 2m kz).
 // NTS: non-temporal stores of X, which is read back only by the Hadamard, long after its lines
 // would have left the caches (1.20 -> 1.04 ms for the 2M adaptive workload).  Only where each
 // block writes whole lines: with several components (or vectors in one group) the blocks of one
@@ -1294,7 +1296,7 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
     }
   }
   __syncthreads();
-  Real* Xs = X + (((int64_t)slot * MD * vg + vi) * 8 * e0 + i) * 2;
+  Real* Xs = X + (((int64_t)slot * MD * vg + vi) * 8 * e0 + 8 * i) * 2;
   for (int k0 = 0; k0 < NH; k0 += KC) {
     const int kn = (NH - k0 < KC) ? NH - k0 : KC;
 #pragma unroll
@@ -1338,8 +1340,8 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
           // out of the unrolled loop (they would not fit in registers for FP64)
           Real* xb = Xs;
           asm volatile("" : "+s"(xb));
-          Real* d0 = xb + (f0 * vg * 8 * e0 + c * e0) * 2;
-          Real* d1 = xb + (f1 * vg * 8 * e0 + c * e0) * 2;
+          Real* d0 = xb + (f0 * vg * 8 * e0 + c) * 2;
+          Real* d1 = xb + (f1 * vg * 8 * e0 + c) * 2;
           if constexpr (NTS) {
             __builtin_nontemporal_store(er + orr, &d0[0]);
             __builtin_nontemporal_store(ei + oi, &d0[1]);
@@ -1398,8 +1400,9 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), 4) void k_fft_up_m(const i
 }
 
 // Inverse DFT of the 8 children of one target parent, component j (grid n_slots * e1),
-// evaluated at the check-surface lattice points only.  z stage per (c, kx, ky) task straight
-// from Y (output z < m needs P = Y[a] + Y[a+m] for even z, Q = Y[a] - Y[a+m] for odd z) into
+// evaluated at the check-surface lattice points only; Y rows are component-major
+// (Y[slot][f][8j+c] complex), so a component's 8 children are one 64-byte segment.  z stage
+// per (c, kx, ky) task straight from Y (output z < m needs P = Y[a] + Y[a+m] for even z, Q = Y[a] - Y[a+m] for odd z) into
 // LDS; then each thread runs the y stage
 // of its own lines (c, y, z) and accumulates the c2r x sum in registers; the lattice values
 // are added into dn at the end.
@@ -1433,7 +1436,7 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), (sizeof(Real) == 8 ? 1 : 4
   for (int q = 0; q < LPT; q++)
 #pragma unroll
     for (int x = 0; x < M; x++) acc[q][x] = 0;
-  const Real* Ys = Y + (((int64_t)slot * MD * vg + vi) * 8 * e1 + j) * 2;
+  const Real* Ys = Y + (((int64_t)slot * MD * vg + vi) * 8 * e1 + 8 * j) * 2;
   for (int k0 = 0; k0 < NH; k0 += KC) {
     const int kn = (NH - k0 < KC) ? NH - k0 : KC;
     const int ntask = 8 * N1 * kn;
@@ -1442,8 +1445,8 @@ __global__ __launch_bounds__((DftShape<Real, M>::NT), (sizeof(Real) == 8 ? 1 : 4
       Real pr_[M], pi_[M], qr_[M], qi_[M];
 #pragma unroll
       for (int a = 0; a < M; a++) {
-        const Real* s0 = Ys + ((kx + (int64_t)NH * (ky + N1 * a)) * vg * 8 * e1 + c * e1) * 2;
-        const Real* s1 = Ys + ((kx + (int64_t)NH * (ky + N1 * (a + M))) * vg * 8 * e1 + c * e1) * 2;
+        const Real* s0 = Ys + ((kx + (int64_t)NH * (ky + N1 * a)) * vg * 8 * e1 + c) * 2;
+        const Real* s1 = Ys + ((kx + (int64_t)NH * (ky + N1 * (a + M))) * vg * 8 * e1 + c) * 2;
         const Real ur = s0[0], ui = s0[1], vr = s1[0], vi = s1[1];
         pr_[a] = ur + vr; pi_[a] = ui + vi;
         qr_[a] = ur - vr; qi_[a] = ui - vi;

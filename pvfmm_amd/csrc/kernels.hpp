@@ -79,7 +79,7 @@ struct DeviceKernels {
                         int nv = 1, int64_t avs = 0, int64_t cvs = 0);
 
   // V-list: forward DFT of the 8 children's equivalent densities of each source parent.
-  // X layout [slot][f][8*e0] complex.  lattice[3p..3p+2] is the m^3 lattice cell of surface
+  // X layout [slot][f][8*e0] complex, component-major within a row ([slot][f][i][child]).  lattice[3p..3p+2] is the m^3 lattice cell of surface
   // point p; lat_inv[(z*m + y)*m + x] is its inverse (-1 inside the cube).
   // vg, vi: X holds vector groups of vg density vectors, [slot][f][vector vg][8*e0 complex];
   // this call writes vector vi of the group (vg = 1, vi = 0: one vector).  bound (vg = 1 only;
@@ -88,7 +88,8 @@ struct DeviceKernels {
   static void fft_up(hipStream_t s, int n_slots, const int32_t* first_child, const Real* up, int K0, int e0,
                      int m, const int32_t* lattice, const int32_t* lat_inv, const Real* scal, Real* X, int vg = 1,
                      int vi = 0, float* bound = nullptr);
-  // Y[slot][f][8*e1] = sum_R V[R][f] X[partner[slot][R]][f] on MFMA; Vp is the transfer block
+  // Y[slot][f][8*e1] = sum_R V[R][f] X[partner[slot][R]][f] on MFMA (rows component-major,
+  // as fft_up writes X and ifft_down reads Y); Vp is the transfer block
   // in real form, pre-permuted per lane (layout [R][f][kb][nb][lane 64][4]).  The target slots
   // form tiles of at most 64 consecutive slots [tile_t0[k], tile_t0[k+1])
   // whose distinct partner source slots usrc[tile_u0[k] .. tile_u0[k+1]) (at most
