@@ -296,6 +296,7 @@ CASES = [  # kernel, precision, m, bndry, fp64 bar vs oracle, mixed bar
     ("laplace_poten", "fp64", 6, "px"),
     ("laplace_poten", "fp64", 6, "pxy"),
     ("stokes_vel", "fp64", 4, "pxy"),
+    ("stokes_vel", "mixed", 6, "px"),
 ]
 
 
